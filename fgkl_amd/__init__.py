@@ -1,0 +1,321 @@
+"""fgkl_amd: PairHMM and partially determined PairHMM likelihoods on one AMD MI355X (gfx950).
+
+The Python surface mirrors the reference's ``fgkl_pairhmm`` crate API
+(crates/pairhmm/src/lib.rs:30-344, pd.rs:13-110): ``Precision``, ``Config``, ``ReadRef``,
+``PairHmm.compute_log10_likelihoods``, ``PdHaplotype``, ``PdPairHmm``, ``fallback_pairs`` and
+the ``Error`` variants with the same messages.  All computation runs in the HIP engine
+(``libfgkl.so``) through its C-ABI; there is no CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, last_error
+
+__all__ = [
+    "Precision", "Config", "ReadRef", "PdHaplotype", "PairHmm", "PdPairHmm",
+    "Error", "EmptyRead", "MismatchedReadArrays", "EmptyHaplotype",
+    "MismatchedHaplotypeArrays", "OutputLength", "BackendUnavailable", "InvalidArgument",
+    "NativeError", "SNP", "DEL_START", "DEL_END", "ALT_A", "ALT_C", "ALT_G", "ALT_T",
+    "compute_native", "pd_compute_native", "Batch", "backend_name", "device_count",
+]
+
+# pd.rs:15-27
+SNP, DEL_START, DEL_END, ALT_A, ALT_C, ALT_G, ALT_T = 1, 2, 4, 8, 16, 32, 64
+
+FGKL_OK, FGKL_INVALID_ARGUMENT, FGKL_INTERNAL_ERROR = 0, 1, 2
+MODE_FP32_RESCUE, MODE_FP64, MODE_FP64_PER_REGION, MODE_FP32_NO_RESCUE = 0, 1, 2, 3
+
+
+class Precision(enum.Enum):
+    """lib.rs:72-105"""
+    FLOAT = "float"
+    DOUBLE = "double"
+
+
+@dataclass
+class Config:
+    """lib.rs:213-238: the backend is always the HIP engine; suffix sharing is not offered."""
+    precision: Precision = Precision.FLOAT
+    double_fallback: bool = True
+
+
+# ---- errors (lib.rs:305-344) -----------------------------------------------------------------
+class Error(Exception):
+    """A refused computation; ``str()`` is the reference's Display text."""
+
+
+class EmptyRead(Error):
+    def __init__(self, index: int):
+        self.index = index
+        super().__init__(f"read {index} has no bases")
+
+
+class MismatchedReadArrays(Error):
+    def __init__(self, index: int):
+        self.index = index
+        super().__init__(f"read {index}: bases, qualities and penalties differ in length")
+
+
+class EmptyHaplotype(Error):
+    def __init__(self, index: int):
+        self.index = index
+        super().__init__(f"haplotype {index} has no bases")
+
+
+class MismatchedHaplotypeArrays(Error):
+    def __init__(self, index: int):
+        self.index = index
+        super().__init__(f"haplotype {index}: bases and flags differ in length")
+
+
+class OutputLength(Error):
+    def __init__(self, expected: int, actual: int):
+        self.expected, self.actual = expected, actual
+        super().__init__(f"output has {actual} elements but reads x haplotypes is {expected}")
+
+
+class BackendUnavailable(Error):
+    pass
+
+
+class InvalidArgument(Error):
+    """FGKL_INVALID_ARGUMENT from the C-ABI: the JNI shim throws IllegalArgumentException."""
+
+
+class NativeError(RuntimeError):
+    """FGKL_INTERNAL_ERROR from the C-ABI: the JNI shim throws RuntimeException."""
+
+
+def _check(rc: int) -> None:
+    if rc == FGKL_OK:
+        return
+    msg = last_error()
+    if rc == FGKL_INVALID_ARGUMENT:
+        raise InvalidArgument(msg)
+    raise NativeError(msg)
+
+
+def backend_name() -> str:
+    return lib().fgkl_backend_name().decode()
+
+
+def device_count() -> int:
+    return int(lib().fgkl_device_count())
+
+
+# ---- array helpers -----------------------------------------------------------------------------
+def _u8(a) -> np.ndarray:
+    if isinstance(a, (bytes, bytearray)):
+        return np.frombuffer(bytes(a), dtype=np.uint8)
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class ReadRef:
+    """lib.rs:30-68: one read's bases and per-base penalties, all of one length."""
+    bases: bytes
+    quals: Sequence[int]
+    ins_gop: Sequence[int]
+    del_gop: Sequence[int]
+    gcp: Sequence[int]
+
+    def __len__(self) -> int:
+        return len(self.bases)
+
+    def validate(self, index: int) -> None:
+        n = len(self.bases)
+        if n == 0:
+            raise EmptyRead(index)
+        if any(len(x) != n for x in (self.quals, self.ins_gop, self.del_gop, self.gcp)):
+            raise MismatchedReadArrays(index)
+
+
+@dataclass
+class PdHaplotype:
+    """pd.rs:29-37"""
+    bases: bytes
+    flags: Sequence[int]
+
+
+def _concat_reads(reads: Sequence[ReadRef]):
+    offs = np.zeros(len(reads) + 1, dtype=np.int64)
+    for i, r in enumerate(reads):
+        offs[i + 1] = offs[i] + len(r.bases)
+    cat = [np.concatenate([_u8(getattr(r, f)) for r in reads]) if reads else np.zeros(0, np.uint8)
+           for f in ("bases", "quals", "ins_gop", "del_gop", "gcp")]
+    return offs, cat
+
+
+def _concat(seqs) -> tuple[np.ndarray, np.ndarray]:
+    offs = np.zeros(len(seqs) + 1, dtype=np.int64)
+    for i, s in enumerate(seqs):
+        offs[i + 1] = offs[i] + len(s)
+    data = np.concatenate([_u8(s) for s in seqs]) if seqs else np.zeros(0, np.uint8)
+    return offs, data
+
+
+class _Engine:
+    _pd = False
+
+    def __init__(self, config: Config | None = None):
+        self.config = config or Config()
+        h = lib().fgkl_pairhmm_new(int(self._pd), int(self.config.precision is Precision.DOUBLE),
+                                   int(self.config.double_fallback))
+        if not h:
+            raise BackendUnavailable(last_error())
+        self._h = C.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().fgkl_pairhmm_free(h)
+            self._h = None
+
+    def backend(self) -> str:
+        return backend_name()
+
+    def precision(self) -> Precision:
+        return self.config.precision
+
+    def fallback_pairs(self) -> int:
+        """lib.rs:263-268"""
+        return int(lib().fgkl_pairhmm_fallback_pairs(self._h))
+
+    def _run(self, reads, hap_bases, hap_flags, out):
+        expected = len(reads) * len(hap_bases)
+        if out is None:
+            out = np.zeros(expected, dtype=np.float64)
+        elif len(out) != expected:
+            raise OutputLength(expected, len(out))
+        if expected == 0:
+            return out
+        roff, (b, q, i, d, g) = _concat_reads(reads)
+        hoff, hb = _concat(hap_bases)
+        hf = _concat(hap_flags)[1] if hap_flags is not None else None
+        res = np.zeros(expected, dtype=np.float64)
+        _check(lib().fgkl_pairhmm_compute(
+            self._h, len(reads), _ptr(roff, C.c_int64), _ptr(b, C.c_uint8), _ptr(q, C.c_uint8),
+            _ptr(i, C.c_uint8), _ptr(d, C.c_uint8), _ptr(g, C.c_uint8), len(hap_bases),
+            _ptr(hoff, C.c_int64), _ptr(hb, C.c_uint8),
+            _ptr(hf, C.c_uint8) if hf is not None else None, _ptr(res, C.c_double), expected))
+        out[:] = res
+        return out
+
+
+class PairHmm(_Engine):
+    """lib.rs:240-294: a configured PairHMM on the MI355X engine."""
+
+    def compute_log10_likelihoods(self, reads: Sequence[ReadRef], haplotypes: Sequence[bytes],
+                                  out: np.ndarray | None = None) -> np.ndarray:
+        """``out[r * len(haplotypes) + h]`` is log10 P(read r | haplotype h) (lib.rs:270-293)."""
+        for idx, r in enumerate(reads):
+            r.validate(idx)
+        for idx, h in enumerate(haplotypes):
+            if len(h) == 0:
+                raise EmptyHaplotype(idx)
+        return self._run(reads, list(haplotypes), None, out)
+
+
+class PdPairHmm(_Engine):
+    """pd.rs:39-96: the partially determined PairHMM on the MI355X engine."""
+    _pd = True
+
+    def compute_log10_likelihoods(self, reads: Sequence[ReadRef],
+                                  haplotypes: Sequence[PdHaplotype],
+                                  out: np.ndarray | None = None) -> np.ndarray:
+        for idx, r in enumerate(reads):
+            r.validate(idx)
+        for idx, h in enumerate(haplotypes):
+            if len(h.bases) == 0:
+                raise EmptyHaplotype(idx)
+            if len(h.flags) != len(h.bases):
+                raise MismatchedHaplotypeArrays(idx)
+        return self._run(reads, [h.bases for h in haplotypes], [h.flags for h in haplotypes], out)
+
+
+# ---- the JNI-shaped entry points (crates/jni/src/lib.rs:159-254) ------------------------------
+def _native(pd, bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_flags, hap_offsets,
+            double_precision, likelihoods):
+    arrs = [_u8(x) for x in (bases, quals, ins, dels, gcp)]
+    ro = np.ascontiguousarray(read_offsets, dtype=np.int32)
+    ho = np.ascontiguousarray(hap_offsets, dtype=np.int32)
+    hb = _u8(hap_bases)
+    args = []
+    for a in arrs:
+        args += [_ptr(a, C.c_uint8), len(a)]
+    args += [_ptr(ro, C.c_int32), len(ro), _ptr(hb, C.c_uint8), len(hb)]
+    if pd:
+        hf = _u8(hap_flags)
+        args += [_ptr(hf, C.c_uint8), len(hf)]
+    args += [_ptr(ho, C.c_int32), len(ho), int(bool(double_precision)),
+             likelihoods.ctypes.data_as(C.POINTER(C.c_double)), len(likelihoods)]
+    fn = lib().fgkl_pdhmm_compute_native if pd else lib().fgkl_pairhmm_compute_native
+    _check(fn(*args))
+
+
+def compute_native(bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_offsets,
+                   double_precision, likelihoods: np.ndarray) -> None:
+    """FgklPairHmm.computeNative (crates/jni/src/lib.rs:159-195)."""
+    _native(False, bases, quals, ins, dels, gcp, read_offsets, hap_bases, None, hap_offsets,
+            double_precision, likelihoods)
+
+
+def pd_compute_native(bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_flags,
+                      hap_offsets, double_precision, likelihoods: np.ndarray) -> None:
+    """FgklPdHmm.computeNative (crates/jni/src/lib.rs:207-254)."""
+    _native(True, bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_flags, hap_offsets,
+            double_precision, likelihoods)
+
+
+# ---- batched engine --------------------------------------------------------------------------
+class Batch:
+    """Many regions resident in device memory (include/fgkl.h, fgkl_batch_*)."""
+
+    def __init__(self, data: "BatchData", pd: bool | None = None):
+        self.data = data
+        self.pd = data.pd if pd is None else pd
+        h = C.c_void_p()
+        d = data
+        _check(lib().fgkl_batch_create(
+            C.byref(h), int(self.pd), d.n_regions, _ptr(d.region_reads, C.c_int64),
+            _ptr(d.region_haps, C.c_int64), _ptr(d.read_offsets, C.c_int64),
+            _ptr(d.bases, C.c_uint8), _ptr(d.quals, C.c_uint8), _ptr(d.ins, C.c_uint8),
+            _ptr(d.dels, C.c_uint8), _ptr(d.gcp, C.c_uint8), _ptr(d.hap_offsets, C.c_int64),
+            _ptr(d.hap_bases, C.c_uint8), _ptr(d.hap_flags, C.c_uint8)))
+        self._h = h
+
+    def run(self, mode: int = MODE_FP32_RESCUE, share: bool = True, graph: bool = False):
+        st = _lib.RunStats()
+        _check(lib().fgkl_batch_run(self._h, mode, int(share), int(graph), C.byref(st)))
+        return st
+
+    def results(self) -> np.ndarray:
+        n = int(lib().fgkl_batch_pairs(self._h))
+        out = np.empty(n, dtype=np.float64)
+        _check(lib().fgkl_batch_results(self._h, _ptr(out, C.c_double), n))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().fgkl_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+from .synthetic import BatchData  # noqa: E402  (re-export)

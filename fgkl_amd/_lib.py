@@ -1,0 +1,94 @@
+"""ctypes binding of the engine's C-ABI (include/fgkl.h).
+
+The library is built in-tree (``python -m fgkl_amd.build``) and loaded from
+``fgkl_amd/libfgkl.so``; there is no fallback implementation, so a missing library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfgkl.so"
+
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+u64p = C.POINTER(C.c_uint64)
+f64p = C.POINTER(C.c_double)
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double),
+        ("primary_ms", C.c_double),
+        ("rescue_ms", C.c_double),
+        ("rescued_pairs", C.c_int64),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/fgkl.h declares.
+SIGNATURES = {
+    "fgkl_last_error": (C.c_char_p, []),
+    "fgkl_backend_name": (C.c_char_p, []),
+    "fgkl_device_count": (C.c_int, []),
+    "fgkl_pairhmm_compute_native": (
+        C.c_int,
+        [u8p, C.c_int64] * 5 + [i32p, C.c_int64, u8p, C.c_int64, i32p, C.c_int64, C.c_int,
+                                 f64p, C.c_int64],
+    ),
+    "fgkl_pdhmm_compute_native": (
+        C.c_int,
+        [u8p, C.c_int64] * 5 + [i32p, C.c_int64, u8p, C.c_int64, u8p, C.c_int64, i32p,
+                                 C.c_int64, C.c_int, f64p, C.c_int64],
+    ),
+    "fgkl_pairhmm_new": (C.c_void_p, [C.c_int, C.c_int, C.c_int]),
+    "fgkl_pairhmm_free": (None, [C.c_void_p]),
+    "fgkl_pairhmm_fallback_pairs": (C.c_uint64, [C.c_void_p]),
+    "fgkl_pairhmm_compute": (
+        C.c_int,
+        [C.c_void_p, C.c_int64, i64p, u8p, u8p, u8p, u8p, u8p, C.c_int64, i64p, u8p, u8p,
+         f64p, C.c_int64],
+    ),
+    "fgkl_batch_create": (
+        C.c_int,
+        [C.POINTER(C.c_void_p), C.c_int, C.c_int64, i64p, i64p, i64p, u8p, u8p, u8p, u8p, u8p,
+         i64p, u8p, u8p],
+    ),
+    "fgkl_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)]),
+    "fgkl_batch_results": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
+    "fgkl_batch_pairs": (C.c_int64, [C.c_void_p]),
+    "fgkl_batch_free": (None, [C.c_void_p]),
+    "fgkl_synth_region": (C.c_void_p, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                       C.c_int]),
+    "fgkl_synth_workload": (C.c_void_p, [C.c_int, C.c_uint64, C.c_int64]),
+    "fgkl_synth_sizes": (None, [C.c_void_p, i64p, i64p, i64p, i64p, i64p]),
+    "fgkl_synth_copy": (None, [C.c_void_p, i64p, i64p, i64p, u8p, u8p, u8p, u8p, u8p, i64p,
+                               u8p, u8p]),
+    "fgkl_synth_free": (None, [C.c_void_p]),
+    "fgkl_synth_splitmix": (None, [C.c_uint64, C.c_int64, u64p]),
+    "fgkl_selftest_dpp": (C.c_int, [i32p]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"fgkl engine library {LIB_PATH} is not built; run `python -m fgkl_amd.build`"
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().fgkl_last_error()
+    return msg.decode() if msg else ""

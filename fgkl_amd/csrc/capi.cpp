@@ -1,0 +1,298 @@
+// The C-ABI of include/fgkl.h: argument validation with the reference's error classes and
+// messages, and the calls into the device engine.
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fgkl.h"
+#include "fgkl_engine.hpp"
+#include "fgkl_internal.hpp"
+#include "fgkl_synthetic.hpp"
+
+using namespace fgkl;
+
+struct fgkl_pairhmm {
+  int pd;
+  int double_precision;
+  int double_fallback;
+  std::atomic<uint64_t> fallback_pairs{0};
+};
+
+struct fgkl_batch {
+  std::unique_ptr<DeviceBatch> batch;
+};
+
+struct fgkl_synth {
+  Batch b;
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  set_error(code, msg);
+  return code;
+}
+
+// crates/jni/src/lib.rs:343-363
+int split(const int32_t* off, int64_t n, int64_t total, std::vector<int64_t>& out) {
+  if (n < 1 || !off || off[0] != 0) return fail(kInvalidArgument, "offset table must start at zero");
+  if (off[n - 1] < 0 || int64_t(off[n - 1]) != total)
+    return fail(kInvalidArgument, "offset table must end at the array length");
+  out.resize(size_t(n));
+  for (int64_t i = 0; i < n; ++i) {
+    if (i > 0 && off[i] < off[i - 1]) return fail(kInvalidArgument, "offset table must be non-decreasing");
+    out[size_t(i)] = off[i];
+  }
+  return kOk;
+}
+
+// PairHmm::compute_log10_likelihoods (lib.rs:272-293) / PdPairHmm (pd.rs:69-95) on one region.
+int compute_region(int pd, int double_precision, int double_fallback, int64_t n_reads,
+                   const int64_t* read_off, const uint8_t* bases, const uint8_t* quals,
+                   const uint8_t* ins, const uint8_t* del, const uint8_t* gcp, int64_t n_haps,
+                   const int64_t* hap_off, const uint8_t* hap_bases, const uint8_t* hap_flags,
+                   double* out, int64_t out_len, uint64_t* fallbacks) {
+  if (n_reads < 0 || n_haps < 0) return fail(kInvalidArgument, "negative read or haplotype count");
+  for (int64_t i = 0; i < n_reads; ++i)
+    if (read_off[i + 1] == read_off[i])
+      return fail(kInvalidArgument, "read " + std::to_string(i) + " has no bases");
+  for (int64_t i = 0; i < n_haps; ++i)
+    if (hap_off[i + 1] == hap_off[i])
+      return fail(kInvalidArgument, "haplotype " + std::to_string(i) + " has no bases");
+  const int64_t expected = n_reads * n_haps;
+  if (out_len != expected)
+    return fail(kInvalidArgument, "output has " + std::to_string(out_len) +
+                                      " elements but reads x haplotypes is " +
+                                      std::to_string(expected));
+  if (expected == 0) return kOk;
+  BatchInput in;
+  in.pd = pd != 0;
+  in.n_regions = 1;
+  const int64_t region_read[2] = {0, n_reads}, region_hap[2] = {0, n_haps};
+  in.region_read = region_read;
+  in.region_hap = region_hap;
+  in.read_off = read_off;
+  in.bases = bases; in.quals = quals; in.ins = ins; in.del = del; in.gcp = gcp;
+  in.hap_off = hap_off;
+  in.hap_bases = hap_bases;
+  in.hap_flags = hap_flags;
+  std::unique_ptr<DeviceBatch> b;
+  if (int rc = DeviceBatch::create(in, b)) return rc;
+  const int mode = double_precision ? kModeFp64 : (double_fallback ? kModeFp32Rescue : kModeFp32NoRescue);
+  RunStats st;
+  if (int rc = b->run(mode, 1, 0, &st)) return rc;
+  if (fallbacks) *fallbacks = uint64_t(st.rescued_pairs);
+  return b->results(out, out_len);
+}
+
+int compute_native(int pd, const uint8_t* bases, int64_t bases_len, const uint8_t* quals,
+                   int64_t quals_len, const uint8_t* ins, int64_t ins_len, const uint8_t* del,
+                   int64_t del_len, const uint8_t* gcp, int64_t gcp_len, const int32_t* read_offsets,
+                   int64_t read_offsets_len, const uint8_t* hap_bases, int64_t hap_bases_len,
+                   const uint8_t* hap_flags, int64_t hap_flags_len, const int32_t* hap_offsets,
+                   int64_t hap_offsets_len, int double_precision, double* likelihoods,
+                   int64_t likelihoods_len) {
+  try {
+    // ReadBatch::fetch (crates/jni/src/lib.rs:105-133)
+    if (quals_len != bases_len || ins_len != bases_len || del_len != bases_len || gcp_len != bases_len)
+      return fail(kInvalidArgument, "read bases, qualities and penalties differ in total length");
+    std::vector<int64_t> roff, hoff;
+    if (int rc = split(read_offsets, read_offsets_len, bases_len, roff)) return rc;
+    if (pd && hap_flags_len != hap_bases_len)  // lib.rs:232-237
+      return fail(kInvalidArgument, "haplotype bases and PD flags differ in total length");
+    if (int rc = split(hap_offsets, hap_offsets_len, hap_bases_len, hoff)) return rc;
+    const int64_t n_reads = read_offsets_len - 1, n_haps = hap_offsets_len - 1;
+    return compute_region(pd, double_precision, 1, n_reads, roff.data(), bases, quals, ins, del,
+                          gcp, n_haps, hoff.data(), hap_bases, pd ? hap_flags : nullptr,
+                          likelihoods, likelihoods_len, nullptr);
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fgkl_last_error(void) { return fgkl::last_error(); }
+const char* fgkl_backend_name(void) { return "hip-gfx950"; }
+int fgkl_device_count(void) { return device_available(); }
+
+int fgkl_pairhmm_compute_native(const uint8_t* bases, int64_t bases_len, const uint8_t* quals,
+                                int64_t quals_len, const uint8_t* ins_gop, int64_t ins_gop_len,
+                                const uint8_t* del_gop, int64_t del_gop_len, const uint8_t* gcp,
+                                int64_t gcp_len, const int32_t* read_offsets,
+                                int64_t read_offsets_len, const uint8_t* hap_bases,
+                                int64_t hap_bases_len, const int32_t* hap_offsets,
+                                int64_t hap_offsets_len, int double_precision, double* likelihoods,
+                                int64_t likelihoods_len) {
+  return compute_native(0, bases, bases_len, quals, quals_len, ins_gop, ins_gop_len, del_gop,
+                        del_gop_len, gcp, gcp_len, read_offsets, read_offsets_len, hap_bases,
+                        hap_bases_len, nullptr, 0, hap_offsets, hap_offsets_len, double_precision,
+                        likelihoods, likelihoods_len);
+}
+
+int fgkl_pdhmm_compute_native(const uint8_t* bases, int64_t bases_len, const uint8_t* quals,
+                              int64_t quals_len, const uint8_t* ins_gop, int64_t ins_gop_len,
+                              const uint8_t* del_gop, int64_t del_gop_len, const uint8_t* gcp,
+                              int64_t gcp_len, const int32_t* read_offsets,
+                              int64_t read_offsets_len, const uint8_t* hap_bases,
+                              int64_t hap_bases_len, const uint8_t* hap_flags,
+                              int64_t hap_flags_len, const int32_t* hap_offsets,
+                              int64_t hap_offsets_len, int double_precision, double* likelihoods,
+                              int64_t likelihoods_len) {
+  return compute_native(1, bases, bases_len, quals, quals_len, ins_gop, ins_gop_len, del_gop,
+                        del_gop_len, gcp, gcp_len, read_offsets, read_offsets_len, hap_bases,
+                        hap_bases_len, hap_flags, hap_flags_len, hap_offsets, hap_offsets_len,
+                        double_precision, likelihoods, likelihoods_len);
+}
+
+fgkl_pairhmm* fgkl_pairhmm_new(int pd, int double_precision, int double_fallback) {
+  if (!device_available()) {
+    set_error(kInternal, "no HIP device is available: the fgkl GPU engine needs an MI355X (gfx950)");
+    return nullptr;
+  }
+  auto* h = new fgkl_pairhmm;
+  h->pd = pd;
+  h->double_precision = double_precision;
+  h->double_fallback = double_fallback;
+  return h;
+}
+
+void fgkl_pairhmm_free(fgkl_pairhmm* hmm) { delete hmm; }
+
+uint64_t fgkl_pairhmm_fallback_pairs(const fgkl_pairhmm* hmm) {
+  return hmm ? hmm->fallback_pairs.load(std::memory_order_relaxed) : 0;
+}
+
+int fgkl_pairhmm_compute(fgkl_pairhmm* hmm, int64_t n_reads, const int64_t* read_offsets,
+                         const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
+                         const uint8_t* del_gop, const uint8_t* gcp, int64_t n_haps,
+                         const int64_t* hap_offsets, const uint8_t* hap_bases,
+                         const uint8_t* hap_flags, double* out, int64_t out_len) {
+  if (!hmm) return fail(kInvalidArgument, "null PairHMM handle");
+  if (hmm->pd && n_haps > 0 && !hap_flags)
+    return fail(kInvalidArgument, "a PD PairHMM needs haplotype flags");
+  try {
+    uint64_t fb = 0;
+    const int rc = compute_region(hmm->pd, hmm->double_precision, hmm->double_fallback, n_reads,
+                                  read_offsets, bases, quals, ins_gop, del_gop, gcp, n_haps,
+                                  hap_offsets, hap_bases, hmm->pd ? hap_flags : nullptr, out,
+                                  out_len, &fb);
+    if (rc == kOk) hmm->fallback_pairs.fetch_add(fb, std::memory_order_relaxed);
+    return rc;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int fgkl_batch_create(fgkl_batch** out, int pd, int64_t n_regions, const int64_t* region_reads,
+                      const int64_t* region_haps, const int64_t* read_offsets,
+                      const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
+                      const uint8_t* del_gop, const uint8_t* gcp, const int64_t* hap_offsets,
+                      const uint8_t* hap_bases, const uint8_t* hap_flags) {
+  if (!out) return fail(kInvalidArgument, "null output handle");
+  *out = nullptr;
+  if (pd && !hap_flags) return fail(kInvalidArgument, "a PD batch needs haplotype flags");
+  try {
+    BatchInput in;
+    in.pd = pd != 0;
+    in.n_regions = n_regions;
+    in.region_read = region_reads;
+    in.region_hap = region_haps;
+    in.read_off = read_offsets;
+    in.bases = bases; in.quals = quals; in.ins = ins_gop; in.del = del_gop; in.gcp = gcp;
+    in.hap_off = hap_offsets;
+    in.hap_bases = hap_bases;
+    in.hap_flags = pd ? hap_flags : nullptr;
+    auto* b = new fgkl_batch;
+    if (int rc = DeviceBatch::create(in, b->batch)) {
+      delete b;
+      return rc;
+    }
+    *out = b;
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_graph,
+                   fgkl_run_stats* stats) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  RunStats st;
+  const int rc = batch->batch->run(mode, share_prefixes, use_graph, &st);
+  if (rc == kOk && stats) {
+    stats->total_ms = st.total_ms;
+    stats->primary_ms = st.primary_ms;
+    stats->rescue_ms = st.rescue_ms;
+    stats->rescued_pairs = st.rescued_pairs;
+  }
+  return rc;
+}
+
+int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoods_len) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  return batch->batch->results(likelihoods, likelihoods_len);
+}
+
+int64_t fgkl_batch_pairs(const fgkl_batch* batch) { return batch ? batch->batch->n_pairs() : 0; }
+
+void fgkl_batch_free(fgkl_batch* batch) { delete batch; }
+
+fgkl_synth* fgkl_synth_region(uint64_t seed, int64_t num_reads, int64_t read_len,
+                              int64_t num_haps, int64_t hap_len, int pd) {
+  auto* s = new fgkl_synth;
+  if (pd)
+    pd_region_generate(s->b, seed, num_reads, read_len, num_haps, hap_len);
+  else
+    region_generate(s->b, seed, num_reads, read_len, num_haps, hap_len);
+  s->b.end_batch();
+  return s;
+}
+
+fgkl_synth* fgkl_synth_workload(int workload, uint64_t seed, int64_t n_regions) {
+  auto* s = new fgkl_synth;
+  workload_generate(s->b, workload, seed, n_regions);
+  return s;
+}
+
+void fgkl_synth_sizes(const fgkl_synth* s, int64_t* n_regions, int64_t* n_reads,
+                      int64_t* read_bases, int64_t* n_haps, int64_t* hap_bases) {
+  *n_regions = int64_t(s->b.region_reads.size()) - 1;
+  *n_reads = int64_t(s->b.read_offsets.size()) - 1;
+  *read_bases = int64_t(s->b.bases.size());
+  *n_haps = int64_t(s->b.hap_offsets.size()) - 1;
+  *hap_bases = int64_t(s->b.hap_bases.size());
+}
+
+void fgkl_synth_copy(const fgkl_synth* s, int64_t* region_reads, int64_t* region_haps,
+                     int64_t* read_offsets, uint8_t* bases, uint8_t* quals, uint8_t* ins_gop,
+                     uint8_t* del_gop, uint8_t* gcp, int64_t* hap_offsets, uint8_t* hap_bases,
+                     uint8_t* hap_flags) {
+  const Batch& b = s->b;
+  std::memcpy(region_reads, b.region_reads.data(), b.region_reads.size() * sizeof(int64_t));
+  std::memcpy(region_haps, b.region_haps.data(), b.region_haps.size() * sizeof(int64_t));
+  std::memcpy(read_offsets, b.read_offsets.data(), b.read_offsets.size() * sizeof(int64_t));
+  std::memcpy(bases, b.bases.data(), b.bases.size());
+  std::memcpy(quals, b.quals.data(), b.quals.size());
+  std::memcpy(ins_gop, b.ins.data(), b.ins.size());
+  std::memcpy(del_gop, b.del.data(), b.del.size());
+  std::memcpy(gcp, b.gcp.data(), b.gcp.size());
+  std::memcpy(hap_offsets, b.hap_offsets.data(), b.hap_offsets.size() * sizeof(int64_t));
+  std::memcpy(hap_bases, b.hap_bases.data(), b.hap_bases.size());
+  std::memcpy(hap_flags, b.hap_flags.data(), b.hap_flags.size());
+}
+
+void fgkl_synth_free(fgkl_synth* s) { delete s; }
+
+void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out) {
+  Rng rng(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = rng.next_u64();
+}
+
+int fgkl_selftest_dpp(int32_t* out64) { return dpp_selftest(out64); }
+
+}  // extern "C"

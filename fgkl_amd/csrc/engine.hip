@@ -1,0 +1,942 @@
+// The MI355X (gfx950) PairHMM / PD-HMM engine: device kernels and the batch runner.
+//
+// Kernel layout (one work item per wave64):
+//  * A read of n bases is laid out down the lanes: lane l owns the R consecutive DP rows
+//    R*l+1 .. R*l+R (R = ceil(n/64), 1..8), L = ceil(n/R) lanes are live.  If n is not a multiple
+//    of R the first R*L-n rows of lane 0 are identity rows that reproduce DP row 0 exactly
+//    (prior 0, M->I 0, I->I 0, M->D 0, D->D 1, left value (0, 0, C/m)), so the read's last row is
+//    always the last register of lane L-1.
+//  * The haplotype is swept along anti-diagonals: at step t lane l computes column t-l+1 for its
+//    R rows.  The values of the row above a lane's first row arrive from lane l-1 with a DPP
+//    wave_shr:1 move of the registers that lane computed one step earlier; lane 0 receives DP
+//    row 0 (0, 0, C/m) through the DPP "old" operand.  The haplotype column word travels the
+//    same way, lane 0 reading it from an LDS copy of the haplotype.
+//  * A work item is one read against every sorted haplotype of its region.  A haplotype starts
+//    at the column where it diverges from the previous sorted haplotype, from a snapshot of that
+//    column that an earlier sweep wrote into the wave's HBM scratch slot, rescaled by the ratio
+//    of haplotype lengths (the recurrence is linear in C/m; kernel.rs:610-678 does the same on
+//    the CPU).  Rescue and per-pair work items run a single haplotype from column 0.
+//  * FP32 sweeps scale by 2^120 and flag a pair whose raw sum is below 1e-28 or NaN
+//    (simd.rs:102-114, kernel.rs:897-914): it is appended to a device list that the FP64 kernel
+//    drains in the same stream.  FP64 sweeps scale by 2^1020 and are never rescued.
+//  * The recurrence is not a dense contraction, so MFMA is not used: every cell is 4 multiplies
+//    and 4 FMAs on the vector ALU (kernel.rs:1011-1013 operation order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fgkl_internal.hpp"
+#include "fgkl_engine.hpp"
+
+namespace fgkl {
+
+// ---------------------------------------------------------------------------------------------
+// Numeric constants (simd.rs:102-141).
+template <typename T> struct Num;
+template <> struct Num<float> {
+  static constexpr double kScale = 0x1p120;                       // 2^120
+  static constexpr double kLog10Scale = 120.0 * 0.301029995663981195213738894724493027;
+  static constexpr double kMinAccepted = 1e-28;
+};
+template <> struct Num<double> {
+  static constexpr double kScale = 0x1p1020;                      // 2^1020
+  static constexpr double kLog10Scale = 1020.0 * 0.301029995663981195213738894724493027;
+  static constexpr double kMinAccepted = 0.0;                     // never rescued
+};
+
+template <typename T> struct DevTab {
+  const T* pm;  // 1 - e(q)
+  const T* px;  // e(q) / 3
+  const T* ep;  // e(q)
+  const T* im;  // 1 - e(gcp)
+  const T* mm;  // quantized match-to-match, triangular
+};
+
+struct SweepArgs {
+  const uint8_t *bases, *quals, *ins, *del, *gcp;
+  const int64_t* read_off;
+  const int32_t* read_region;
+  const int32_t* read_local;
+  const int64_t* region_out;
+  const int32_t* region_hap0;
+  const int32_t* region_nhap;
+  const HapPlan* plan;
+  const SnapEntry* snaps;
+  const uint8_t* hap_bytes;
+  const uint32_t* hap_words;
+  const int32_t* items;        // read indices, or (read, sorted hap) pairs in pair mode
+  int32_t n_items;
+  const int32_t* n_items_dev;  // when set, the item count is read from device memory
+  int32_t* counter;            // work-queue head for this launch
+  int32_t pair_mode;
+  int32_t share;
+  void* scratch;
+  int64_t scratch_wave_stride;  // elements of T per wave
+  double* out;
+  int32_t* rescue_pairs;
+  int32_t* rescue_count;
+  int32_t rescue;
+  int32_t lds_words;           // LDS words per wave for the staged haplotype
+};
+
+// ---- wave64 primitives ----------------------------------------------------------------------
+// DPP wave_shr:1 (dpp_ctrl 0x138): lane i receives lane i-1's value; lane 0 keeps `old`.
+__device__ __forceinline__ int shr1i(int old, int src) {
+  return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t shr1(uint32_t old, uint32_t src) {
+  return uint32_t(shr1i(int(old), int(src)));
+}
+__device__ __forceinline__ float shr1(float old, float src) {
+  return __builtin_bit_cast(float, shr1i(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src)));
+}
+__device__ __forceinline__ double shr1(double old, double src) {
+  const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
+  const uint32_t lo = uint32_t(shr1i(int(uint32_t(o)), int(uint32_t(s))));
+  const uint32_t hi = uint32_t(shr1i(int(uint32_t(o >> 32)), int(uint32_t(s >> 32))));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ float readlane(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double readlane(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), l));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), l));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Qualities above 254 are clamped (model.rs:53-64).
+__device__ __forceinline__ int dclamp(uint32_t q) { return q > uint32_t(kMaxQual) ? kMaxQual : int(q); }
+
+__device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
+  switch (b) {
+    case 'A': case 'a': return 1u;
+    case 'C': case 'c': return 2u;
+    case 'G': case 'g': return 4u;
+    case 'T': case 't': return 8u;
+    default: return 0u;
+  }
+}
+
+// Per-lane parameters of its R rows (Workspace::prepare, kernel.rs:485-529).
+template <typename T, int R, bool PD> struct Rows {
+  T pm[R], px[R], mm[R], im[R], mi[R], ii[R], md[R], dd[R];
+  uint32_t rb[R];
+  uint32_t ralt[R];
+  uint32_t kind[R];  // 0: row >= 2, 1: row 1, 2: identity row (or a lane past the read)
+};
+
+template <typename T, int R, bool PD>
+__device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& tab, int64_t o,
+                                          int L, int pad, int lane, Rows<T, R, PD>& rw) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int arow = R * lane + k - pad;
+    if (lane < L && arow >= 0) {
+      const int64_t i = o + arow;
+      const uint32_t b = a.bases[i];
+      const int q = dclamp(a.quals[i]), iq = dclamp(a.ins[i]), dq = dclamp(a.del[i]);
+      const int gq = dclamp(a.gcp[i]);
+      rw.pm[k] = tab.pm[q];
+      rw.px[k] = b == 'N' ? rw.pm[k] : tab.px[q];  // an N read base matches everything
+      rw.mm[k] = tab.mm[((iq >= dq ? iq * (iq + 1) : dq * (dq + 1)) >> 1) + (iq >= dq ? dq : iq)];
+      rw.mi[k] = tab.ep[iq];
+      rw.md[k] = tab.ep[dq];
+      rw.ii[k] = tab.ep[gq];
+      rw.dd[k] = tab.ep[gq];
+      rw.im[k] = tab.im[gq];
+      rw.rb[k] = b;
+      rw.ralt[k] = alt_bit(b);
+      rw.kind[k] = arow == 0 ? 1u : 0u;
+    } else {
+      rw.pm[k] = T(0); rw.px[k] = T(0); rw.mm[k] = T(0); rw.im[k] = T(0);
+      rw.mi[k] = T(0); rw.ii[k] = T(0); rw.md[k] = T(0);
+      rw.dd[k] = lane < L ? T(1) : T(0);
+      rw.rb[k] = 0x100u;
+      rw.ralt[k] = 0u;
+      rw.kind[k] = 2u;
+    }
+  }
+}
+
+// Number of scratch values a lane keeps per snapshot slot.
+template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 7 : 3 * R + 4; }
+
+// One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
+// column start+1+t (zero beyond W).  Returns the lane's raw scaled sum over the read's last row
+// (meaningful in lane L-1).
+template <typename T, int R, bool PD>
+__device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
+                   const T* snap_in, T scale, const SnapEntry* snaps, int nsnap, int start,
+                   T* scratch) {
+  constexpr int NV = snap_values<R, PD>();
+  T lm[R], lx[R], ly[R];
+  T lbm[PD ? R : 1], lbx[PD ? R : 1], lby[PD ? R : 1];
+  T dm, dx, dy, dbm = T(0), dbx = T(0), dby = T(0);
+  T acc;
+  if (snap_in) {
+    const T* s = snap_in + lane;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      lm[k] = s[(3 * k + 0) * 64] * scale;
+      lx[k] = s[(3 * k + 1) * 64] * scale;
+      ly[k] = s[(3 * k + 2) * 64] * scale;
+      if constexpr (PD) {
+        lbm[k] = s[(3 * R + 3 * k + 0) * 64] * scale;
+        lbx[k] = s[(3 * R + 3 * k + 1) * 64] * scale;
+        lby[k] = s[(3 * R + 3 * k + 2) * 64] * scale;
+      }
+    }
+    constexpr int U = PD ? 6 * R : 3 * R;
+    dm = s[(U + 0) * 64] * scale;
+    dx = s[(U + 1) * 64] * scale;
+    dy = s[(U + 2) * 64] * scale;
+    if constexpr (PD) {
+      dbm = s[(U + 3) * 64] * scale;
+      dbx = s[(U + 4) * 64] * scale;
+      dby = s[(U + 5) * 64] * scale;
+    }
+    acc = s[(NV - 1) * 64] * scale;
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      lm[k] = T(0);
+      lx[k] = T(0);
+      ly[k] = (rw.kind[k] == 2u && lane < L) ? init : T(0);
+      if constexpr (PD) { lbm[k] = T(0); lbx[k] = T(0); lby[k] = T(0); }
+    }
+    dm = T(0);
+    dx = T(0);
+    dy = lane == 0 ? init : T(0);
+    acc = T(0);
+  }
+
+  uint32_t w = 0;
+  int s_lo = 0;
+  const int steps = W + L - 1;
+  uint32_t w_next = lw[0];
+  for (int t = 0; t < steps; ++t) {
+    const uint32_t w0 = w_next;
+    w_next = lw[t + 1];
+    w = shr1(w0, w);
+    const T um = shr1(T(0), lm[R - 1]);
+    const T ux = shr1(T(0), lx[R - 1]);
+    const T uy = shr1(init, ly[R - 1]);
+    T ubm = T(0), ubx = T(0), uby = T(0);
+    if constexpr (PD) {
+      ubm = shr1(T(0), lbm[R - 1]);
+      ubx = shr1(T(0), lbx[R - 1]);
+      uby = shr1(T(0), lby[R - 1]);
+    }
+    if (t >= lane) {
+      const uint32_t hb = w & 0xFFu;
+      const bool hN = hb == uint32_t('N');
+      T pdm = dm, pdx = dx, pdy = dy, pum = um, pux = ux;
+      T pdbm = dbm, pdbx = dbx, pdby = dby, pubm = ubm, pubx = ubx;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        bool match = hN || hb == rw.rb[k];
+        if constexpr (PD) match = match || (((w >> 8) & rw.ralt[k]) != 0u);
+        const T prior = match ? rw.pm[k] : rw.px[k];
+        if constexpr (!PD) {
+          // kernel.rs:1011-1013
+          const T M = prior * fma(pdy, rw.im[k], fma(pdx, rw.im[k], pdm * rw.mm[k]));
+          const T X = fma(pux, rw.ii[k], pum * rw.mi[k]);
+          const T Y = fma(ly[k], rw.dd[k], lm[k] * rw.md[k]);
+          pdm = lm[k]; pdx = lx[k]; pdy = ly[k];
+          lm[k] = M; lx[k] = X; ly[k] = Y;
+          pum = M; pux = X;
+        } else {
+          // pd_reference.rs:62-100 with the column's deletion state.
+          uint32_t st, de;
+          if (rw.kind[k] == 2u) { st = kInsideDel; de = 0u; }
+          else {
+            st = rw.kind[k] == 1u ? ((w >> 12) & 3u) : ((w >> 14) & 3u);
+            de = (w >> 16) & 1u;
+          }
+          const bool A = st == kAfterDel, I = st == kInsideDel, N = st == kNormal;
+          const T dme = A ? fmax(pdbm, pdm) : pdm;
+          const T dxe = A ? fmax(pdbx, pdx) : pdx;
+          const T dye = A ? fmax(pdby, pdy) : pdy;
+          const T M = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
+          const T lme = A ? fmax(lbm[k], lm[k]) : lm[k];
+          const T lxe = A ? fmax(lbx[k], lx[k]) : lx[k];
+          const T lye = A ? fmax(lby[k], ly[k]) : ly[k];
+          const T Y = fma(lye, rw.dd[k], lme * rw.md[k]);
+          const T BM = N ? lm[k] : (I ? lbm[k] : lme);
+          const T BX = N ? lx[k] : (I ? lbx[k] : lxe);
+          const T BY = N ? ly[k] : (I ? lby[k] : lye);
+          const T ume = de ? fmax(pubm, pum) : pum;
+          const T uxe = de ? fmax(pubx, pux) : pux;
+          const T X = fma(uxe, rw.ii[k], ume * rw.mi[k]);
+          pdm = lm[k]; pdx = lx[k]; pdy = ly[k];
+          pdbm = lbm[k]; pdbx = lbx[k]; pdby = lby[k];
+          lm[k] = M; lx[k] = X; ly[k] = Y;
+          lbm[k] = BM; lbx[k] = BX; lby[k] = BY;
+          pum = M; pux = X; pubm = BM; pubx = BX;
+        }
+      }
+      dm = um; dx = ux; dy = uy;
+      if constexpr (PD) { dbm = ubm; dbx = ubx; dby = uby; }
+      acc += lm[R - 1] + lx[R - 1];
+    }
+    // Snapshot columns: lane lw reaches column q at step q - start - 1 + lw.
+    for (int s = s_lo; s < nsnap; ++s) {
+      const int t0 = snaps[s].col - start - 1;
+      if (t < t0) break;
+      const int wl = t - t0;
+      if (wl >= L) { s_lo = s + 1; continue; }
+      if (lane == wl) {
+        T* d = scratch + size_t(snaps[s].slot) * NV * 64 + lane;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          d[(3 * k + 0) * 64] = lm[k];
+          d[(3 * k + 1) * 64] = lx[k];
+          d[(3 * k + 2) * 64] = ly[k];
+          if constexpr (PD) {
+            d[(3 * R + 3 * k + 0) * 64] = lbm[k];
+            d[(3 * R + 3 * k + 1) * 64] = lbx[k];
+            d[(3 * R + 3 * k + 2) * 64] = lby[k];
+          }
+        }
+        constexpr int U = PD ? 6 * R : 3 * R;
+        d[(U + 0) * 64] = um;
+        d[(U + 1) * 64] = ux;
+        d[(U + 2) * 64] = uy;
+        if constexpr (PD) {
+          d[(U + 3) * 64] = ubm;
+          d[(U + 4) * 64] = ubx;
+          d[(U + 5) * 64] = uby;
+        }
+        d[(NV - 1) * 64] = acc;
+      }
+    }
+  }
+  return acc;
+}
+
+// Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded.
+template <bool PD>
+__device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int L,
+                                          int lane, uint32_t* lw) {
+  const int W = p.len - start;
+  const int total = W + L + 2;
+  wave_sync_lds();
+  for (int i = lane; i < total; i += 64) {
+    uint32_t v = 0;
+    if (i < W) v = PD ? a.hap_words[p.off + start + i] : uint32_t(a.hap_bytes[p.off + start + i]);
+    lw[i] = v;
+  }
+  wave_sync_lds();
+}
+
+template <typename T, int R, bool PD>
+__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) {
+  extern __shared__ uint32_t lds_all[];
+  const int lane = threadIdx.x & 63;
+  const int wave_in_block = threadIdx.x >> 6;
+  const int64_t wave_global = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave_in_block;
+  uint32_t* lw = lds_all + wave_in_block * a.lds_words;
+  T* scratch = a.scratch ? static_cast<T*>(a.scratch) + wave_global * a.scratch_wave_stride : nullptr;
+  const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
+  constexpr int NV = snap_values<R, PD>();
+
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(a.counter, 1);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= n_items) break;
+
+    int r, k_begin, k_end;
+    if (a.pair_mode) {
+      r = a.items[2 * item];
+      k_begin = a.items[2 * item + 1];
+      k_end = k_begin + 1;
+    } else {
+      r = a.items[item];
+      k_begin = 0;
+      k_end = -1;
+    }
+    const int64_t o = a.read_off[r];
+    const int n = int(a.read_off[r + 1] - o);
+    const int L = (n + R - 1) / R;
+    const int pad = R * L - n;
+    const int g = a.read_region[r];
+    const int hap0 = a.region_hap0[g];
+    const int H = a.region_nhap[g];
+    if (k_end < 0) k_end = H;
+    const int64_t out_base = a.region_out[g] + int64_t(a.read_local[r]) * H;
+
+    Rows<T, R, PD> rw;
+    load_rows<T, R, PD>(a, tab, o, L, pad, lane, rw);
+
+    double prev = 0.0;
+    for (int k = k_begin; k < k_end; ++k) {
+      const HapPlan p = a.plan[hap0 + k];
+      double res;
+      if (!a.pair_mode && p.dup) {
+        res = prev;  // kernel.rs:237-246
+      } else {
+        const bool shared = a.share && !a.pair_mode;
+        const int start = shared ? p.lcp : 0;
+        const int nsnap = shared ? p.nsnap : 0;
+        const T* snap_in = nullptr;
+        T scale = T(1);
+        if (shared && p.start_slot >= 0) {
+          snap_in = scratch + size_t(p.start_slot) * NV * 64;
+          scale = T(double(p.start_src_len) / double(p.len));  // kernel.rs:641
+        }
+        const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
+        stage_hap<PD>(a, p, start, L, lane, lw);
+        const T acc = sweep<T, R, PD>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
+                                      a.snaps + p.snap0, nsnap, start, scratch);
+        const double raw = double(readlane(acc, L - 1));
+        if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted))
+          res = __builtin_nan("");
+        else
+          res = log10(raw) - Num<T>::kLog10Scale;
+      }
+      if (lane == 0) {
+        a.out[out_base + p.src] = res;
+        if (a.rescue && res != res) {
+          const int idx = atomicAdd(a.rescue_count, 1);
+          a.rescue_pairs[2 * idx] = r;
+          a.rescue_pairs[2 * idx + 1] = k;
+        }
+      }
+      prev = res;
+    }
+  }
+}
+
+__global__ void fill_i32(int32_t* p, int n, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void dpp_selftest_kernel(const int32_t* in, int32_t* out) {
+  const int lane = threadIdx.x & 63;
+  out[lane] = shr1i(-1, in[lane]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side of the engine.
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      set_error(kInternal, std::string("HIP error: ") + hipGetErrorString(_e) + " at " +   \
+                                 #expr);                                                     \
+      return kInternal;                                                                      \
+    }                                                                                        \
+  } while (0)
+
+namespace {
+
+using KernelF = void (*)(SweepArgs, DevTab<float>);
+using KernelD = void (*)(SweepArgs, DevTab<double>);
+
+template <typename T> struct KernelTable;
+template <> struct KernelTable<float> {
+  static constexpr KernelF plain[kMaxRowsPerLane] = {
+      sweep_kernel<float, 1, false>, sweep_kernel<float, 2, false>, sweep_kernel<float, 3, false>,
+      sweep_kernel<float, 4, false>, sweep_kernel<float, 5, false>, sweep_kernel<float, 6, false>,
+      sweep_kernel<float, 7, false>, sweep_kernel<float, 8, false>};
+  static constexpr KernelF pd[kMaxRowsPerLane] = {
+      sweep_kernel<float, 1, true>, sweep_kernel<float, 2, true>, sweep_kernel<float, 3, true>,
+      sweep_kernel<float, 4, true>, sweep_kernel<float, 5, true>, sweep_kernel<float, 6, true>,
+      sweep_kernel<float, 7, true>, sweep_kernel<float, 8, true>};
+};
+template <> struct KernelTable<double> {
+  static constexpr KernelD plain[kMaxRowsPerLane] = {
+      sweep_kernel<double, 1, false>, sweep_kernel<double, 2, false>,
+      sweep_kernel<double, 3, false>, sweep_kernel<double, 4, false>,
+      sweep_kernel<double, 5, false>, sweep_kernel<double, 6, false>,
+      sweep_kernel<double, 7, false>, sweep_kernel<double, 8, false>};
+  static constexpr KernelD pd[kMaxRowsPerLane] = {
+      sweep_kernel<double, 1, true>, sweep_kernel<double, 2, true>, sweep_kernel<double, 3, true>,
+      sweep_kernel<double, 4, true>, sweep_kernel<double, 5, true>, sweep_kernel<double, 6, true>,
+      sweep_kernel<double, 7, true>, sweep_kernel<double, 8, true>};
+};
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+struct DeviceState {
+  int device = -1;
+  int num_cus = 0;
+  DevTab<float> tf{};
+  DevTab<double> td{};
+  // Resident blocks per CU for each kernel, at the LDS size it was queried with.
+  int occ[2][2][kMaxRowsPerLane] = {};
+  int occ_lds[2][2][kMaxRowsPerLane] = {};
+};
+
+std::mutex g_dev_mutex;
+DeviceState* g_dev = nullptr;
+
+template <typename T>
+int upload_tables(DevTab<T>& out) {
+  const Tables& t = tables();
+  std::vector<T> pm(kMaxQual + 1), px(kMaxQual + 1), ep(kMaxQual + 1), im(kMaxQual + 1),
+      mm(kMatchToMatchSize);
+  for (int q = 0; q <= kMaxQual; ++q) {
+    const double e = t.error_prob[q];
+    pm[q] = T(1.0 - e);                 // model.rs:80-83, narrowed as kernel.rs:516-522
+    px[q] = T(e / 3.0);
+    ep[q] = T(e);
+    im[q] = T(1.0 - e);                 // model.rs:73
+  }
+  for (int i = 0; i < kMatchToMatchSize; ++i) mm[i] = T(t.match_to_match[i]);
+  T* d = nullptr;
+  const size_t n4 = size_t(kMaxQual + 1);
+  HIP_TRY(hipMalloc(&d, sizeof(T) * (4 * n4 + kMatchToMatchSize)));
+  HIP_TRY(hipMemcpy(d, pm.data(), sizeof(T) * n4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d + n4, px.data(), sizeof(T) * n4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d + 2 * n4, ep.data(), sizeof(T) * n4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d + 3 * n4, im.data(), sizeof(T) * n4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d + 4 * n4, mm.data(), sizeof(T) * kMatchToMatchSize, hipMemcpyHostToDevice));
+  out.pm = d;
+  out.px = d + n4;
+  out.ep = d + 2 * n4;
+  out.im = d + 3 * n4;
+  out.mm = d + 4 * n4;
+  return kOk;
+}
+
+int device_state(DeviceState** out) {
+  std::lock_guard<std::mutex> lock(g_dev_mutex);
+  if (!g_dev) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+      set_error(kInternal, "no HIP device is available: the fgkl GPU engine needs an MI355X (gfx950)");
+      return kInternal;
+    }
+    auto* st = new DeviceState;
+    HIP_TRY(hipGetDevice(&st->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, st->device));
+    st->num_cus = prop.multiProcessorCount;
+    if (int rc = upload_tables(st->tf)) return rc;
+    if (int rc = upload_tables(st->td)) return rc;
+    g_dev = st;
+  }
+  *out = g_dev;
+  return kOk;
+}
+
+template <typename T>
+int resident_blocks(DeviceState* st, bool pd, int R, int lds_bytes, int* blocks) {
+  const int ti = std::is_same<T, double>::value ? 1 : 0;
+  int& occ = st->occ[ti][pd][R - 1];
+  int& occ_lds = st->occ_lds[ti][pd][R - 1];
+  if (occ == 0 || occ_lds != lds_bytes) {
+    int nb = 0;
+    auto kern = pd ? KernelTable<T>::pd[R - 1] : KernelTable<T>::plain[R - 1];
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, size_t(lds_bytes)));
+    occ = std::max(nb, 1);
+    occ_lds = lds_bytes;
+  }
+  *blocks = occ;
+  return kOk;
+}
+
+template <typename T> DevTab<T> dev_tab(DeviceState* st);
+template <> DevTab<float> dev_tab<float>(DeviceState* st) { return st->tf; }
+template <> DevTab<double> dev_tab<double>(DeviceState* st) { return st->td; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+struct DeviceBatch::Impl {
+  DeviceState* st = nullptr;
+  HostPlan plan;
+  bool pd = false;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // device copies
+  uint8_t *bases = nullptr, *quals = nullptr, *ins = nullptr, *del = nullptr, *gcp = nullptr;
+  int64_t* read_off = nullptr;
+  int32_t *read_region = nullptr, *read_local = nullptr;
+  int64_t* region_out = nullptr;
+  int32_t *region_hap0 = nullptr, *region_nhap = nullptr;
+  HapPlan* hplan = nullptr;
+  SnapEntry* snaps = nullptr;
+  uint8_t* hap_bytes = nullptr;
+  uint32_t* hap_words = nullptr;
+  int32_t* bucket_items[kMaxRowsPerLane] = {};
+  int32_t bucket_n[kMaxRowsPerLane] = {};
+  int64_t bucket_pairs[kMaxRowsPerLane] = {};
+  int32_t* rescue_pairs[kMaxRowsPerLane] = {};
+  int32_t* counters = nullptr;  // [0,8) main, [8,16) rescue, [16,24) rescue counts
+  double* out = nullptr;
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // per-region pair lists, for the per-call mode
+  int32_t* region_pairs = nullptr;
+  std::vector<int64_t> region_pair_off;  // (G * 8 + 1) offsets in pairs, bucket-major per region
+  int32_t* region_counters = nullptr;
+  int64_t n_region_launches = 0;
+  hipEvent_t ev[4] = {};
+  hipGraphExec_t graph[8] = {};  // by mode * 2 + share
+  int lds_words = 0;
+
+  ~Impl() {
+    for (auto& g : graph)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <typename P>
+  int alloc(P** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), sizeof(P) * count));
+    allocs.push_back(*p);
+    return kOk;
+  }
+  template <typename P>
+  int upload(P** p, const P* src, size_t count) {
+    if (int rc = alloc(p, count)) return rc;
+    if (count) HIP_TRY(hipMemcpyAsync(*p, src, sizeof(P) * count, hipMemcpyHostToDevice, stream));
+    return kOk;
+  }
+
+  SweepArgs base_args() const {
+    SweepArgs a{};
+    a.bases = bases; a.quals = quals; a.ins = ins; a.del = del; a.gcp = gcp;
+    a.read_off = read_off; a.read_region = read_region; a.read_local = read_local;
+    a.region_out = region_out; a.region_hap0 = region_hap0; a.region_nhap = region_nhap;
+    a.plan = hplan; a.snaps = snaps; a.hap_bytes = hap_bytes; a.hap_words = hap_words;
+    a.scratch = scratch; a.out = out; a.lds_words = lds_words;
+    return a;
+  }
+
+  template <typename T>
+  int launch(int R, SweepArgs a, int64_t max_items) {
+    if (max_items <= 0) return kOk;
+    const int lds_bytes = lds_words * 4 * kWavesPerBlock;
+    int per_cu = 0;
+    if (int rc = resident_blocks<T>(st, pd, R, lds_bytes, &per_cu)) return rc;
+    const int64_t want = (max_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int grid = int(std::min<int64_t>(int64_t(per_cu) * st->num_cus, want));
+    auto kern = pd ? KernelTable<T>::pd[R - 1] : KernelTable<T>::plain[R - 1];
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds_bytes, stream, a, dev_tab<T>(st));
+    HIP_TRY(hipGetLastError());
+    return kOk;
+  }
+
+  template <typename T>
+  int64_t scratch_need(int R) {
+    const int lds_bytes = lds_words * 4 * kWavesPerBlock;
+    int per_cu = 0;
+    if (resident_blocks<T>(st, pd, R, lds_bytes, &per_cu)) return -1;
+    const int64_t waves = int64_t(per_cu) * st->num_cus * kWavesPerBlock;
+    const int nv = pd ? 6 * R + 7 : 3 * R + 4;
+    return waves * int64_t(plan.max_slots) * nv * 64 * int64_t(sizeof(T));
+  }
+  template <typename T>
+  int64_t scratch_stride(int R) const {
+    const int nv = pd ? 6 * R + 7 : 3 * R + 4;
+    return int64_t(plan.max_slots) * nv * 64;
+  }
+
+  int enqueue(int mode, int share);
+};
+
+int DeviceBatch::Impl::enqueue(int mode, int share) {
+  // Reset the work-queue heads and rescue counts with a kernel (graph-safe).
+  hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, stream, counters, 24, 0);
+  HIP_TRY(hipGetLastError());
+  if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue) {
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      if (!bucket_n[R - 1]) continue;
+      SweepArgs a = base_args();
+      a.items = bucket_items[R - 1];
+      a.n_items = bucket_n[R - 1];
+      a.counter = counters + (R - 1);
+      a.share = share;
+      a.scratch_wave_stride = scratch_stride<float>(R);
+      a.rescue = 1;
+      a.rescue_pairs = rescue_pairs[R - 1];
+      a.rescue_count = counters + 16 + (R - 1);
+      if (int rc = launch<float>(R, a, bucket_n[R - 1])) return rc;
+    }
+    HIP_TRY(hipEventRecord(ev[1], stream));
+    for (int R = 1; R <= kMaxRowsPerLane && mode == kModeFp32Rescue; ++R) {
+      if (!bucket_n[R - 1]) continue;
+      SweepArgs a = base_args();
+      a.items = rescue_pairs[R - 1];
+      a.n_items = 0;
+      a.n_items_dev = counters + 16 + (R - 1);
+      a.counter = counters + 8 + (R - 1);
+      a.pair_mode = 1;
+      a.share = 0;
+      a.scratch = nullptr;
+      // The grid is sized for the worst case; waves beyond the rescued count exit at once.
+      if (int rc = launch<double>(R, a, bucket_pairs[R - 1])) return rc;
+    }
+  } else {
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      if (!bucket_n[R - 1]) continue;
+      SweepArgs a = base_args();
+      a.items = bucket_items[R - 1];
+      a.n_items = bucket_n[R - 1];
+      a.counter = counters + (R - 1);
+      a.share = share;
+      a.scratch_wave_stride = scratch_stride<double>(R);
+      if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
+    }
+    HIP_TRY(hipEventRecord(ev[1], stream));
+  }
+  return kOk;
+}
+
+DeviceBatch::DeviceBatch() : impl_(new Impl) {}
+DeviceBatch::~DeviceBatch() { delete impl_; }
+
+int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out) {
+  std::unique_ptr<DeviceBatch> b(new DeviceBatch);
+  Impl& d = *b->impl_;
+  if (int rc = build_plan(in, d.plan)) return rc;
+  if (int rc = device_state(&d.st)) return rc;
+  d.pd = in.pd;
+  HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+  for (auto& e : d.ev) HIP_TRY(hipEventCreate(&e));
+  const HostPlan& p = d.plan;
+  const int64_t NR = p.n_reads, NH = p.n_haps, G = int64_t(p.region_hap0.size());
+  const size_t rb = size_t(in.read_off[NR]);
+  const size_t hb = size_t(in.hap_off[NH]);
+  int rc = 0;
+  if ((rc = d.upload(&d.bases, in.bases, rb)) || (rc = d.upload(&d.quals, in.quals, rb)) ||
+      (rc = d.upload(&d.ins, in.ins, rb)) || (rc = d.upload(&d.del, in.del, rb)) ||
+      (rc = d.upload(&d.gcp, in.gcp, rb)) ||
+      (rc = d.upload(&d.read_off, in.read_off, size_t(NR + 1))) ||
+      (rc = d.upload(&d.read_region, p.read_region.data(), size_t(NR))) ||
+      (rc = d.upload(&d.read_local, p.read_local.data(), size_t(NR))) ||
+      (rc = d.upload(&d.region_out, p.region_out.data(), size_t(G + 1))) ||
+      (rc = d.upload(&d.region_hap0, p.region_hap0.data(), size_t(G))) ||
+      (rc = d.upload(&d.region_nhap, p.region_nhap.data(), size_t(G))) ||
+      (rc = d.upload(&d.hplan, p.plan.data(), size_t(NH))) ||
+      (rc = d.upload(&d.snaps, p.snaps.data(), p.snaps.size())))
+    return rc;
+  if (in.pd) {
+    if ((rc = d.upload(&d.hap_words, p.words.data(), p.words.size()))) return rc;
+  } else {
+    if ((rc = d.upload(&d.hap_bytes, in.hap_bases, hb))) return rc;
+  }
+  for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+    const auto& items = p.bucket_reads[R - 1];
+    d.bucket_n[R - 1] = int32_t(items.size());
+    int64_t pairs = 0;
+    for (int32_t r : items) pairs += p.region_nhap[size_t(p.read_region[size_t(r)])];
+    d.bucket_pairs[R - 1] = pairs;
+    if ((rc = d.upload(&d.bucket_items[R - 1], items.data(), items.size()))) return rc;
+    if ((rc = d.alloc(&d.rescue_pairs[R - 1], size_t(2 * pairs)))) return rc;
+  }
+  if ((rc = d.alloc(&d.counters, 32))) return rc;
+  if ((rc = d.alloc(&d.out, size_t(p.n_pairs)))) return rc;
+  // LDS per wave: the longest haplotype plus the ramp of the widest read, in words.
+  d.lds_words = ((p.max_hap_len + 64 + 8) + 3) & ~3;
+  if (int64_t(d.lds_words) * 4 * kWavesPerBlock > 64 * 1024) {
+    set_error(kInvalidArgument, "haplotype of " + std::to_string(p.max_hap_len) +
+                                    " bases exceeds the engine's LDS staging capacity");
+    return kInvalidArgument;
+  }
+  int64_t need = 0;
+  if (p.max_slots > 0) {
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      if (!d.bucket_n[R - 1]) continue;
+      need = std::max(need, d.scratch_need<float>(R));
+      need = std::max(need, d.scratch_need<double>(R));
+    }
+  }
+  if (need < 0) return kInternal;
+  if (need > 0) {
+    HIP_TRY(hipMalloc(&d.scratch, size_t(need)));
+    d.allocs.push_back(d.scratch);
+    d.scratch_bytes = size_t(need);
+  }
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  out = std::move(b);
+  return kOk;
+}
+
+int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
+  Impl& d = *impl_;
+  if (mode != kModeFp32Rescue && mode != kModeFp64 && mode != kModeFp64PerRegion &&
+      mode != kModeFp32NoRescue) {
+    set_error(kInvalidArgument, "unknown run mode");
+    return kInvalidArgument;
+  }
+  if (d.plan.n_pairs == 0) {
+    if (stats) *stats = RunStats{};
+    return kOk;
+  }
+  if (mode == kModeFp64PerRegion) return run_per_region(stats);
+  HIP_TRY(hipEventRecord(d.ev[0], d.stream));
+  if (use_graph) {
+    hipGraphExec_t& ge = d.graph[mode * 2 + (share ? 1 : 0)];
+    if (!ge) {
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+      const int rc = d.enqueue(mode, share);
+      hipGraph_t captured = nullptr;
+      const hipError_t e = hipStreamEndCapture(d.stream, &captured);
+      if (rc) return rc;
+      HIP_TRY(e);
+      graph = captured;
+      HIP_TRY(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+      HIP_TRY(hipGraphDestroy(graph));
+    }
+    HIP_TRY(hipGraphLaunch(ge, d.stream));
+  } else {
+    if (int rc = d.enqueue(mode, share)) return rc;
+  }
+  HIP_TRY(hipEventRecord(d.ev[2], d.stream));
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  if (stats) {
+    *stats = RunStats{};
+    float total = 0.f;
+    HIP_TRY(hipEventElapsedTime(&total, d.ev[0], d.ev[2]));
+    stats->total_ms = total;
+    int32_t counts[32];
+    HIP_TRY(hipMemcpy(counts, d.counters, sizeof(counts), hipMemcpyDeviceToHost));
+    int64_t rescued = 0;
+    if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue)
+      for (int R = 0; R < kMaxRowsPerLane; ++R) rescued += counts[16 + R];
+    stats->rescued_pairs = rescued;
+    if (!use_graph) {
+      float first = 0.f;
+      HIP_TRY(hipEventElapsedTime(&first, d.ev[0], d.ev[1]));
+      stats->primary_ms = first;
+      stats->rescue_ms = total - first;
+    } else {
+      stats->primary_ms = -1.0;
+      stats->rescue_ms = -1.0;
+    }
+  }
+  return kOk;
+}
+
+int DeviceBatch::run_per_region(RunStats* stats) {
+  Impl& d = *impl_;
+  const HostPlan& p = d.plan;
+  const int64_t G = int64_t(p.region_hap0.size());
+  if (!d.region_pairs) {
+    // Pair lists per region, grouped by rows per lane: the work one per-region call launches.
+    std::vector<int32_t> pairs;
+    pairs.reserve(size_t(2 * p.n_pairs));
+    d.region_pair_off.assign(size_t(G * kMaxRowsPerLane + 1), 0);
+    // Reads of a region in index order; regions are contiguous in read index.
+    int64_t r = 0;
+    for (int64_t g = 0; g < G; ++g) {
+      const int32_t H = p.region_nhap[size_t(g)];
+      int64_t r_end = r;
+      while (r_end < p.n_reads && p.read_region[size_t(r_end)] == g) ++r_end;
+      for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+        d.region_pair_off[size_t(g * kMaxRowsPerLane + R - 1)] = int64_t(pairs.size() / 2);
+        for (int64_t rr = r; rr < r_end; ++rr) {
+          if (p.read_rows[size_t(rr)] != R) continue;
+          for (int32_t k = 0; k < H; ++k) {
+            pairs.push_back(int32_t(rr));
+            pairs.push_back(k);
+          }
+        }
+      }
+      r = r_end;
+    }
+    d.region_pair_off[size_t(G * kMaxRowsPerLane)] = int64_t(pairs.size() / 2);
+    if (int rc = d.upload(&d.region_pairs, pairs.data(), pairs.size())) return rc;
+    d.n_region_launches = G * kMaxRowsPerLane;
+    if (int rc = d.alloc(&d.region_counters, size_t(d.n_region_launches))) return rc;
+  }
+  const int n = int(d.n_region_launches);
+  hipLaunchKernelGGL(fill_i32, dim3((n + 255) / 256), dim3(256), 0, d.stream, d.region_counters, n, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  HIP_TRY(hipEventRecord(d.ev[0], d.stream));
+  for (int64_t g = 0; g < G; ++g) {
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      const int64_t i = g * kMaxRowsPerLane + R - 1;
+      const int64_t lo = d.region_pair_off[size_t(i)], hi = d.region_pair_off[size_t(i + 1)];
+      if (hi == lo) continue;
+      SweepArgs a = d.base_args();
+      a.items = d.region_pairs + 2 * lo;
+      a.n_items = int32_t(hi - lo);
+      a.counter = d.region_counters + i;
+      a.pair_mode = 1;
+      a.scratch = nullptr;
+      if (int rc = d.launch<double>(R, a, hi - lo)) return rc;
+    }
+    // One synchronous call per region, as HaplotypeCaller makes them.
+    HIP_TRY(hipStreamSynchronize(d.stream));
+  }
+  HIP_TRY(hipEventRecord(d.ev[2], d.stream));
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  if (stats) {
+    *stats = RunStats{};
+    float total = 0.f;
+    HIP_TRY(hipEventElapsedTime(&total, d.ev[0], d.ev[2]));
+    stats->total_ms = total;
+    stats->primary_ms = total;
+  }
+  return kOk;
+}
+
+int DeviceBatch::results(double* out, int64_t n) {
+  Impl& d = *impl_;
+  if (n != d.plan.n_pairs) {
+    set_error(kInvalidArgument, "output has " + std::to_string(n) +
+                                    " elements but the batch holds " +
+                                    std::to_string(d.plan.n_pairs) + " pairs");
+    return kInvalidArgument;
+  }
+  if (n) HIP_TRY(hipMemcpy(out, d.out, sizeof(double) * size_t(n), hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+int64_t DeviceBatch::n_pairs() const { return impl_->plan.n_pairs; }
+int64_t DeviceBatch::scratch_bytes() const { return int64_t(impl_->scratch_bytes); }
+
+int dpp_selftest(int32_t* out64) {
+  DeviceState* st = nullptr;
+  if (int rc = device_state(&st)) return rc;
+  int32_t in[64];
+  for (int i = 0; i < 64; ++i) in[i] = 1000 + i;
+  int32_t *din = nullptr, *dout = nullptr;
+  HIP_TRY(hipMalloc(&din, sizeof(in)));
+  HIP_TRY(hipMalloc(&dout, sizeof(in)));
+  HIP_TRY(hipMemcpy(din, in, sizeof(in), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(dpp_selftest_kernel, dim3(1), dim3(64), 0, nullptr, din, dout);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(out64, dout, sizeof(in), hipMemcpyDeviceToHost));
+  HIP_TRY(hipFree(din));
+  HIP_TRY(hipFree(dout));
+  return kOk;
+}
+
+int device_available() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+  return count;
+}
+
+}  // namespace fgkl

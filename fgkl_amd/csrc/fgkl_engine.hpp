@@ -1,0 +1,48 @@
+// The device engine's host interface: a batch of regions resident in HBM and the ways to run it.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+
+#include "fgkl_internal.hpp"
+
+namespace fgkl {
+
+enum RunMode : int {
+  kModeFp32Rescue = 0,    // FP32 sweeps, FP64 recomputation of pairs below 1e-28 (lib.rs:377-549)
+  kModeFp64 = 1,          // FP64 sweeps for every pair
+  kModeFp64PerRegion = 2, // FP64, one pair per wave, one synchronous launch set per region
+  kModeFp32NoRescue = 3,  // FP32 only; underflowing pairs are left NaN (double_fallback off)
+};
+
+struct RunStats {
+  double total_ms = 0;     // device time of the whole run (HIP events on the batch's stream)
+  double primary_ms = 0;   // the FP32 (or FP64) sweep kernels; -1 when run as one graph
+  double rescue_ms = 0;    // the FP64 rescue kernels; -1 when run as one graph
+  int64_t rescued_pairs = 0;
+};
+
+class DeviceBatch {
+ public:
+  ~DeviceBatch();
+  // Validates and plans `in`, then copies it into device memory.
+  static int create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out);
+  // Computes every pair; `share` enables prefix sharing; `use_graph` replays one captured
+  // hipGraph of the whole run.
+  int run(int mode, int share, int use_graph, RunStats* stats);
+  // Copies the log10 likelihoods out, region by region, out[region_base + r * H + h].
+  int results(double* out, int64_t n);
+  int64_t n_pairs() const;
+  int64_t scratch_bytes() const;
+
+ private:
+  DeviceBatch();
+  int run_per_region(RunStats* stats);
+  struct Impl;
+  Impl* impl_;
+};
+
+int dpp_selftest(int32_t* out64);
+int device_available();
+
+}  // namespace fgkl

@@ -1,0 +1,109 @@
+// Internal declarations shared by the host planner, the HIP engine and the C-ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fgkl {
+
+// Status codes of the C-ABI (include/fgkl.h).
+enum Status : int { kOk = 0, kInvalidArgument = 1, kInternal = 2 };
+
+// Thread-local error message for the last failing call on this thread.
+void set_error(int code, const std::string& msg);
+const char* last_error();
+
+// ---------------------------------------------------------------------------------------------
+// Phred tables, built in f64 exactly as model.rs:36-106 builds them.
+constexpr int kMaxQual = 254;                                      // model.rs:11
+constexpr int kMatchToMatchSize = (kMaxQual + 1) * (kMaxQual + 2) / 2;  // model.rs:39
+
+struct Tables {
+  double error_prob[kMaxQual + 1];
+  double match_to_match[kMatchToMatchSize];
+};
+const Tables& tables();
+
+inline int clamp_qual(uint8_t q) { return q > kMaxQual ? kMaxQual : q; }
+inline int tri_index(uint8_t a8, uint8_t b8) {  // model.rs:59-64
+  int a = clamp_qual(a8), b = clamp_qual(b8);
+  int lo = a <= b ? a : b, hi = a <= b ? b : a;
+  return ((hi * (hi + 1)) >> 1) + lo;
+}
+
+// PD flag bits (pd.rs:15-27) and deletion states (pd_kernel.rs:35-37).
+constexpr uint8_t kSnp = 1, kDelStart = 2, kDelEnd = 4;
+constexpr uint8_t kAltA = 8, kAltC = 16, kAltG = 32, kAltT = 64;
+constexpr uint32_t kNormal = 0, kInsideDel = 1, kAfterDel = 2;
+
+// The per-column word the PD kernel passes between lanes: haplotype byte, accepted alternates,
+// the deletion state of the column in row 1 and in rows >= 2, and its DEL_END flag.
+inline uint32_t pd_word(uint8_t base, uint8_t flag, uint32_t st1, uint32_t st2) {
+  uint32_t alts = (flag & kSnp) ? ((flag >> 3) & 0xFu) : 0u;  // MatchSet::of (pd_kernel.rs:196-214)
+  return uint32_t(base) | (alts << 8) | (st1 << 12) | (st2 << 14) |
+         (uint32_t((flag & kDelEnd) != 0) << 16);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Plan of one batch: haplotypes sorted per region for prefix sharing (kernel.rs:53-81,
+// pd_kernel.rs:61-136) and the snapshot columns each sorted haplotype must record
+// (kernel.rs:236-259, 556-562), all laid out for the device.
+
+struct HapPlan {        // one per haplotype, region-contiguous, in sorted order
+  int64_t off;          // offset of the haplotype in the batch's haplotype byte / word arrays
+  int32_t len;          // m
+  int32_t lcp;          // common prefix with the previous sorted haplotype (0 for the first)
+  int32_t src;          // caller's index of this haplotype within its region
+  int32_t dup;          // equal to the previous sorted haplotype: copies its results
+  int32_t start_slot;   // snapshot slot holding column `lcp`, or -1 to start from column 0
+  int32_t start_src_len;  // length of the haplotype that wrote that snapshot
+  int32_t snap0;        // first entry in the snapshot list
+  int32_t nsnap;        // columns this haplotype must snapshot
+};
+static_assert(sizeof(HapPlan) == 40, "HapPlan layout");
+
+struct SnapEntry {
+  int32_t col;   // 1-based haplotype column
+  int32_t slot;  // scratch slot
+};
+
+// A batch as the caller hands it over: regions of reads and haplotypes, concatenated.
+struct BatchInput {
+  bool pd = false;
+  int64_t n_regions = 0;
+  const int64_t* region_read = nullptr;  // n_regions + 1, read index ranges
+  const int64_t* region_hap = nullptr;   // n_regions + 1, haplotype index ranges
+  const int64_t* read_off = nullptr;     // n_reads + 1, byte ranges into the read arrays
+  const uint8_t *bases = nullptr, *quals = nullptr, *ins = nullptr, *del = nullptr,
+                *gcp = nullptr;
+  const int64_t* hap_off = nullptr;  // n_haps + 1, byte ranges into hap_bases / hap_flags
+  const uint8_t* hap_bases = nullptr;
+  const uint8_t* hap_flags = nullptr;  // PD only
+};
+
+constexpr int kMaxRowsPerLane = 8;   // reads up to 512 bases
+constexpr int kMaxReadLen = 64 * kMaxRowsPerLane;
+
+struct HostPlan {
+  int64_t n_reads = 0, n_haps = 0, n_pairs = 0;
+  std::vector<int32_t> read_region, read_local, read_rows;  // rows per lane R per read
+  std::vector<int64_t> region_out;  // output base per region
+  std::vector<int32_t> region_hap0, region_nhap;
+  std::vector<HapPlan> plan;        // n_haps
+  std::vector<SnapEntry> snaps;
+  std::vector<uint32_t> words;      // PD column words (same offsets as the haplotype bytes)
+  int32_t max_slots = 0;
+  int32_t max_hap_len = 0;
+  // Work: read indices per rows-per-lane bucket (index R-1), longest work first.
+  std::vector<int32_t> bucket_reads[kMaxRowsPerLane];
+};
+
+// Validates the batch and builds its plan.  Returns kOk or kInvalidArgument with a message
+// set through set_error.
+int build_plan(const BatchInput& in, HostPlan& out);
+
+// Rows per lane for a read of n bases.
+inline int rows_per_lane(int64_t n) { return int((n + 63) / 64); }
+
+}  // namespace fgkl

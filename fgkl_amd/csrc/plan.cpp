@@ -1,0 +1,311 @@
+// Host side of the engine: the Phred tables, input validation and the per-batch plan (sorted
+// haplotypes, common prefixes, duplicates, snapshot columns, PD column words, work buckets).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+
+#include "fgkl_internal.hpp"
+
+namespace fgkl {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_last_error = msg;
+}
+const char* last_error() { return g_last_error.c_str(); }
+
+// ---- model.rs:36-106: GATK's tables, in f64 --------------------------------------------------
+namespace {
+constexpr double kJacobianMaxTolerance = 8.0;   // model.rs:23
+constexpr double kJacobianTableStep = 0.0001;   // model.rs:24
+const double kJacobianInvStep = 1.0 / kJacobianTableStep;                 // model.rs:25
+const double kInvLn10 = 1.0 / 2.302585092994045684017991454684364208;     // model.rs:26
+
+int64_t fast_round(double d) { return d > 0.0 ? int64_t(d + 0.5) : int64_t(d - 0.5); }
+
+double jacobian(double diff) {
+  const int64_t k = fast_round(diff * kJacobianInvStep);
+  return std::log10(1.0 + std::pow(10.0, -double(k) * kJacobianTableStep));
+}
+
+double approximate_log10_sum_log10(double a, double b) {
+  if (a > b) return approximate_log10_sum_log10(b, a);
+  if (a == -HUGE_VAL) return b;
+  const double diff = b - a;
+  return diff < kJacobianMaxTolerance ? b + jacobian(diff) : b;
+}
+
+Tables* build_tables() {
+  auto* t = new Tables;
+  for (int q = 0; q <= kMaxQual; ++q) t->error_prob[q] = std::pow(10.0, double(q) / -10.0);
+  int offset = 0;
+  for (int i = 0; i <= kMaxQual; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      const double log10_sum = approximate_log10_sum_log10(-0.1 * double(i), -0.1 * double(j));
+      const double p = std::min(std::pow(10.0, log10_sum), 1.0);
+      const double m2m_log10 = std::log1p(-p) * kInvLn10;
+      t->match_to_match[offset + j] = std::pow(10.0, m2m_log10);
+    }
+    offset += i + 1;
+  }
+  return t;
+}
+}  // namespace
+
+const Tables& tables() {
+  static const Tables* t = build_tables();
+  return *t;
+}
+
+// ---- validation (lib.rs:54-67, 272-289; pd.rs:75-92) and planning -------------------------
+namespace {
+
+// segments() of pd_kernel.rs:723-743 reduced to the state before each column and the state
+// carried out of the row.
+uint32_t column_states(const uint8_t* flags, int64_t m, uint32_t state, uint32_t* st) {
+  for (int64_t j = 0; j < m; ++j) {
+    const uint8_t f = flags[j];
+    if (st) st[j] = state;
+    if (state == kAfterDel) state = kNormal;
+    if (f & kDelStart) state = kInsideDel;
+    if (f & kDelEnd) state = kAfterDel;
+  }
+  return state;
+}
+
+struct RegionHaps {
+  const BatchInput* in;
+  const std::vector<uint32_t>* end_state;
+  int64_t len(int64_t h) const { return in->hap_off[h + 1] - in->hap_off[h]; }
+  const uint8_t* b(int64_t h) const { return in->hap_bases + in->hap_off[h]; }
+  const uint8_t* f(int64_t h) const { return in->hap_flags + in->hap_off[h]; }
+
+  // Lexicographic order of kernel.rs:55 (plain) or of (row-end state, interleaved bases and
+  // flags) as pd_kernel.rs:67-76 sorts.
+  bool less(int64_t a, int64_t c) const {
+    if (in->pd) {
+      if ((*end_state)[a] != (*end_state)[c]) return (*end_state)[a] < (*end_state)[c];
+      const int64_t la = len(a), lc = len(c), n = std::min(la, lc);
+      const uint8_t *ba = b(a), *bc = b(c), *fa = f(a), *fc = f(c);
+      for (int64_t j = 0; j < n; ++j) {
+        if (ba[j] != bc[j]) return ba[j] < bc[j];
+        if (fa[j] != fc[j]) return fa[j] < fc[j];
+      }
+      return la < lc;
+    }
+    const int64_t la = len(a), lc = len(c);
+    const int r = std::memcmp(b(a), b(c), size_t(std::min(la, lc)));
+    return r != 0 ? r < 0 : la < lc;
+  }
+
+  // Common prefix of two haplotypes (kernel.rs:68-70; zero across PD end states,
+  // pd_kernel.rs:115-122).
+  int64_t lcp(int64_t a, int64_t c) const {
+    if (in->pd && (*end_state)[a] != (*end_state)[c]) return 0;
+    const int64_t n = std::min(len(a), len(c));
+    const uint8_t *ba = b(a), *bc = b(c);
+    int64_t j = 0;
+    if (in->pd) {
+      const uint8_t *fa = f(a), *fc = f(c);
+      while (j < n && ba[j] == bc[j] && fa[j] == fc[j]) ++j;
+    } else {
+      while (j < n && ba[j] == bc[j]) ++j;
+    }
+    return j;
+  }
+};
+
+bool check_offsets(const int64_t* off, int64_t count, const char* what) {
+  if (off[0] != 0) {
+    set_error(kInvalidArgument, std::string(what) + " offset table must start at zero");
+    return false;
+  }
+  for (int64_t i = 0; i < count; ++i)
+    if (off[i + 1] < off[i]) {
+      set_error(kInvalidArgument, std::string(what) + " offset table must be non-decreasing");
+      return false;
+    }
+  return true;
+}
+
+}  // namespace
+
+int build_plan(const BatchInput& in, HostPlan& p) {
+  const int64_t G = in.n_regions;
+  if (G < 0) {
+    set_error(kInvalidArgument, "negative region count");
+    return kInvalidArgument;
+  }
+  if (!check_offsets(in.region_read, G, "region read") ||
+      !check_offsets(in.region_hap, G, "region haplotype"))
+    return kInvalidArgument;
+  const int64_t NR = in.region_read[G], NH = in.region_hap[G];
+  if (!check_offsets(in.read_off, NR, "read") || !check_offsets(in.hap_off, NH, "haplotype"))
+    return kInvalidArgument;
+
+  // lib.rs:276-283: every read validated before any haplotype; indices are region-local as the
+  // caller's per-region call would report them.
+  for (int64_t g = 0; g < G; ++g) {
+    for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r) {
+      const int64_t n = in.read_off[r + 1] - in.read_off[r];
+      if (n == 0) {
+        set_error(kInvalidArgument, "read " + std::to_string(r - in.region_read[g]) +
+                                        " has no bases");
+        return kInvalidArgument;
+      }
+      if (n > kMaxReadLen) {
+        set_error(kInvalidArgument, "read " + std::to_string(r - in.region_read[g]) + " has " +
+                                        std::to_string(n) +
+                                        " bases; the GPU engine supports reads of at most " +
+                                        std::to_string(kMaxReadLen));
+        return kInvalidArgument;
+      }
+    }
+    for (int64_t h = in.region_hap[g]; h < in.region_hap[g + 1]; ++h) {
+      if (in.hap_off[h + 1] == in.hap_off[h]) {
+        set_error(kInvalidArgument, "haplotype " + std::to_string(h - in.region_hap[g]) +
+                                        " has no bases");
+        return kInvalidArgument;
+      }
+      if (in.hap_off[h + 1] - in.hap_off[h] > (int64_t(1) << 30)) {
+        set_error(kInvalidArgument, "haplotype too long");
+        return kInvalidArgument;
+      }
+    }
+  }
+
+  p = HostPlan{};
+  p.n_reads = NR;
+  p.n_haps = NH;
+  p.read_region.resize(size_t(NR));
+  p.read_local.resize(size_t(NR));
+  p.read_rows.resize(size_t(NR));
+  p.region_out.resize(size_t(G) + 1);
+  p.region_hap0.resize(size_t(G));
+  p.region_nhap.resize(size_t(G));
+  p.plan.resize(size_t(NH));
+
+  std::vector<uint32_t> end_state(in.pd ? size_t(NH) : 0);
+  if (in.pd) {
+    p.words.resize(size_t(in.hap_off[NH]));
+    std::vector<uint32_t> st1, st2;
+    for (int64_t h = 0; h < NH; ++h) {
+      const int64_t o = in.hap_off[h], m = in.hap_off[h + 1] - o;
+      st1.resize(size_t(m));
+      st2.resize(size_t(m));
+      const uint32_t e = column_states(in.hap_flags + o, m, kNormal, st1.data());
+      column_states(in.hap_flags + o, m, e, st2.data());
+      end_state[size_t(h)] = e;
+      for (int64_t j = 0; j < m; ++j)
+        p.words[size_t(o + j)] =
+            pd_word(in.hap_bases[o + j], in.hap_flags[o + j], st1[size_t(j)], st2[size_t(j)]);
+    }
+  }
+  RegionHaps rh{&in, &end_state};
+
+  int64_t out_base = 0;
+  std::vector<int64_t> order;
+  std::vector<int64_t> lcp;
+  std::map<int64_t, int32_t> counts;
+  for (int64_t g = 0; g < G; ++g) {
+    const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
+    const int64_t h0 = in.region_hap[g], h1 = in.region_hap[g + 1], H = h1 - h0;
+    p.region_out[size_t(g)] = out_base;
+    p.region_hap0[size_t(g)] = int32_t(h0);
+    p.region_nhap[size_t(g)] = int32_t(H);
+    out_base += (r1 - r0) * H;
+    for (int64_t r = r0; r < r1; ++r) {
+      p.read_region[size_t(r)] = int32_t(g);
+      p.read_local[size_t(r)] = int32_t(r - r0);
+      p.read_rows[size_t(r)] = rows_per_lane(in.read_off[r + 1] - in.read_off[r]);
+    }
+    if (H == 0) continue;
+
+    // Stable sort, as Rust's sort_by (kernel.rs:55, pd_kernel.rs:72-76).
+    order.resize(size_t(H));
+    std::iota(order.begin(), order.end(), h0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t c) { return rh.less(a, c); });
+    lcp.assign(size_t(H), 0);
+    for (int64_t k = 1; k < H; ++k) lcp[size_t(k)] = rh.lcp(order[size_t(k - 1)], order[size_t(k)]);
+
+    // lcp_counts (kernel.rs:556-562) and one scratch slot per distinct start column.
+    counts.clear();
+    std::vector<char> dup(size_t(H), 0);
+    for (int64_t k = 1; k < H; ++k) {
+      const int64_t a = order[size_t(k - 1)], c = order[size_t(k)];
+      dup[size_t(k)] = rh.len(a) == rh.len(c) && lcp[size_t(k)] == rh.len(c);
+      if (!dup[size_t(k)]) counts[lcp[size_t(k)]] += 1;
+    }
+    std::map<int64_t, int32_t> slot_of;
+    for (auto& kv : counts)
+      if (kv.first > 0) slot_of.emplace(kv.first, int32_t(slot_of.size()));
+    std::vector<int32_t> last_len(slot_of.size(), 0);
+    p.max_slots = std::max<int32_t>(p.max_slots, int32_t(slot_of.size()));
+
+    for (int64_t k = 0; k < H; ++k) {
+      const int64_t h = order[size_t(k)];
+      HapPlan& hp = p.plan[size_t(h0 + k)];
+      hp.off = in.hap_off[h];
+      hp.len = int32_t(rh.len(h));
+      hp.lcp = int32_t(lcp[size_t(k)]);
+      hp.src = int32_t(h - h0);
+      hp.dup = dup[size_t(k)];
+      hp.start_slot = -1;
+      hp.start_src_len = 0;
+      hp.snap0 = int32_t(p.snaps.size());
+      hp.nsnap = 0;
+      p.max_hap_len = std::max(p.max_hap_len, hp.len);
+      if (hp.dup) continue;
+      const int64_t start = k == 0 ? 0 : lcp[size_t(k)];
+      hp.lcp = int32_t(start);
+      if (k > 0) counts[start] -= 1;  // kernel.rs:248-250
+      if (start > 0) {
+        hp.start_slot = slot_of.at(start);
+        hp.start_src_len = last_len[size_t(hp.start_slot)];
+      }
+      // Columns in (start, len] some later haplotype starts from (kernel.rs:253-258).
+      for (auto it = slot_of.upper_bound(start); it != slot_of.end() && it->first <= hp.len; ++it) {
+        if (counts[it->first] > 0) {
+          p.snaps.push_back(SnapEntry{int32_t(it->first), it->second});
+          last_len[size_t(it->second)] = hp.len;
+          hp.nsnap += 1;
+        }
+      }
+    }
+  }
+  p.region_out[size_t(G)] = out_base;
+  p.n_pairs = out_base;
+
+  // Work buckets by rows per lane, costliest first so the persistent grid's tail is short.
+  std::vector<int64_t> cost(size_t(NR), 0);
+  for (int64_t g = 0; g < G; ++g) {
+    int64_t cols = 0;
+    const int32_t H = p.region_nhap[size_t(g)];
+    for (int32_t k = 0; k < H; ++k) {
+      const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
+      if (!hp.dup) cols += hp.len - hp.lcp;
+    }
+    for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r) {
+      const int64_t n = in.read_off[r + 1] - in.read_off[r];
+      const int R = p.read_rows[size_t(r)];
+      const int64_t L = (n + R - 1) / R;
+      cost[size_t(r)] = (cols + int64_t(H) * (L - 1)) * R;
+    }
+  }
+  for (int64_t r = 0; r < NR; ++r) p.bucket_reads[p.read_rows[size_t(r)] - 1].push_back(int32_t(r));
+  for (auto& b : p.bucket_reads)
+    std::stable_sort(b.begin(), b.end(),
+                     [&](int32_t a, int32_t c) { return cost[size_t(a)] > cost[size_t(c)]; });
+  return kOk;
+}
+
+}  // namespace fgkl

@@ -1,0 +1,160 @@
+/*
+ * fgkl-mi355x: the C-ABI of the MI355X (gfx950) PairHMM / PD-HMM engine.
+ *
+ * Plain pointers and sizes only.  Every function returns FGKL_OK (0), FGKL_INVALID_ARGUMENT (1)
+ * or FGKL_INTERNAL_ERROR (2); fgkl_last_error() gives the calling thread's message for the last
+ * failure.  The reference maps the same two failure classes to IllegalArgumentException and
+ * RuntimeException (crates/jni/src/lib.rs:21-92).
+ *
+ * Reads are passed as five concatenated byte arrays (bases, base qualities, insertion and
+ * deletion gap-open qualities, gap-continuation penalties) delimited by an offset table, and
+ * haplotypes (with PD flags) likewise; the log10 likelihood of read r given haplotype h of a
+ * region lands at likelihoods[r * H + h] (FgklPairHmm.java:100-105, lib.rs:270-271).
+ */
+#ifndef FGKL_H
+#define FGKL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGKL_OK 0
+#define FGKL_INVALID_ARGUMENT 1
+#define FGKL_INTERNAL_ERROR 2
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* fgkl_last_error(void);
+
+/* Replaces FgklPairHmm.backendNative / FgklPdHmm.backendNative
+ * (crates/jni/src/lib.rs:149-156, :197-204): the engine's name, "hip-gfx950". */
+const char* fgkl_backend_name(void);
+
+/* Number of HIP devices visible to the process (0 on a host without a GPU). */
+int fgkl_device_count(void);
+
+/*
+ * Replaces Java_com_fulcrumgenomics_fgkl_pairhmm_FgklPairHmm_computeNative
+ * (crates/jni/src/lib.rs:159-195; Java declaration FgklPairHmm.java:106).
+ * One region: read_offsets has R+1 entries, hap_offsets H+1, likelihoods R*H.
+ * Validation follows the reference: the five read arrays must have equal total length
+ * (lib.rs:122-131); offset tables start at zero, never decrease and end at the array length
+ * (lib.rs:343-363); likelihoods must hold R*H values (lib.rs:325-335); reads and haplotypes
+ * must be non-empty (crates/pairhmm/src/lib.rs:54-67, :279-281).  double_precision selects the
+ * FP64 engine, otherwise FP32 with FP64 recomputation of pairs below 1e-28
+ * (crates/jni/src/lib.rs:319-322).
+ */
+int fgkl_pairhmm_compute_native(const uint8_t* bases, int64_t bases_len,
+                                const uint8_t* quals, int64_t quals_len,
+                                const uint8_t* ins_gop, int64_t ins_gop_len,
+                                const uint8_t* del_gop, int64_t del_gop_len,
+                                const uint8_t* gcp, int64_t gcp_len,
+                                const int32_t* read_offsets, int64_t read_offsets_len,
+                                const uint8_t* hap_bases, int64_t hap_bases_len,
+                                const int32_t* hap_offsets, int64_t hap_offsets_len,
+                                int double_precision,
+                                double* likelihoods, int64_t likelihoods_len);
+
+/*
+ * Replaces Java_com_fulcrumgenomics_fgkl_pdhmm_FgklPdHmm_computeNative
+ * (crates/jni/src/lib.rs:207-254; Java declaration FgklPdHmm.java:121).  As above plus the
+ * per-base PD flags (pd.rs:15-27), which must be as long as hap_bases (lib.rs:232-237).
+ */
+int fgkl_pdhmm_compute_native(const uint8_t* bases, int64_t bases_len,
+                              const uint8_t* quals, int64_t quals_len,
+                              const uint8_t* ins_gop, int64_t ins_gop_len,
+                              const uint8_t* del_gop, int64_t del_gop_len,
+                              const uint8_t* gcp, int64_t gcp_len,
+                              const int32_t* read_offsets, int64_t read_offsets_len,
+                              const uint8_t* hap_bases, int64_t hap_bases_len,
+                              const uint8_t* hap_flags, int64_t hap_flags_len,
+                              const int32_t* hap_offsets, int64_t hap_offsets_len,
+                              int double_precision,
+                              double* likelihoods, int64_t likelihoods_len);
+
+/*
+ * A configured PairHMM or PD-HMM, replacing PairHmm::new / PdPairHmm::new with Config
+ * {precision, double_fallback} (crates/pairhmm/src/lib.rs:213-265, pd.rs:41-66).
+ * pd selects the PD kernel; double_precision the FP64 engine; double_fallback recomputes
+ * underflowing FP32 pairs in FP64 (off: they are left NaN, lib.rs:221-226).
+ * Returns NULL (message in fgkl_last_error) if no GPU is available.
+ */
+typedef struct fgkl_pairhmm fgkl_pairhmm;
+fgkl_pairhmm* fgkl_pairhmm_new(int pd, int double_precision, int double_fallback);
+void fgkl_pairhmm_free(fgkl_pairhmm* hmm);
+/* Replaces PairHmm::fallback_pairs (lib.rs:263-268): FP32 pairs recomputed so far. */
+uint64_t fgkl_pairhmm_fallback_pairs(const fgkl_pairhmm* hmm);
+/*
+ * Replaces PairHmm::compute_log10_likelihoods (lib.rs:272-293) and
+ * PdPairHmm::compute_log10_likelihoods (pd.rs:69-95) for one region.  read_offsets has
+ * n_reads+1 entries into the five read arrays, hap_offsets n_haps+1 into hap_bases (and
+ * hap_flags, NULL unless the handle is PD); out holds n_reads*n_haps values.
+ */
+int fgkl_pairhmm_compute(fgkl_pairhmm* hmm,
+                         int64_t n_reads, const int64_t* read_offsets,
+                         const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
+                         const uint8_t* del_gop, const uint8_t* gcp,
+                         int64_t n_haps, const int64_t* hap_offsets,
+                         const uint8_t* hap_bases, const uint8_t* hap_flags,
+                         double* out, int64_t out_len);
+
+/*
+ * Batched engine (no reference counterpart: the reference computes one region per call on the
+ * calling thread, crates/pairhmm/src/lib.rs:377-397).  A batch holds many regions resident in
+ * device memory; region g owns reads region_reads[g]..region_reads[g+1] and haplotypes
+ * region_haps[g]..region_haps[g+1]; results land region after region, each region's block laid
+ * out as likelihoods[r * H + h].
+ */
+typedef struct fgkl_batch fgkl_batch;
+#define FGKL_MODE_FP32_RESCUE 0   /* FP32, FP64 recomputation of pairs below 1e-28 */
+#define FGKL_MODE_FP64 1          /* FP64 for every pair */
+#define FGKL_MODE_FP64_PER_REGION 2 /* FP64, one pair per wave, one synchronous call per region */
+#define FGKL_MODE_FP32_NO_RESCUE 3  /* FP32, underflowing pairs left NaN */
+
+typedef struct fgkl_run_stats {
+  double total_ms;        /* device time of the run, HIP events on the batch's stream */
+  double primary_ms;      /* sweep kernels of the first precision (-1 inside a graph) */
+  double rescue_ms;       /* FP64 rescue kernels (-1 inside a graph) */
+  int64_t rescued_pairs;  /* FP32 pairs recomputed in FP64 */
+} fgkl_run_stats;
+
+int fgkl_batch_create(fgkl_batch** out, int pd, int64_t n_regions,
+                      const int64_t* region_reads, const int64_t* region_haps,
+                      const int64_t* read_offsets, const uint8_t* bases, const uint8_t* quals,
+                      const uint8_t* ins_gop, const uint8_t* del_gop, const uint8_t* gcp,
+                      const int64_t* hap_offsets, const uint8_t* hap_bases,
+                      const uint8_t* hap_flags);
+int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_graph,
+                   fgkl_run_stats* stats);
+int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoods_len);
+int64_t fgkl_batch_pairs(const fgkl_batch* batch);
+void fgkl_batch_free(fgkl_batch* batch);
+
+/*
+ * Synthetic data (synthetic.rs:10-334 restated: Region::generate, PdRegion::generate) and the
+ * benchmark workloads.  A generated batch is read back with fgkl_synth_sizes/fgkl_synth_copy.
+ * workload: 0 one 32x8 region (151 bp reads, 300-350 bp haplotypes), 1 batched uneven regions,
+ * 2 underflow-heavy regions, 3 PD regions, 4 adjoint regions.
+ */
+typedef struct fgkl_synth fgkl_synth;
+fgkl_synth* fgkl_synth_region(uint64_t seed, int64_t num_reads, int64_t read_len,
+                              int64_t num_haps, int64_t hap_len, int pd);
+fgkl_synth* fgkl_synth_workload(int workload, uint64_t seed, int64_t n_regions);
+void fgkl_synth_sizes(const fgkl_synth* s, int64_t* n_regions, int64_t* n_reads,
+                      int64_t* read_bases, int64_t* n_haps, int64_t* hap_bases);
+void fgkl_synth_copy(const fgkl_synth* s, int64_t* region_reads, int64_t* region_haps,
+                     int64_t* read_offsets, uint8_t* bases, uint8_t* quals, uint8_t* ins_gop,
+                     uint8_t* del_gop, uint8_t* gcp, int64_t* hap_offsets, uint8_t* hap_bases,
+                     uint8_t* hap_flags);
+void fgkl_synth_free(fgkl_synth* s);
+/* SplitMix64 (synthetic.rs:19-40): n successive next_u64 draws from seed. */
+void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out);
+
+/* Checks the DPP lane shift the kernels rely on: out[i] receives 1000+i-1, out[0] = -1. */
+int fgkl_selftest_dpp(int32_t* out64);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FGKL_H */

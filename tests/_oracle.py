@@ -1,0 +1,113 @@
+"""Loader for the CPU oracle (oracle/liboracle.so): test infrastructure, used only as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+
+_o = None
+u8p = C.POINTER(C.c_uint8)
+f64p = C.POINTER(C.c_double)
+i64p = C.POINTER(C.c_int64)
+
+
+def oracle() -> C.CDLL:
+    global _o
+    if _o is None:
+        if not ORACLE_LIB.exists():
+            subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+        o = C.CDLL(str(ORACLE_LIB))
+        o.fgklo_error_prob.restype = C.c_double
+        o.fgklo_error_prob.argtypes = [C.c_uint8]
+        o.fgklo_match_to_match.restype = C.c_double
+        o.fgklo_match_to_match.argtypes = [C.c_uint8, C.c_uint8]
+        o.fgklo_transitions.restype = None
+        o.fgklo_transitions.argtypes = [C.c_uint8, C.c_uint8, C.c_uint8, f64p]
+        o.fgklo_priors.restype = None
+        o.fgklo_priors.argtypes = [C.c_uint8, f64p]
+        o.fgklo_approx_log10_sum_log10.restype = C.c_double
+        o.fgklo_approx_log10_sum_log10.argtypes = [C.c_double, C.c_double]
+        o.fgklo_initial_condition.restype = C.c_double
+        o.fgklo_initial_condition_log10.restype = C.c_double
+        o.fgklo_log10_likelihood.restype = C.c_double
+        o.fgklo_log10_likelihood.argtypes = [u8p, C.c_int64, u8p, u8p, u8p, u8p, u8p, C.c_int64]
+        o.fgklo_base_matches_pd.restype = C.c_int
+        o.fgklo_base_matches_pd.argtypes = [C.c_uint8, C.c_uint8]
+        o.fgklo_pd_log10_likelihood.restype = C.c_double
+        o.fgklo_pd_log10_likelihood.argtypes = [u8p, u8p, C.c_int64, u8p, u8p, u8p, u8p, u8p,
+                                                C.c_int64]
+        o.fgklo_batch.restype = None
+        o.fgklo_batch.argtypes = [C.c_int64, i64p, i64p, u8p, u8p, u8p, u8p, u8p, i64p, i64p,
+                                  u8p, u8p, f64p, C.c_int]
+        o.fgklo_log10_likelihood_cont.restype = C.c_double
+        o.fgklo_log10_likelihood_cont.argtypes = [u8p, C.c_int64, u8p, C.c_int64, f64p, f64p,
+                                                  f64p, C.c_double]
+        o.fgklo_log10_likelihood_grad.restype = C.c_double
+        o.fgklo_log10_likelihood_grad.argtypes = [u8p, C.c_int64, u8p, C.c_int64, f64p, f64p,
+                                                  f64p, C.c_double, f64p, f64p, f64p, f64p]
+        _o = o
+    return _o
+
+
+def _u8(a) -> np.ndarray:
+    if isinstance(a, (bytes, bytearray)):
+        return np.frombuffer(bytes(a), dtype=np.uint8).copy()
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _p(a, t=C.c_uint8):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def log10_likelihood(hap, read) -> float:
+    """reference.rs:20-64 via the oracle; `read` is a ReadRef."""
+    h = _u8(hap)
+    b, q, i, d, g = (_u8(x) for x in (read.bases, read.quals, read.ins_gop, read.del_gop, read.gcp))
+    return oracle().fgklo_log10_likelihood(_p(h), len(h), _p(b), _p(q), _p(i), _p(d), _p(g), len(b))
+
+
+def pd_log10_likelihood(hap, flags, read) -> float:
+    """pd_reference.rs:29-118 via the oracle."""
+    h, f = _u8(hap), _u8(flags)
+    b, q, i, d, g = (_u8(x) for x in (read.bases, read.quals, read.ins_gop, read.del_gop, read.gcp))
+    return oracle().fgklo_pd_log10_likelihood(_p(h), _p(f), len(h), _p(b), _p(q), _p(i), _p(d),
+                                              _p(g), len(b))
+
+
+def reference_all(reads, haps) -> np.ndarray:
+    return np.array([log10_likelihood(h, r) for r in reads for h in haps])
+
+
+def pd_reference_all(reads, haps) -> np.ndarray:
+    return np.array([pd_log10_likelihood(h.bases, h.flags, r) for r in reads for h in haps])
+
+
+def batch_reference(data, nthreads: int = 8) -> np.ndarray:
+    """Every pair of a BatchData, region by region (out[base_g + r * H + h])."""
+    out = np.zeros(data.n_pairs(), dtype=np.float64)
+    if len(out) == 0:
+        return out
+    oracle().fgklo_batch(
+        data.n_regions, _p(data.region_reads, C.c_int64), _p(data.read_offsets, C.c_int64),
+        _p(data.bases), _p(data.quals), _p(data.ins), _p(data.dels), _p(data.gcp),
+        _p(data.region_haps, C.c_int64), _p(data.hap_offsets, C.c_int64), _p(data.hap_bases),
+        _p(data.hap_flags) if data.pd else None, _p(out, C.c_double), nthreads)
+    return out
+
+
+def transitions(ins, dele, gcp) -> list[float]:
+    t = (C.c_double * 6)()
+    oracle().fgklo_transitions(ins, dele, gcp, t)
+    return list(t)
+
+
+def priors(q) -> tuple[float, float]:
+    p = (C.c_double * 2)()
+    oracle().fgklo_priors(q, p)
+    return p[0], p[1]

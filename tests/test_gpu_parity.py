@@ -1,0 +1,506 @@
+"""GPU parity: the HIP engine against the CPU oracle on the reference's own test cases.
+
+Restates crates/pairhmm/src/lib.rs:583-1097 and pd.rs:112-465 through the engine's C-ABI.
+Tolerances: FP64 mode within 1e-12 relative of the oracle (|d| <= 1e-12 * max(1, |ref|));
+FP32 mode (with FP64 rescue) within 1e-4 absolute in log10 on every pair; rescued pairs are
+FP64 results and are held to the FP64 bound where the reference holds them to 1e-9.
+"""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import pytest
+
+import fgkl_amd as F
+from fgkl_amd.synthetic import PdRegion, Region, Rng, workload
+from _oracle import batch_reference, pd_reference_all, reference_all
+
+pytestmark = pytest.mark.gpu
+
+FLOAT_TOL = 1e-4
+DOUBLE_REL = 1e-12
+
+
+def assert_close(actual, expected, tol, what=""):
+    actual, expected = np.asarray(actual), np.asarray(expected)
+    assert actual.shape == expected.shape
+    assert np.all(np.isfinite(actual)), f"{what}: non-finite results"
+    d = np.abs(actual - expected)
+    i = int(np.argmax(d))
+    assert d[i] <= tol, f"{what}: pair {i}: got {actual[i]!r}, expected {expected[i]!r} (diff {d[i]})"
+
+
+def assert_rel(actual, expected, rel=DOUBLE_REL, what=""):
+    actual, expected = np.asarray(actual), np.asarray(expected)
+    assert actual.shape == expected.shape
+    assert np.all(np.isfinite(actual)), f"{what}: non-finite results"
+    bound = rel * np.maximum(1.0, np.abs(expected))
+    d = np.abs(actual - expected)
+    i = int(np.argmax(d / bound))
+    assert d[i] <= bound[i], f"{what}: pair {i}: got {actual[i]!r}, expected {expected[i]!r} (diff {d[i]})"
+
+
+def compute(precision, reads, haps, fallback=True):
+    hmm = F.PairHmm(F.Config(precision=precision, double_fallback=fallback))
+    return hmm.compute_log10_likelihoods(reads, haps), hmm.fallback_pairs()
+
+
+def pd_compute(precision, reads, haps):
+    hmm = F.PdPairHmm(F.Config(precision=precision))
+    return hmm.compute_log10_likelihoods(reads, haps), hmm.fallback_pairs()
+
+
+def read(bases: bytes, q=30, i=45, d=45, g=10):
+    n = len(bases)
+    return F.ReadRef(bases, [q] * n, [i] * n, [d] * n, [g] * n)
+
+
+def foreign_reads(seed, n, length):
+    """lib.rs:879-893"""
+    rng = Rng(seed)
+    out = []
+    for _ in range(n):
+        bases = bytes(rng.base() for _ in range(length))
+        quals = [20 + rng.below(21) for _ in range(length)]
+        out.append(F.ReadRef(bases, quals, [45] * length, [45] * length, [10] * length))
+    return out
+
+
+def region():
+    r = Region.generate(11, 70, 120, 24, 300)
+    return r.region_reads_list(), r.region_haps_list()
+
+
+# ---- the engine's own lane layout --------------------------------------------------------------
+def test_dpp_wave_shift_moves_each_lane_to_the_next(gpu):
+    import ctypes as C
+    out = (C.c_int32 * 64)()
+    assert F._lib.lib().fgkl_selftest_dpp(out) == 0
+    assert out[0] == -1
+    assert [out[i] for i in range(1, 64)] == [1000 + i - 1 for i in range(1, 64)]
+
+
+# ---- lib.rs:620-645 ----------------------------------------------------------------------------
+def test_double_precision_matches_reference(gpu):
+    reads, haps = region()
+    out, _ = compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, reference_all(reads, haps), what="double")
+
+
+def test_single_precision_matches_reference_within_float_tolerance(gpu):
+    reads, haps = region()
+    out, _ = compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, reference_all(reads, haps), FLOAT_TOL, "float")
+
+
+def test_haplotype_order_does_not_change_results(gpu):  # lib.rs:647-663
+    reads, haps = region()
+    fwd, _ = compute(F.Precision.DOUBLE, reads, haps)
+    rev, _ = compute(F.Precision.DOUBLE, reads, haps[::-1])
+    n = len(haps)
+    a = fwd.reshape(-1, n)
+    b = rev.reshape(-1, n)[:, ::-1]
+    assert np.max(np.abs(a - b)) < 1e-11
+
+
+def test_prefix_sharing_handles_nested_and_repeated_prefixes(gpu):  # lib.rs:665-681
+    haps = [b"ABBC", b"AAAC", b"AC", b"AAAB", b"ABBB", b"AAAC", b"ABBCC"]
+    reads = Region.generate(3, 20, 6, 1, 8).region_reads_list()
+    expected = reference_all(reads, haps)
+    for p in (F.Precision.DOUBLE, F.Precision.FLOAT):
+        out, _ = compute(p, reads, haps)
+        if p is F.Precision.DOUBLE:
+            assert_rel(out, expected, what="nested double")
+        else:
+            assert_close(out, expected, FLOAT_TOL, "nested float")
+
+
+def test_haplotypes_of_different_lengths_share_prefixes_correctly(gpu):  # lib.rs:683-691
+    haps = [b"ACGTACGTAC", b"ACGTACGT", b"ACGTACGTACGTACGT", b"ACGT", b"A"]
+    reads = Region.generate(9, 40, 12, 1, 16).region_reads_list()
+    out, _ = compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, reference_all(reads, haps), what="lengths")
+
+
+def test_non_acgtn_bases_match_only_themselves(gpu):  # lib.rs:693-716
+    reads = [read(b"ACRTN")]
+    haps = [b"ACRTG", b"ACGTG", b"ACNTG", b"RRRRR"]
+    expected = reference_all(reads, haps)
+    out, _ = compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what="non-acgtn")
+    assert expected[0] > expected[1] and expected[2] >= expected[0]
+
+
+def test_float_underflow_falls_back_to_double(gpu):  # lib.rs:718-739
+    reads = [read(b"A" * 80, q=40)]
+    haps = [b"C" * 80]
+    expected = reference_all(reads, haps)
+    assert np.isfinite(expected[0]) and expected[0] < -50
+    out, fb = compute(F.Precision.FLOAT, reads, haps)
+    assert_rel(out, expected, what="underflow")
+    assert fb == 1
+
+
+def test_without_fallback_underflowing_pairs_are_nan(gpu):  # lib.rs:221-226
+    reads = [read(b"A" * 80, q=40)]
+    out, fb = compute(F.Precision.FLOAT, reads, [b"C" * 80], fallback=False)
+    assert np.isnan(out[0]) and fb == 1
+
+
+def test_reads_unrelated_to_every_haplotype_are_recomputed_in_double(gpu):  # lib.rs:895-914
+    _, haps = region()
+    reads = foreign_reads(5, 40, 120)
+    expected = reference_all(reads, haps)
+    assert np.all(expected < -64.0)
+    out, fb = compute(F.Precision.FLOAT, reads, haps)
+    assert_rel(out, expected, what="foreign")
+    assert fb == len(expected)
+
+
+def test_matching_and_mismatching_reads_of_varied_lengths_share_a_batch(gpu):  # lib.rs:916-938
+    r = Region.generate(11, 70, 120, 24, 300)
+    haps = r.region_haps_list()
+    owned = r.region_reads_list()[:20] + foreign_reads(6, 20, 120)
+    reads = []
+    for i, rd in enumerate(owned):
+        n = 30 + (i * 7) % 90
+        reads.append(F.ReadRef(rd.bases[:n], rd.quals[:n], rd.ins_gop[:n], rd.del_gop[:n], rd.gcp[:n]))
+    out, fb = compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, reference_all(reads, haps), FLOAT_TOL, "mixed")
+    assert fb > 0
+
+
+def test_a_read_gets_the_same_bits_whatever_shares_its_batch(gpu):  # lib.rs:940-955
+    r = Region.generate(11, 70, 120, 24, 300)
+    haps = r.region_haps_list()
+    alone = r.region_reads_list()[:3]
+    batched = alone + foreign_reads(7, 29, 120)
+    for p in (F.Precision.FLOAT, F.Precision.DOUBLE):
+        solo, _ = compute(p, alone, haps)
+        with_others, _ = compute(p, batched, haps)
+        assert np.array_equal(solo, with_others[: len(solo)]), p
+
+
+def test_a_read_gets_the_same_bits_alone_and_in_any_batch_size(gpu):  # lib.rs:957-985
+    r = Region.generate(5, 40, 60, 4, 100)
+    reads, haps = r.region_reads_list(), r.region_haps_list()
+    for p in (F.Precision.FLOAT, F.Precision.DOUBLE):
+        solo = [compute(p, [rd], haps)[0] for rd in reads]
+        for n in (8, 13, 16, 17, 31, 33, 40):
+            allr, _ = compute(p, reads[:n], haps)
+            for i in range(n):
+                assert np.array_equal(allr[i * len(haps):(i + 1) * len(haps)], solo[i]), (p, n, i)
+
+
+def test_a_read_matching_only_the_previous_haplotype_is_recomputed_in_double(gpu):  # lib.rs:1019-1056
+    rng = Rng(8)
+    reference = bytes(rng.base() for _ in range(300))
+    tail = bytes(rng.base() for _ in range(150))
+    h1 = reference
+    h2 = reference[:150] + tail
+    h3 = bytearray(reference)
+    h3[200] = ord("C") if h3[200] == ord("A") else ord("A")
+    haps = [h1, h2, bytes(h3)]
+    reads = []
+    for k in range(12):
+        src = reference if k % 2 == 0 else h2
+        start = 100 + (k * 13) % 80
+        reads.append(read(src[start:start + 100], q=35))
+    expected = reference_all(reads, haps)
+    assert np.any(expected < -64.0) and np.any(expected > -10.0)
+    out, fb = compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, "previous haplotype")
+    assert 0 < fb <= 24
+
+
+def test_more_reads_than_lanes_and_single_read_both_work(gpu):  # lib.rs:1058-1068
+    for seed, n in ((21, 1), (22, 37)):
+        r = Region.generate(seed, n, 50, 3, 60)
+        reads, haps = r.region_reads_list(), r.region_haps_list()
+        out, _ = compute(F.Precision.FLOAT, reads, haps)
+        assert_close(out, reference_all(reads, haps), FLOAT_TOL, f"{n} reads")
+
+
+def test_long_reads_far_below_single_precision(gpu):  # lib.rs:827-877 shapes
+    rng = Rng(99)
+    reference = bytes(rng.base() for _ in range(700))
+    haps = [reference]
+    for pos in (20, 40, 60):
+        h = bytearray(reference)
+        h[pos] = ord("C") if h[pos] == ord("A") else ord("A")
+        haps.append(bytes(h))
+    reads = [read(bytes(rng.base() for _ in range(n)), q=40) for n in (290, 360, 500)]
+    expected = reference_all(reads, haps)
+    assert np.any(expected < -400.0)
+    for p in (F.Precision.DOUBLE, F.Precision.FLOAT):
+        out, _ = compute(p, reads, haps)
+        assert_rel(out, expected, what=str(p))
+
+
+def test_random_small_haplotype_sets_match_reference(gpu):  # lib.rs:761-798
+    from fgkl_amd.synthetic import MASK64  # noqa: F401
+    rng = Rng(31)
+    letter = lambda: ord("A") if rng.chance(0.5) else ord("C")
+    for _ in range(60):
+        haps = [bytes(letter() for _ in range(6 + rng.below(30)))]
+        for _ in range(rng.below(12)):
+            src = haps[rng.below(len(haps))]
+            haps.append(_edited(rng, b"AC", src))
+        reads = []
+        for _ in range(1 + rng.below(20)):
+            n = 1 + rng.below(25)
+            bases = bytes(letter() for _ in range(n))
+            quals = [10 + rng.below(31) for _ in range(n)]
+            ins = [20 + rng.below(26) for _ in range(n)]
+            dele = [20 + rng.below(26) for _ in range(n)]
+            gcp = [1 + rng.below(10) for _ in range(n)]
+            reads.append(F.ReadRef(bases, quals, ins, dele, gcp))
+        expected = reference_all(reads, haps)
+        out, _ = compute(F.Precision.DOUBLE, reads, haps)
+        assert_rel(out, expected, what="random small")
+
+
+def _edited(rng, alphabet, base):  # synthetic.rs:84-98
+    hap = bytearray(base)
+    for _ in range(1 + rng.below(3)):
+        pos = rng.below(len(hap))
+        kind = rng.below(3)
+        if kind == 0:
+            hap[pos] = alphabet[rng.below(len(alphabet))]
+        elif kind == 1:
+            hap.insert(pos, alphabet[rng.below(len(alphabet))])
+        elif len(hap) > 1:
+            del hap[pos]
+    return bytes(hap)
+
+
+# ---- pd.rs:161-436 ----------------------------------------------------------------------------
+def pd_region():
+    r = PdRegion.generate(31, 70, 120, 24, 300)
+    return r.region_reads_list(), r.region_pd_haps_list()
+
+
+def test_pd_double_precision_matches_the_pd_reference(gpu):
+    reads, haps = pd_region()
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, pd_reference_all(reads, haps), what="pd double")
+
+
+def test_pd_single_precision_matches_within_float_tolerance(gpu):
+    reads, haps = pd_region()
+    out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, pd_reference_all(reads, haps), FLOAT_TOL, "pd float")
+
+
+def test_pd_flagged_alternates_and_deletions_change_the_result(gpu):
+    reads, haps = pd_region()
+    unflagged = [F.PdHaplotype(h.bases, [0] * len(h.bases)) for h in haps]
+    w, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    wo, _ = pd_compute(F.Precision.DOUBLE, reads, unflagged)
+    assert np.sum(np.abs(w - wo) > 1.0) >= 20
+    assert np.all(w >= wo)
+
+
+def test_pd_haplotypes_without_flags_agree_with_the_plain_kernel(gpu):
+    r = Region.generate(5, 40, 100, 8, 200)
+    reads, plain = r.region_reads_list(), r.region_haps_list()
+    pd = [F.PdHaplotype(h, [0] * len(h)) for h in plain]
+    for p, tol in ((F.Precision.DOUBLE, 1e-12), (F.Precision.FLOAT, FLOAT_TOL)):
+        a, _ = pd_compute(p, reads, pd)
+        b, _ = compute(p, reads, plain)
+        assert_close(a, b, tol, str(p))
+
+
+def test_pd_flags_on_the_last_columns_carry_their_state_into_the_next_row(gpu):
+    hap = b"ACGTTGCAAGGCTTAGGCTTACG"
+    n = len(hap)
+    ends_del = [0] * n; ends_del[n - 3] = F.DEL_START; ends_del[n - 1] = F.DEL_END
+    starts_del = [0] * n; starts_del[n - 1] = F.DEL_START
+    single_del = [0] * n; single_del[n - 1] = F.DEL_START | F.DEL_END
+    first_del = [0] * n; first_del[0] = F.DEL_START | F.DEL_END; first_del[1] = F.SNP | F.ALT_G
+    snp = [0] * n; snp[n - 1] = F.SNP | F.ALT_C | F.ALT_T
+    haps = [F.PdHaplotype(hap, f) for f in (ends_del, starts_del, single_del, first_del, snp)]
+    reads = [read(b"GTTGCAAGGCTTAG"), read(b"GGCTTAGGCTTACG"), read(b"GGCTTAGGCTTC"),
+             read(b"GGCTTAGGCTTA"), read(b"CGTTGCAAGG"), read(hap)]
+    expected = pd_reference_all(reads, haps)
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what="pd last columns double")
+    out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, "pd last columns float")
+
+
+def test_pd_adjacent_and_single_base_deletions_match_the_reference(gpu):
+    hap = b"ACGTTGCAAGGCTTAGGCTTACGATCGATTACA"
+    n = len(hap)
+    b2b = [0] * n
+    b2b[5] = F.DEL_START; b2b[7] = F.DEL_END; b2b[8] = F.DEL_START | F.DEL_END
+    b2b[9] = F.DEL_START; b2b[12] = F.DEL_END
+    singles = [0] * n
+    for j in range(3, n, 4):
+        singles[j] = F.DEL_START | F.DEL_END
+    snp_in_del = [0] * n
+    snp_in_del[10] = F.DEL_START; snp_in_del[11] = F.SNP | F.ALT_T
+    snp_in_del[13] = F.DEL_END | F.SNP | F.ALT_G
+    haps = [F.PdHaplotype(hap, f) for f in (b2b, singles, snp_in_del)]
+    reads = PdRegion.generate(9, 30, 25, 1, 33).region_reads_list()
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, pd_reference_all(reads, haps), what="pd adjacent")
+
+
+def test_pd_prefix_sharing_handles_nested_prefixes_lengths_and_flag_splits(gpu):
+    base = b"ACGTTGCAAGGCTTAGGCTTACGATCGATTACAGGT"
+    n = len(base)
+    plain = [0] * n
+    snp_early = plain.copy(); snp_early[5] = F.SNP | F.ALT_T
+    snp_late = plain.copy(); snp_late[30] = F.SNP | F.ALT_C
+    del_mid = plain.copy(); del_mid[12] = F.DEL_START; del_mid[15] = F.DEL_END
+    del_mid_snp = del_mid.copy(); del_mid_snp[25] = F.SNP | F.ALT_G
+    del_last = plain.copy(); del_last[n - 2] = F.DEL_START; del_last[n - 1] = F.DEL_END
+    short = base[:20]
+    short_del = [0] * 20; short_del[12] = F.DEL_START; short_del[15] = F.DEL_END
+    longer = base + b"ACGTAC"
+    longer_flags = [0] * len(longer); longer_flags[12] = F.DEL_START; longer_flags[15] = F.DEL_END
+    haps = [F.PdHaplotype(base, del_last), F.PdHaplotype(base, snp_late),
+            F.PdHaplotype(short, short_del), F.PdHaplotype(base, plain),
+            F.PdHaplotype(base, del_mid_snp), F.PdHaplotype(longer, longer_flags),
+            F.PdHaplotype(base, del_mid), F.PdHaplotype(base, snp_early),
+            F.PdHaplotype(base, plain)]
+    reads = PdRegion.generate(13, 40, 30, 1, 36).region_reads_list()
+    expected = pd_reference_all(reads, haps)
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what="pd sharing double")
+    out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, "pd sharing float")
+
+
+def test_pd_haplotype_order_does_not_change_results(gpu):
+    reads, haps = pd_region()
+    fwd, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    rev, _ = pd_compute(F.Precision.DOUBLE, reads, haps[::-1])
+    n = len(haps)
+    assert np.max(np.abs(fwd.reshape(-1, n) - rev.reshape(-1, n)[:, ::-1])) < 1e-11
+
+
+def test_pd_float_underflow_falls_back_to_double(gpu):
+    flags = [0] * 80
+    flags[10] = F.DEL_START
+    flags[20] = F.DEL_END
+    reads = [read(b"A" * 80, q=40)]
+    haps = [F.PdHaplotype(b"C" * 80, flags)]
+    expected = pd_reference_all(reads, haps)
+    assert expected[0] < -50
+    out, fb = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_rel(out, expected, what="pd underflow")
+    assert fb == 1
+
+
+def test_pd_more_reads_than_lanes_and_single_read_both_work(gpu):
+    for seed, n in ((41, 1), (42, 37), (43, 65)):
+        r = PdRegion.generate(seed, n, 50, 3, 60)
+        reads, haps = r.region_reads_list(), r.region_pd_haps_list()
+        out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+        assert_close(out, pd_reference_all(reads, haps), FLOAT_TOL, f"pd {n}")
+
+
+# ---- the JNI-shaped entry (FgklPairHmmTest.java:130-211) ---------------------------------------
+def _native_arrays(data):
+    ro = data.read_offsets.astype(np.int32)
+    ho = data.hap_offsets.astype(np.int32)
+    return ro, ho
+
+
+def test_native_entry_matches_reference_in_both_precisions(gpu):
+    data = Region.generate(1, 60, 120, 20, 300)
+    reads, haps = data.region_reads_list(), data.region_haps_list()
+    expected = reference_all(reads, haps)
+    ro, ho = _native_arrays(data)
+    for dp, tol in ((True, None), (False, FLOAT_TOL)):
+        out = np.zeros(len(expected))
+        F.compute_native(data.bases, data.quals, data.ins, data.dels, data.gcp, ro,
+                         data.hap_bases, ho, dp, out)
+        if tol is None:
+            assert_rel(out, expected, what="native double")
+        else:
+            assert_close(out, expected, tol, "native float")
+
+
+def test_concurrent_native_calls_give_identical_results(gpu):  # FgklPairHmmTest.java:193-211
+    data = Region.generate(8, 200, 150, 30, 400)
+    ro, ho = _native_arrays(data)
+    n = data.n_pairs()
+
+    def run():
+        out = np.zeros(n)
+        F.compute_native(data.bases, data.quals, data.ins, data.dels, data.gcp, ro,
+                         data.hap_bases, ho, False, out)
+        return out
+
+    expected = run()
+    results = [None] * 8
+    errors = []
+
+    def worker(i):
+        try:
+            results[i] = run()
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for r in results:
+        assert np.array_equal(r, expected)
+
+
+def test_pd_native_entry_matches_reference(gpu):
+    data = PdRegion.generate(31, 70, 120, 24, 300)
+    expected = pd_reference_all(data.region_reads_list(), data.region_pd_haps_list())
+    ro, ho = _native_arrays(data)
+    out = np.zeros(len(expected))
+    F.pd_compute_native(data.bases, data.quals, data.ins, data.dels, data.gcp, ro,
+                        data.hap_bases, data.hap_flags, ho, True, out)
+    assert_rel(out, expected, what="pd native")
+
+
+# ---- the batched engine ------------------------------------------------------------------------
+@pytest.mark.parametrize("wl,regions", [(0, 1), (1, 6), (2, 3), (3, 4), (4, 6)])
+def test_batch_workloads_match_the_oracle(gpu, wl, regions):
+    data = workload(wl, seed=1234 + wl, n_regions=regions)
+    expected = batch_reference(data)
+    b = F.Batch(data)
+    st = b.run(F.MODE_FP64, share=True)
+    assert_rel(b.results(), expected, what=f"workload {wl} fp64")
+    st = b.run(F.MODE_FP32_RESCUE, share=True)
+    assert_close(b.results(), expected, FLOAT_TOL, f"workload {wl} fp32")
+    if wl == 2:
+        assert st.rescued_pairs >= 0.25 * data.n_pairs()
+    b.run(F.MODE_FP64_PER_REGION)
+    assert_rel(b.results(), expected, what=f"workload {wl} per-region")
+    b.close()
+
+
+def test_batch_graph_replays_match_eager_runs(gpu):
+    data = workload(1, seed=99, n_regions=5)
+    b = F.Batch(data)
+    b.run(F.MODE_FP32_RESCUE, share=True, graph=False)
+    eager = b.results()
+    for _ in range(3):
+        b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+        assert np.array_equal(b.results(), eager)
+        b.run(F.MODE_FP64, share=True, graph=False)  # eager work between replays
+    b.close()
+
+
+def test_batch_prefix_sharing_only_moves_results_within_tolerance(gpu):
+    data = workload(1, seed=5, n_regions=4)
+    b = F.Batch(data)
+    b.run(F.MODE_FP64, share=True)
+    shared = b.results()
+    b.run(F.MODE_FP64, share=False)
+    unshared = b.results()
+    assert_rel(shared, unshared, what="share vs no-share")
+    b.close()
