@@ -1,0 +1,192 @@
+"""Benchmark: batched FP32 + FP64-rescue PairHMM GCUPS on one MI355X.
+
+A step is one pass of the engine over the whole batch (FP32 sweeps of every read against its
+region's haplotypes with prefix sharing, then the FP64 recomputation of every pair below 1e-28),
+replayed from one captured hipGraph with inputs resident in HBM.  Workload: 4096 uneven regions
+(R~U(1,400) reads of U(50,250) bp, H~U(2,16) haplotypes of read_max+U(50,400) bp), synthetic,
+seeded.  GCUPS counts nominal cells, sum over regions of (sum of read lengths) x (sum of
+haplotype lengths), as the reference counts them (synthetic.rs:166-170).
+
+Alongside: the plain per-call FP64 GPU engine on the same inputs (one pair per wave, one
+synchronous launch set per region), parity of every pair against it, the roofline of the
+dominant sweep kernel, and the CPU oracle timed on a bounded sample (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "PairHMM GCUPS, batched FP32+FP64-rescue vs plain FP64, 1 GPU, |Δlog10L|≤1e-4"
+FLOPS_PER_CELL = 12          # 4 mul + 4 FMA per cell (reference.rs:49-54)
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed) peak
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bucket_cells(data) -> np.ndarray:
+    """Nominal cells per rows-per-lane bucket (R = ceil(n / 64))."""
+    lens = np.diff(data.read_offsets)
+    rows = (lens + 63) // 64
+    region_of = np.repeat(np.arange(data.n_regions), np.diff(data.region_reads))
+    hap_sum = np.diff(data.hap_offsets[data.region_haps])
+    cells = lens.astype(np.int64) * hap_sum[region_of].astype(np.int64)
+    out = np.zeros(8, dtype=np.int64)
+    for r in range(1, 9):
+        out[r - 1] = int(cells[rows == r].sum())
+    return out
+
+
+def cpu_baseline(data, max_regions: int, threads: int):
+    """The CPU oracle (test infrastructure, the checker) timed on the first regions."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _oracle import batch_reference
+    from fgkl_amd.synthetic import BatchData
+
+    g = min(max_regions, data.n_regions)
+    r1, h1 = int(data.region_reads[g]), int(data.region_haps[g])
+    rb, hb = int(data.read_offsets[r1]), int(data.hap_offsets[h1])
+    sub = BatchData(g, data.region_reads[: g + 1].copy(), data.region_haps[: g + 1].copy(),
+                    data.read_offsets[: r1 + 1].copy(), data.bases[:rb], data.quals[:rb],
+                    data.ins[:rb], data.dels[:rb], data.gcp[:rb], data.hap_offsets[: h1 + 1].copy(),
+                    data.hap_bases[:hb], data.hap_flags[:hb], data.pd)
+    t0 = time.perf_counter()
+    ref = batch_reference(sub, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return sub, ref, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--regions", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--cpu-regions", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 per-call run")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: every GPU box of this build has one GPU, and the "
+              "path shards by region with no exchange (see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    import fgkl_amd as F
+    from fgkl_amd.synthetic import workload
+
+    t0 = time.perf_counter()
+    data = workload(1, seed=args.seed, n_regions=args.regions)
+    cells = data.cells()
+    n_pairs = data.n_pairs()
+    log(f"generated {data.n_regions} regions, {data.n_reads} reads, {n_pairs} pairs, "
+        f"{cells:.3e} cells in {time.perf_counter() - t0:.1f}s")
+
+    t0 = time.perf_counter()
+    batch = F.Batch(data)
+    prep_s = time.perf_counter() - t0
+    log(f"planned + uploaded in {prep_s:.2f}s")
+
+    # Eager run: per-bucket kernel times (HIP events on the engine's stream) and results.
+    st = batch.run(F.MODE_FP32_RESCUE, share=True, graph=False)
+    eager_ms = st.total_ms
+    bucket_ms = list(st.bucket_ms)
+    primary_ms, rescue_ms, rescued = st.primary_ms, st.rescue_ms, int(st.rescued_pairs)
+    fp32_res = batch.results()
+    log(f"eager: total {eager_ms:.1f} ms, fp32 {primary_ms:.1f} ms, fp64 rescue {rescue_ms:.1f} ms, "
+        f"rescued {rescued}")
+
+    for _ in range(args.warmup):
+        batch.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+    import ctypes as C
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = batch.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    gcups = cells * args.steps / elapsed / 1e9
+    graph_res = batch.results()
+    log(f"graph steps: {ms_per_step:.1f} ms/step, {gcups:.1f} GCUPS")
+
+    result = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 rescue)", "data": "synthetic",
+        "config": {"workload": "batched uneven PairHMM, 4096 regions: R~U(1,400) reads of "
+                               "U(50,250) bp, H~U(2,16) haplotypes of read_max+U(50,400) bp",
+                   "regions": data.n_regions, "reads": data.n_reads, "pairs": n_pairs,
+                   "cells": cells, "seed": args.seed, "prefix_sharing": True, "graph": True},
+    }
+
+    # Roofline: the dominant sweep kernel (the bucket with the most kernel time).
+    bc = bucket_cells(data)
+    dom = int(np.argmax(bucket_ms))
+    achieved = FLOPS_PER_CELL * bc[dom] / (bucket_ms[dom] * 1e-3) / 1e12
+    result["roofline"] = {
+        "bound": "valu", "kernel": f"sweep_kernel<float, R={dom + 1}, plain>",
+        "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+        "algorithmic_flops_per_launch": int(FLOPS_PER_CELL * bc[dom]),
+        "launch_ms": round(bucket_ms[dom], 3),
+    }
+    result["kernels"] = {
+        "fp32_sweep_ms": round(primary_ms, 3), "fp64_rescue_ms": round(rescue_ms, 3),
+        "bucket_ms": [round(x, 3) for x in bucket_ms],
+        "bucket_cells": [int(x) for x in bc],
+        "fp32_sweep_gcups": round(cells / (primary_ms * 1e-3) / 1e9, 2),
+    }
+    result["rescue"] = {"pairs": rescued, "fraction": round(rescued / n_pairs, 6),
+                        "fp64_time_share": round(rescue_ms / eager_ms, 4) if eager_ms else None}
+    assert np.array_equal(graph_res, fp32_res), "graph replay differs from the eager run"
+
+    if not args.no_baseline:
+        st = batch.run(F.MODE_FP64_PER_REGION)
+        base_ms = st.total_ms
+        fp64 = batch.results()
+        base_gcups = cells / (base_ms * 1e-3) / 1e9
+        d = np.abs(fp32_res - fp64)
+        result["baseline_fp64_per_call"] = {
+            "gcups": round(base_gcups, 2), "ms": round(base_ms, 1),
+            "definition": "FP64 engine, one pair per wave, no prefix sharing, one synchronous "
+                          "launch set per region (as HaplotypeCaller calls per region)"}
+        result["speedup_vs_fp64_per_call"] = round(gcups / base_gcups, 2)
+        result["parity"] = {"max_abs_dlog10L_vs_fp64": float(d.max()),
+                            "pairs_over_1e-4": int((d > 1e-4).sum()),
+                            "nonfinite": int((~np.isfinite(fp32_res)).sum())}
+        log(f"fp64 per-call: {base_ms:.1f} ms, {base_gcups:.1f} GCUPS; max |d| {d.max():.3g}")
+
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        sub, ref, dt = cpu_baseline(data, args.cpu_regions, threads)
+        sub_cells = sub.cells()
+        n_sub = sub.n_pairs()
+        dgpu = np.abs(fp32_res[:n_sub] - ref)
+        result["cpu_baseline"] = {
+            "value": round(sub_cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"first {sub.n_regions} regions of the same workload ({n_pairs and n_sub} "
+                      f"pairs, {sub_cells:.3e} cells), scalar FP64 oracle, one region per thread",
+            "seconds": round(dt, 2)}
+        result.setdefault("parity", {})["max_abs_dlog10L_vs_oracle_sample"] = float(dgpu.max())
+        log(f"cpu oracle: {dt:.1f}s on {threads} threads; gpu vs oracle max |d| {dgpu.max():.3g}")
+
+    print(json.dumps(result), flush=True)
+    batch.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

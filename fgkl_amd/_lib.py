@@ -23,6 +23,7 @@ class RunStats(C.Structure):
         ("primary_ms", C.c_double),
         ("rescue_ms", C.c_double),
         ("rescued_pairs", C.c_int64),
+        ("bucket_ms", C.c_double * 8),
     ]
 
 

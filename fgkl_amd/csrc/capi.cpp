@@ -229,6 +229,7 @@ int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_grap
     stats->primary_ms = st.primary_ms;
     stats->rescue_ms = st.rescue_ms;
     stats->rescued_pairs = st.rescued_pairs;
+    for (int i = 0; i < 8; ++i) stats->bucket_ms[i] = st.bucket_ms[i];
   }
   return rc;
 }

@@ -592,6 +592,7 @@ struct DeviceBatch::Impl {
   int32_t* region_counters = nullptr;
   int64_t n_region_launches = 0;
   hipEvent_t ev[4] = {};
+  hipEvent_t bev[2 * kMaxRowsPerLane] = {};  // around each primary bucket launch
   hipGraphExec_t graph[8] = {};  // by mode * 2 + share
   int lds_words = 0;
 
@@ -599,6 +600,8 @@ struct DeviceBatch::Impl {
     for (auto& g : graph)
       if (g) (void)hipGraphExecDestroy(g);
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : bev)
       if (e) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -658,28 +661,38 @@ struct DeviceBatch::Impl {
     return int64_t(plan.max_slots) * nv * 64;
   }
 
-  int enqueue(int mode, int share);
+  int enqueue(int mode, int share, bool timed);
 };
 
-int DeviceBatch::Impl::enqueue(int mode, int share) {
+int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
   // Reset the work-queue heads and rescue counts with a kernel (graph-safe).
   hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, stream, counters, 24, 0);
   HIP_TRY(hipGetLastError());
-  if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue) {
-    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
-      if (!bucket_n[R - 1]) continue;
-      SweepArgs a = base_args();
-      a.items = bucket_items[R - 1];
-      a.n_items = bucket_n[R - 1];
-      a.counter = counters + (R - 1);
-      a.share = share;
+  const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
+  // The primary sweep, one launch per rows-per-lane bucket, each bracketed by events when the
+  // run is not captured into a graph.
+  for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+    if (!bucket_n[R - 1]) continue;
+    SweepArgs a = base_args();
+    a.items = bucket_items[R - 1];
+    a.n_items = bucket_n[R - 1];
+    a.counter = counters + (R - 1);
+    a.share = share;
+    if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1)], stream));
+    if (fp32) {
       a.scratch_wave_stride = scratch_stride<float>(R);
       a.rescue = 1;
       a.rescue_pairs = rescue_pairs[R - 1];
       a.rescue_count = counters + 16 + (R - 1);
       if (int rc = launch<float>(R, a, bucket_n[R - 1])) return rc;
+    } else {
+      a.scratch_wave_stride = scratch_stride<double>(R);
+      if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
     }
-    HIP_TRY(hipEventRecord(ev[1], stream));
+    if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1) + 1], stream));
+  }
+  if (timed) HIP_TRY(hipEventRecord(ev[1], stream));
+  if (fp32) {
     for (int R = 1; R <= kMaxRowsPerLane && mode == kModeFp32Rescue; ++R) {
       if (!bucket_n[R - 1]) continue;
       SweepArgs a = base_args();
@@ -693,18 +706,6 @@ int DeviceBatch::Impl::enqueue(int mode, int share) {
       // The grid is sized for the worst case; waves beyond the rescued count exit at once.
       if (int rc = launch<double>(R, a, bucket_pairs[R - 1])) return rc;
     }
-  } else {
-    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
-      if (!bucket_n[R - 1]) continue;
-      SweepArgs a = base_args();
-      a.items = bucket_items[R - 1];
-      a.n_items = bucket_n[R - 1];
-      a.counter = counters + (R - 1);
-      a.share = share;
-      a.scratch_wave_stride = scratch_stride<double>(R);
-      if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
-    }
-    HIP_TRY(hipEventRecord(ev[1], stream));
   }
   return kOk;
 }
@@ -720,6 +721,7 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
   d.pd = in.pd;
   HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
   for (auto& e : d.ev) HIP_TRY(hipEventCreate(&e));
+  for (auto& e : d.bev) HIP_TRY(hipEventCreate(&e));
   const HostPlan& p = d.plan;
   const int64_t NR = p.n_reads, NH = p.n_haps, G = int64_t(p.region_hap0.size());
   const size_t rb = size_t(in.read_off[NR]);
@@ -797,7 +799,7 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
     if (!ge) {
       hipGraph_t graph = nullptr;
       HIP_TRY(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
-      const int rc = d.enqueue(mode, share);
+      const int rc = d.enqueue(mode, share, false);
       hipGraph_t captured = nullptr;
       const hipError_t e = hipStreamEndCapture(d.stream, &captured);
       if (rc) return rc;
@@ -808,7 +810,7 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
     }
     HIP_TRY(hipGraphLaunch(ge, d.stream));
   } else {
-    if (int rc = d.enqueue(mode, share)) return rc;
+    if (int rc = d.enqueue(mode, share, true)) return rc;
   }
   HIP_TRY(hipEventRecord(d.ev[2], d.stream));
   HIP_TRY(hipStreamSynchronize(d.stream));
@@ -828,6 +830,12 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
       HIP_TRY(hipEventElapsedTime(&first, d.ev[0], d.ev[1]));
       stats->primary_ms = first;
       stats->rescue_ms = total - first;
+      for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+        if (!d.bucket_n[R - 1]) continue;
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, d.bev[2 * (R - 1)], d.bev[2 * (R - 1) + 1]));
+        stats->bucket_ms[R - 1] = ms;
+      }
     } else {
       stats->primary_ms = -1.0;
       stats->rescue_ms = -1.0;

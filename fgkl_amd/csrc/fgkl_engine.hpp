@@ -20,6 +20,7 @@ struct RunStats {
   double primary_ms = 0;   // the FP32 (or FP64) sweep kernels; -1 when run as one graph
   double rescue_ms = 0;    // the FP64 rescue kernels; -1 when run as one graph
   int64_t rescued_pairs = 0;
+  double bucket_ms[kMaxRowsPerLane] = {};  // primary sweep launch of each rows-per-lane bucket
 };
 
 class DeviceBatch {
