@@ -74,10 +74,23 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--regions", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--cpu-regions", type=int, default=24)
+    ap.add_argument("--cpu-regions", type=int, default=16, help="regions in the CPU probe")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 per-call run")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: every GPU box of this build has one GPU, and the "
               "path shards by region with no exchange (see DESIGN.md); run with --gpus 1",
@@ -170,7 +183,12 @@ def main() -> int:
 
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
+        # A probe sizes the sample to about --cpu-seconds of oracle work.
         sub, ref, dt = cpu_baseline(data, args.cpu_regions, threads)
+        per_region = dt / max(sub.n_regions, 1)
+        want = int(min(data.n_regions, max(sub.n_regions, args.cpu_seconds / max(per_region, 1e-6))))
+        if want > sub.n_regions:
+            sub, ref, dt = cpu_baseline(data, want, threads)
         sub_cells = sub.cells()
         n_sub = sub.n_pairs()
         dgpu = np.abs(fp32_res[:n_sub] - ref)
