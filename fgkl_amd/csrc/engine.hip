@@ -1,16 +1,15 @@
 // The MI355X (gfx950) PairHMM / PD-HMM engine: device kernels and the batch runner.
 //
 // Kernel layout (one work item per wave64):
-//  * A read of n bases is laid out down the lanes: lane l owns the R consecutive DP rows
-//    R*l+1 .. R*l+R (R = ceil(n/64), 1..8), L = ceil(n/R) lanes are live.  If n is not a multiple
-//    of R the first R*L-n rows of lane 0 are identity rows that reproduce DP row 0 exactly
-//    (prior 0, M->I 0, I->I 0, M->D 0, D->D 1, left value (0, 0, C/m)), so the read's last row is
-//    always the last register of lane L-1.
+//  * A read of n bases is laid out down the lanes: lane l owns R consecutive rows, with
+//    R = ceil((n+1)/64) (1..8) and L = ceil((n+1)/R) live lanes.  The first R*L-n >= 1 rows of
+//    lane 0 are identity rows that reproduce DP row 0 exactly (prior 0, M->I 0, I->I 0, M->D 0,
+//    D->D 1, left value (0, 0, C/m)), so the read's last row is always the last register of lane
+//    L-1 and lane 0 never needs the row above it.
 //  * The haplotype is swept along anti-diagonals: at step t lane l computes column t-l+1 for its
 //    R rows.  The values of the row above a lane's first row arrive from lane l-1 with a DPP
-//    wave_shr:1 move of the registers that lane computed one step earlier; lane 0 receives DP
-//    row 0 (0, 0, C/m) through the DPP "old" operand.  The haplotype column word travels the
-//    same way, lane 0 reading it from an LDS copy of the haplotype.
+//    wave_shr:1 move of the registers that lane computed one step earlier.  The haplotype column
+//    word travels the same way, lane 0 reading it from an LDS copy of the haplotype.
 //  * A work item is one read against every sorted haplotype of its region.  A haplotype starts
 //    at the column where it diverges from the previous sorted haplotype, from a snapshot of that
 //    column that an earlier sweep wrote into the wave's HBM scratch slot, rescaled by the ratio
@@ -174,157 +173,286 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
 // Number of scratch values a lane keeps per snapshot slot.
 template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 7 : 3 * R + 4; }
 
-// One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
-// column start+1+t (zero beyond W).  Returns the lane's raw scaled sum over the read's last row
-// (meaningful in lane L-1).
+// DPP wave_shr:1 with lane 0 receiving zero (bound_ctrl): no "old" operand to materialise.
+__device__ __forceinline__ float shr1z(float src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, src), 0x138,
+                                                            0xF, 0xF, true));
+}
+__device__ __forceinline__ double shr1z(double src) {
+  const uint64_t s = __builtin_bit_cast(uint64_t, src);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_mov_dpp(int(uint32_t(s)), 0x138, 0xF, 0xF, true));
+  const uint32_t hi =
+      uint32_t(__builtin_amdgcn_mov_dpp(int(uint32_t(s >> 32)), 0x138, 0xF, 0xF, true));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// The values of one DP row in one column: match, insertion, deletion and (PD) branch values.
+template <typename T> struct Cell {
+  T m, x, y, bm, bx, by;
+};
+
+// What a lane carries from one anti-diagonal step to the next: its R rows' values in the column
+// it computed last (the left neighbours of its next column), its running last-row sum, and the
+// word of the column it is on.  The row above its first row (diagonal and up neighbours) lives
+// in Cell values the sweep passes in.
+template <typename T, int R, bool PD> struct Lane {
+  T m[R], x[R], y[R];
+  T bm[PD ? R : 1], bx[PD ? R : 1], by[PD ? R : 1];
+  T acc;
+  uint32_t w;
+};
+
+// The row above a lane's first row in the column it computes next: lane l-1's last row, moved
+// with DPP before either lane updates.  Lane 0 receives zeros; its first row is always an
+// identity row, which ignores them.
 template <typename T, int R, bool PD>
+__device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u) {
+  u.m = shr1z(s.m[R - 1]);
+  u.x = shr1z(s.x[R - 1]);
+  u.y = shr1z(s.y[R - 1]);
+  if constexpr (PD) {
+    u.bm = shr1z(s.bm[R - 1]);
+    u.bx = shr1z(s.bx[R - 1]);
+    u.by = shr1z(s.by[R - 1]);
+  }
+}
+
+// The R cells of one column.  Rows are updated in place: match, deletion and branch values
+// bottom-up (each row still reads the old column of the row above), then insertion values
+// top-down (each row reads the new column of the row above), so no value is copied.
+template <typename T, int R, bool PD>
+__device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
+                                      const Cell<T>& d, const Cell<T>& u) {
+  const uint32_t w = s.w;
+  const uint32_t hb = w & 0xFFu;
+  const bool hN = hb == uint32_t('N');
+  uint32_t st[PD ? R : 1], de[PD ? R : 1];
+  if constexpr (PD) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      if (rw.kind[k] == 2u) {
+        st[k] = kInsideDel;  // identity rows reproduce row 0, branch values included
+        de[k] = 0u;
+      } else {
+        st[k] = rw.kind[k] == 1u ? ((w >> 12) & 3u) : ((w >> 14) & 3u);
+        de[k] = (w >> 16) & 1u;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = R - 1; k >= 0; --k) {
+    const T pdm = k ? s.m[k - 1] : d.m;
+    const T pdx = k ? s.x[k - 1] : d.x;
+    const T pdy = k ? s.y[k - 1] : d.y;
+    bool match = hN || hb == rw.rb[k];
+    if constexpr (PD) match = match || (((w >> 8) & rw.ralt[k]) != 0u);
+    const T prior = match ? rw.pm[k] : rw.px[k];
+    if constexpr (!PD) {
+      // kernel.rs:1011-1013
+      const T Y = fma(s.y[k], rw.dd[k], s.m[k] * rw.md[k]);
+      s.m[k] = prior * fma(pdy, rw.im[k], fma(pdx, rw.im[k], pdm * rw.mm[k]));
+      s.y[k] = Y;
+    } else {
+      // pd_reference.rs:62-94 for the column's deletion state.
+      const bool A = st[k] == kAfterDel, I = st[k] == kInsideDel, N = st[k] == kNormal;
+      const T pdbm = k ? s.bm[k - 1] : d.bm;
+      const T pdbx = k ? s.bx[k - 1] : d.bx;
+      const T pdby = k ? s.by[k - 1] : d.by;
+      const T dme = A ? fmax(pdbm, pdm) : pdm;
+      const T dxe = A ? fmax(pdbx, pdx) : pdx;
+      const T dye = A ? fmax(pdby, pdy) : pdy;
+      const T lme = A ? fmax(s.bm[k], s.m[k]) : s.m[k];
+      const T lxe = A ? fmax(s.bx[k], s.x[k]) : s.x[k];
+      const T lye = A ? fmax(s.by[k], s.y[k]) : s.y[k];
+      const T Y = fma(lye, rw.dd[k], lme * rw.md[k]);
+      const T BM = N ? s.m[k] : (I ? s.bm[k] : lme);
+      const T BX = N ? s.x[k] : (I ? s.bx[k] : lxe);
+      const T BY = N ? s.y[k] : (I ? s.by[k] : lye);
+      s.m[k] = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
+      s.y[k] = Y;
+      s.bm[k] = BM;
+      s.bx[k] = BX;
+      s.by[k] = BY;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const T pum = k ? s.m[k - 1] : u.m;
+    const T pux = k ? s.x[k - 1] : u.x;
+    if constexpr (!PD) {
+      s.x[k] = fma(pux, rw.ii[k], pum * rw.mi[k]);
+    } else {
+      // pd_reference.rs:95-100
+      const T pubm = k ? s.bm[k - 1] : u.bm;
+      const T pubx = k ? s.bx[k - 1] : u.bx;
+      const T ume = de[k] ? fmax(pubm, pum) : pum;
+      const T uxe = de[k] ? fmax(pubx, pux) : pux;
+      s.x[k] = fma(uxe, rw.ii[k], ume * rw.mi[k]);
+    }
+  }
+  s.acc += s.m[R - 1] + s.x[R - 1];
+}
+
+// Writes a lane's state into snapshot slot memory (layout [value][lane]); `u` is the row above
+// its first row in the column just computed.
+template <typename T, int R, bool PD>
+__device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Cell<T>& u, T* d) {
+  constexpr int NV = snap_values<R, PD>();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    d[(3 * k + 0) * 64] = s.m[k];
+    d[(3 * k + 1) * 64] = s.x[k];
+    d[(3 * k + 2) * 64] = s.y[k];
+    if constexpr (PD) {
+      d[(3 * R + 3 * k + 0) * 64] = s.bm[k];
+      d[(3 * R + 3 * k + 1) * 64] = s.bx[k];
+      d[(3 * R + 3 * k + 2) * 64] = s.by[k];
+    }
+  }
+  constexpr int U = PD ? 6 * R : 3 * R;
+  d[(U + 0) * 64] = u.m;
+  d[(U + 1) * 64] = u.x;
+  d[(U + 2) * 64] = u.y;
+  if constexpr (PD) {
+    d[(U + 3) * 64] = u.bm;
+    d[(U + 4) * 64] = u.bx;
+    d[(U + 5) * 64] = u.by;
+  }
+  d[(NV - 1) * 64] = s.acc;
+}
+
+// Snapshot windows of one sweep: lane wl reaches snapshot column q at step q - start - 1 + wl,
+// so the window of q is open for L steps.  Windows open and close in column order.
+struct SnapCursor {
+  const SnapEntry* snaps;
+  int n, lo, t0_lo, start, L;
+  __device__ void init(const SnapEntry* s, int count, int st, int lanes) {
+    snaps = s;
+    n = count;
+    lo = 0;
+    start = st;
+    L = lanes;
+    t0_lo = n ? __builtin_amdgcn_readfirstlane(snaps[0].col) - start - 1 : 0x7fffffff;
+  }
+  template <typename T, int R, bool PD>
+  __device__ __forceinline__ void visit(int t, int lane, const Lane<T, R, PD>& s,
+                                        const Cell<T>& u, T* scratch) {
+    if (t < t0_lo) return;
+    constexpr int NV = snap_values<R, PD>();
+    for (int i = lo; i < n; ++i) {
+      const int t0 = __builtin_amdgcn_readfirstlane(snaps[i].col) - start - 1;
+      if (t < t0) break;
+      const int wl = t - t0;
+      if (wl >= L) {  // closed: every lane has passed this column
+        lo = i + 1;
+        t0_lo = lo < n ? __builtin_amdgcn_readfirstlane(snaps[lo].col) - start - 1 : 0x7fffffff;
+        continue;
+      }
+      if (lane == wl) {
+        const int slot = __builtin_amdgcn_readfirstlane(snaps[i].slot);
+        store_snapshot<T, R, PD>(s, u, scratch + size_t(slot) * NV * 64 + lane);
+      }
+    }
+  }
+};
+
+// One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
+// column start+1+t (zero beyond W, 16-byte aligned).  Returns the lane's raw scaled sum over the
+// read's last row (meaningful in lane L-1).
+template <typename T, int R, bool PD, bool SNAP>
 __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
                    const T* snap_in, T scale, const SnapEntry* snaps, int nsnap, int start,
                    T* scratch) {
   constexpr int NV = snap_values<R, PD>();
-  T lm[R], lx[R], ly[R];
-  T lbm[PD ? R : 1], lbx[PD ? R : 1], lby[PD ? R : 1];
-  T dm, dx, dy, dbm = T(0), dbx = T(0), dby = T(0);
-  T acc;
+  Lane<T, R, PD> s;
+  Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
   if (snap_in) {
-    const T* s = snap_in + lane;
+    const T* p = snap_in + lane;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      lm[k] = s[(3 * k + 0) * 64] * scale;
-      lx[k] = s[(3 * k + 1) * 64] * scale;
-      ly[k] = s[(3 * k + 2) * 64] * scale;
+      s.m[k] = p[(3 * k + 0) * 64] * scale;
+      s.x[k] = p[(3 * k + 1) * 64] * scale;
+      s.y[k] = p[(3 * k + 2) * 64] * scale;
       if constexpr (PD) {
-        lbm[k] = s[(3 * R + 3 * k + 0) * 64] * scale;
-        lbx[k] = s[(3 * R + 3 * k + 1) * 64] * scale;
-        lby[k] = s[(3 * R + 3 * k + 2) * 64] * scale;
+        s.bm[k] = p[(3 * R + 3 * k + 0) * 64] * scale;
+        s.bx[k] = p[(3 * R + 3 * k + 1) * 64] * scale;
+        s.by[k] = p[(3 * R + 3 * k + 2) * 64] * scale;
       }
     }
     constexpr int U = PD ? 6 * R : 3 * R;
-    dm = s[(U + 0) * 64] * scale;
-    dx = s[(U + 1) * 64] * scale;
-    dy = s[(U + 2) * 64] * scale;
+    P.m = p[(U + 0) * 64] * scale;
+    P.x = p[(U + 1) * 64] * scale;
+    P.y = p[(U + 2) * 64] * scale;
     if constexpr (PD) {
-      dbm = s[(U + 3) * 64] * scale;
-      dbx = s[(U + 4) * 64] * scale;
-      dby = s[(U + 5) * 64] * scale;
+      P.bm = p[(U + 3) * 64] * scale;
+      P.bx = p[(U + 4) * 64] * scale;
+      P.by = p[(U + 5) * 64] * scale;
     }
-    acc = s[(NV - 1) * 64] * scale;
+    s.acc = p[(NV - 1) * 64] * scale;
   } else {
+    // Column 0: zeros below row 0; identity rows carry row 0's deletion value C/m.
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      lm[k] = T(0);
-      lx[k] = T(0);
-      ly[k] = (rw.kind[k] == 2u && lane < L) ? init : T(0);
-      if constexpr (PD) { lbm[k] = T(0); lbx[k] = T(0); lby[k] = T(0); }
+      s.m[k] = T(0);
+      s.x[k] = T(0);
+      s.y[k] = (rw.kind[k] == 2u && lane < L) ? init : T(0);
+      if constexpr (PD) {
+        s.bm[k] = T(0);
+        s.bx[k] = T(0);
+        s.by[k] = T(0);
+      }
     }
-    dm = T(0);
-    dx = T(0);
-    dy = lane == 0 ? init : T(0);
-    acc = T(0);
+    s.acc = T(0);
   }
+  s.w = 0;
 
-  uint32_t w = 0;
-  int s_lo = 0;
+  SnapCursor cur;
+  if constexpr (SNAP) cur.init(snaps, nsnap, start, L);
   const int steps = W + L - 1;
-  uint32_t w_next = lw[0];
-  for (int t = 0; t < steps; ++t) {
-    const uint32_t w0 = w_next;
-    w_next = lw[t + 1];
-    w = shr1(w0, w);
-    const T um = shr1(T(0), lm[R - 1]);
-    const T ux = shr1(T(0), lx[R - 1]);
-    const T uy = shr1(init, ly[R - 1]);
-    T ubm = T(0), ubx = T(0), uby = T(0);
-    if constexpr (PD) {
-      ubm = shr1(T(0), lbm[R - 1]);
-      ubx = shr1(T(0), lbx[R - 1]);
-      uby = shr1(T(0), lby[R - 1]);
-    }
+  // Ramp: lane l starts at step l and must keep its column-`start` state until then.  Rounded
+  // up to four steps so the steady loop reads aligned groups of four column words.
+  const int ramp = min(steps, (L - 1 + 3) & ~3);
+  int t = 0;
+  Cell<T> Q;
+  for (; t < ramp; ++t) {
+    s.w = shr1(lw[t], s.w);
+    shift_up(s, Q);
     if (t >= lane) {
-      const uint32_t hb = w & 0xFFu;
-      const bool hN = hb == uint32_t('N');
-      T pdm = dm, pdx = dx, pdy = dy, pum = um, pux = ux;
-      T pdbm = dbm, pdbx = dbx, pdby = dby, pubm = ubm, pubx = ubx;
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        bool match = hN || hb == rw.rb[k];
-        if constexpr (PD) match = match || (((w >> 8) & rw.ralt[k]) != 0u);
-        const T prior = match ? rw.pm[k] : rw.px[k];
-        if constexpr (!PD) {
-          // kernel.rs:1011-1013
-          const T M = prior * fma(pdy, rw.im[k], fma(pdx, rw.im[k], pdm * rw.mm[k]));
-          const T X = fma(pux, rw.ii[k], pum * rw.mi[k]);
-          const T Y = fma(ly[k], rw.dd[k], lm[k] * rw.md[k]);
-          pdm = lm[k]; pdx = lx[k]; pdy = ly[k];
-          lm[k] = M; lx[k] = X; ly[k] = Y;
-          pum = M; pux = X;
-        } else {
-          // pd_reference.rs:62-100 with the column's deletion state.
-          uint32_t st, de;
-          if (rw.kind[k] == 2u) { st = kInsideDel; de = 0u; }
-          else {
-            st = rw.kind[k] == 1u ? ((w >> 12) & 3u) : ((w >> 14) & 3u);
-            de = (w >> 16) & 1u;
-          }
-          const bool A = st == kAfterDel, I = st == kInsideDel, N = st == kNormal;
-          const T dme = A ? fmax(pdbm, pdm) : pdm;
-          const T dxe = A ? fmax(pdbx, pdx) : pdx;
-          const T dye = A ? fmax(pdby, pdy) : pdy;
-          const T M = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
-          const T lme = A ? fmax(lbm[k], lm[k]) : lm[k];
-          const T lxe = A ? fmax(lbx[k], lx[k]) : lx[k];
-          const T lye = A ? fmax(lby[k], ly[k]) : ly[k];
-          const T Y = fma(lye, rw.dd[k], lme * rw.md[k]);
-          const T BM = N ? lm[k] : (I ? lbm[k] : lme);
-          const T BX = N ? lx[k] : (I ? lbx[k] : lxe);
-          const T BY = N ? ly[k] : (I ? lby[k] : lye);
-          const T ume = de ? fmax(pubm, pum) : pum;
-          const T uxe = de ? fmax(pubx, pux) : pux;
-          const T X = fma(uxe, rw.ii[k], ume * rw.mi[k]);
-          pdm = lm[k]; pdx = lx[k]; pdy = ly[k];
-          pdbm = lbm[k]; pdbx = lbx[k]; pdby = lby[k];
-          lm[k] = M; lx[k] = X; ly[k] = Y;
-          lbm[k] = BM; lbx[k] = BX; lby[k] = BY;
-          pum = M; pux = X; pubm = BM; pubx = BX;
-        }
-      }
-      dm = um; dx = ux; dy = uy;
-      if constexpr (PD) { dbm = ubm; dbx = ubx; dby = uby; }
-      acc += lm[R - 1] + lx[R - 1];
+      cells(s, rw, P, Q);
+      P = Q;
     }
-    // Snapshot columns: lane lw reaches column q at step q - start - 1 + lw.
-    for (int s = s_lo; s < nsnap; ++s) {
-      const int t0 = snaps[s].col - start - 1;
-      if (t < t0) break;
-      const int wl = t - t0;
-      if (wl >= L) { s_lo = s + 1; continue; }
-      if (lane == wl) {
-        T* d = scratch + size_t(snaps[s].slot) * NV * 64 + lane;
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          d[(3 * k + 0) * 64] = lm[k];
-          d[(3 * k + 1) * 64] = lx[k];
-          d[(3 * k + 2) * 64] = ly[k];
-          if constexpr (PD) {
-            d[(3 * R + 3 * k + 0) * 64] = lbm[k];
-            d[(3 * R + 3 * k + 1) * 64] = lbx[k];
-            d[(3 * R + 3 * k + 2) * 64] = lby[k];
-          }
-        }
-        constexpr int U = PD ? 6 * R : 3 * R;
-        d[(U + 0) * 64] = um;
-        d[(U + 1) * 64] = ux;
-        d[(U + 2) * 64] = uy;
-        if constexpr (PD) {
-          d[(U + 3) * 64] = ubm;
-          d[(U + 4) * 64] = ubx;
-          d[(U + 5) * 64] = uby;
-        }
-        d[(NV - 1) * 64] = acc;
-      }
-    }
+    if constexpr (SNAP) cur.visit(t, lane, s, P, scratch);
   }
-  return acc;
+  // Steady state: every lane is live; the diagonal and up neighbours alternate between P and Q.
+  uint4 q = *reinterpret_cast<const uint4*>(lw + t);
+  for (; t + 4 <= steps; t += 4) {
+    const uint4 cq = q;
+    q = *reinterpret_cast<const uint4*>(lw + t + 4);
+    s.w = shr1(cq.x, s.w);
+    shift_up(s, Q);
+    cells(s, rw, P, Q);
+    if constexpr (SNAP) cur.visit(t, lane, s, Q, scratch);
+    s.w = shr1(cq.y, s.w);
+    shift_up(s, P);
+    cells(s, rw, Q, P);
+    if constexpr (SNAP) cur.visit(t + 1, lane, s, P, scratch);
+    s.w = shr1(cq.z, s.w);
+    shift_up(s, Q);
+    cells(s, rw, P, Q);
+    if constexpr (SNAP) cur.visit(t + 2, lane, s, Q, scratch);
+    s.w = shr1(cq.w, s.w);
+    shift_up(s, P);
+    cells(s, rw, Q, P);
+    if constexpr (SNAP) cur.visit(t + 3, lane, s, P, scratch);
+  }
+  for (; t < steps; ++t) {
+    s.w = shr1(lw[t], s.w);
+    shift_up(s, Q);
+    cells(s, rw, P, Q);
+    P = Q;
+    if constexpr (SNAP) cur.visit(t, lane, s, P, scratch);
+  }
+  return s.acc;
 }
 
 // Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded.
@@ -332,7 +460,7 @@ template <bool PD>
 __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int L,
                                           int lane, uint32_t* lw) {
   const int W = p.len - start;
-  const int total = W + L + 2;
+  const int total = W + L + 8;  // the steady loop reads one group of four ahead
   wave_sync_lds();
   for (int i = lane; i < total; i += 64) {
     uint32_t v = 0;
@@ -344,7 +472,7 @@ __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, 
 
 template <typename T, int R, bool PD>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) {
-  extern __shared__ uint32_t lds_all[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds_all[];
   const int lane = threadIdx.x & 63;
   const int wave_in_block = threadIdx.x >> 6;
   const int64_t wave_global = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave_in_block;
@@ -359,23 +487,24 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
     item = __builtin_amdgcn_readfirstlane(item);
     if (item >= n_items) break;
 
+    // Everything below is uniform across the wave; readfirstlane keeps it in scalar registers.
     int r, k_begin, k_end;
     if (a.pair_mode) {
-      r = a.items[2 * item];
-      k_begin = a.items[2 * item + 1];
+      r = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
+      k_begin = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
       k_end = k_begin + 1;
     } else {
-      r = a.items[item];
+      r = __builtin_amdgcn_readfirstlane(a.items[item]);
       k_begin = 0;
       k_end = -1;
     }
     const int64_t o = a.read_off[r];
-    const int n = int(a.read_off[r + 1] - o);
-    const int L = (n + R - 1) / R;
+    const int n = __builtin_amdgcn_readfirstlane(int(a.read_off[r + 1] - o));
+    const int L = (n + R) / R;  // n rows plus at least one identity row
     const int pad = R * L - n;
-    const int g = a.read_region[r];
-    const int hap0 = a.region_hap0[g];
-    const int H = a.region_nhap[g];
+    const int g = __builtin_amdgcn_readfirstlane(a.read_region[r]);
+    const int hap0 = __builtin_amdgcn_readfirstlane(a.region_hap0[g]);
+    const int H = __builtin_amdgcn_readfirstlane(a.region_nhap[g]);
     if (k_end < 0) k_end = H;
     const int64_t out_base = a.region_out[g] + int64_t(a.read_local[r]) * H;
 
@@ -384,7 +513,13 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
 
     double prev = 0.0;
     for (int k = k_begin; k < k_end; ++k) {
-      const HapPlan p = a.plan[hap0 + k];
+      HapPlan p = a.plan[hap0 + k];
+      p.len = __builtin_amdgcn_readfirstlane(p.len);
+      p.lcp = __builtin_amdgcn_readfirstlane(p.lcp);
+      p.dup = __builtin_amdgcn_readfirstlane(p.dup);
+      p.nsnap = __builtin_amdgcn_readfirstlane(p.nsnap);
+      p.snap0 = __builtin_amdgcn_readfirstlane(p.snap0);
+      p.start_slot = __builtin_amdgcn_readfirstlane(p.start_slot);
       double res;
       if (!a.pair_mode && p.dup) {
         res = prev;  // kernel.rs:237-246
@@ -400,8 +535,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
         stage_hap<PD>(a, p, start, L, lane, lw);
-        const T acc = sweep<T, R, PD>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                      a.snaps + p.snap0, nsnap, start, scratch);
+        const T acc =
+            nsnap ? sweep<T, R, PD, true>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
+                                          a.snaps + p.snap0, nsnap, start, scratch)
+                  : sweep<T, R, PD, false>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
+                                           a.snaps + p.snap0, 0, start, scratch);
         const double raw = double(readlane(acc, L - 1));
         if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted))
           res = __builtin_nan("");

@@ -82,8 +82,10 @@ struct BatchInput {
   const uint8_t* hap_flags = nullptr;  // PD only
 };
 
-constexpr int kMaxRowsPerLane = 8;   // reads up to 512 bases
-constexpr int kMaxReadLen = 64 * kMaxRowsPerLane;
+// A wave holds the n read rows plus at least one identity row that reproduces DP row 0, at
+// R rows per lane over 64 lanes: reads of up to 64 * 8 - 1 = 511 bases.
+constexpr int kMaxRowsPerLane = 8;
+constexpr int kMaxReadLen = 64 * kMaxRowsPerLane - 1;
 
 struct HostPlan {
   int64_t n_reads = 0, n_haps = 0, n_pairs = 0;
@@ -103,7 +105,9 @@ struct HostPlan {
 // set through set_error.
 int build_plan(const BatchInput& in, HostPlan& out);
 
-// Rows per lane for a read of n bases.
-inline int rows_per_lane(int64_t n) { return int((n + 63) / 64); }
+// Rows per lane for a read of n bases (n rows plus one identity row over 64 lanes) and the
+// lanes it then occupies.
+inline int rows_per_lane(int64_t n) { return int((n + 64) / 64); }
+inline int64_t lanes_for(int64_t n, int R) { return (n + R) / R; }
 
 }  // namespace fgkl

@@ -297,7 +297,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r) {
       const int64_t n = in.read_off[r + 1] - in.read_off[r];
       const int R = p.read_rows[size_t(r)];
-      const int64_t L = (n + R - 1) / R;
+      const int64_t L = lanes_for(n, R);
       cost[size_t(r)] = (cols + int64_t(H) * (L - 1)) * R;
     }
   }
