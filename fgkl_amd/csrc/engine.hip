@@ -362,7 +362,7 @@ struct SnapCursor {
 template <typename T, int R, bool PD, bool SNAP>
 __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
                    const T* snap_in, T scale, const SnapEntry* snaps, int nsnap, int start,
-                   T* scratch) {
+                   T* scratch, bool row0_above) {
   constexpr int NV = snap_values<R, PD>();
   Lane<T, R, PD> s;
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
@@ -403,6 +403,9 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
       }
     }
     s.acc = T(0);
+    // When identity rows fill lane 0, read row 1 is lane 1's first row and its diagonal
+    // neighbour in column 1 is DP row 0's deletion value (Y[0][0] = C/m, reference.rs:27-28).
+    if (row0_above) P.y = init;
   }
   s.w = 0;
 
@@ -535,11 +538,13 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
         stage_hap<PD>(a, p, start, L, lane, lw);
+        // Lane 1 starts below DP row 0 when identity rows fill lane 0 (n a multiple of R).
+        const bool row0_above = lane == 1 && pad == R;
         const T acc =
             nsnap ? sweep<T, R, PD, true>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                          a.snaps + p.snap0, nsnap, start, scratch)
+                                          a.snaps + p.snap0, nsnap, start, scratch, row0_above)
                   : sweep<T, R, PD, false>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                           a.snaps + p.snap0, 0, start, scratch);
+                                           a.snaps + p.snap0, 0, start, scratch, row0_above);
         const double raw = double(readlane(acc, L - 1));
         if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted))
           res = __builtin_nan("");

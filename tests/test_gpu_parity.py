@@ -238,6 +238,31 @@ def test_long_reads_far_below_single_precision(gpu):  # lib.rs:827-877 shapes
         assert_rel(out, expected, what=str(p))
 
 
+def test_every_lane_layout_matches_reference(gpu):
+    # Read lengths around every rows-per-lane boundary (R = ceil((n+1)/64)) and identity-row
+    # count, including n a multiple of R (identity rows fill lane 0) and the 511-base maximum.
+    rng = Rng(77)
+    lengths = sorted({1, 2, 3, 5, 8, 31, 32, 62, 63, 64, 65, 100, 126, 127, 128, 129, 150, 189,
+                      190, 191, 192, 250, 252, 254, 255, 256, 300, 320, 383, 384, 448, 500, 510,
+                      511})
+    ref = bytes(rng.base() for _ in range(700))
+    haps = [ref[:620], ref[5:640], ref[:600] + b"ACGTACGT"]
+    reads = []
+    for n in lengths:
+        start = rng.below(100)
+        bases = bytearray(ref[start:start + n])
+        for i in range(0, n, 17):
+            bases[i] = rng.base()
+        reads.append(F.ReadRef(bytes(bases), [20 + rng.below(21) for _ in range(n)],
+                               [30 + rng.below(16) for _ in range(n)],
+                               [30 + rng.below(16) for _ in range(n)], [10] * n))
+    expected = reference_all(reads, haps)
+    out, _ = compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what="lane layouts double")
+    out, _ = compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, "lane layouts float")
+
+
 def test_random_small_haplotype_sets_match_reference(gpu):  # lib.rs:761-798
     from fgkl_amd.synthetic import MASK64  # noqa: F401
     rng = Rng(31)
