@@ -36,9 +36,9 @@ def log(msg: str) -> None:
 
 
 def bucket_cells(data) -> np.ndarray:
-    """Nominal cells per rows-per-lane bucket (R = ceil(n / 64))."""
+    """Nominal cells per rows-per-lane bucket (R = ceil((n + 1) / 64): n rows + an identity row)."""
     lens = np.diff(data.read_offsets)
-    rows = (lens + 63) // 64
+    rows = (lens + 64) // 64
     region_of = np.repeat(np.arange(data.n_regions), np.diff(data.region_reads))
     hap_sum = np.diff(data.hap_offsets[data.region_haps])
     cells = lens.astype(np.int64) * hap_sum[region_of].astype(np.int64)
