@@ -321,48 +321,18 @@ __device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Ce
   d[(NV - 1) * 64] = s.acc;
 }
 
-// Snapshot windows of one sweep: lane wl reaches snapshot column q at step q - start - 1 + wl,
-// so the window of q is open for L steps.  Windows open and close in column order.
-struct SnapCursor {
-  const SnapEntry* snaps;
-  int n, lo, t0_lo, start, L;
-  __device__ void init(const SnapEntry* s, int count, int st, int lanes) {
-    snaps = s;
-    n = count;
-    lo = 0;
-    start = st;
-    L = lanes;
-    t0_lo = n ? __builtin_amdgcn_readfirstlane(snaps[0].col) - start - 1 : 0x7fffffff;
-  }
-  template <typename T, int R, bool PD>
-  __device__ __forceinline__ void visit(int t, int lane, const Lane<T, R, PD>& s,
-                                        const Cell<T>& u, T* scratch) {
-    if (t < t0_lo) return;
-    constexpr int NV = snap_values<R, PD>();
-    for (int i = lo; i < n; ++i) {
-      const int t0 = __builtin_amdgcn_readfirstlane(snaps[i].col) - start - 1;
-      if (t < t0) break;
-      const int wl = t - t0;
-      if (wl >= L) {  // closed: every lane has passed this column
-        lo = i + 1;
-        t0_lo = lo < n ? __builtin_amdgcn_readfirstlane(snaps[lo].col) - start - 1 : 0x7fffffff;
-        continue;
-      }
-      if (lane == wl) {
-        const int slot = __builtin_amdgcn_readfirstlane(snaps[i].slot);
-        store_snapshot<T, R, PD>(s, u, scratch + size_t(slot) * NV * 64 + lane);
-      }
-    }
-  }
-};
+// A staged column word carries, in bits 20..31, one plus the scratch slot that must record the
+// column's state (0: none).  The word travels down the lanes with the column, so each lane sees
+// the mark exactly when it has just computed that column.
+constexpr uint32_t kSlotShift = 20;
+constexpr int kMaxSlots = 4095;
 
 // One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
 // column start+1+t (zero beyond W, 16-byte aligned).  Returns the lane's raw scaled sum over the
 // read's last row (meaningful in lane L-1).
 template <typename T, int R, bool PD, bool SNAP>
 __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
-                   const T* snap_in, T scale, const SnapEntry* snaps, int nsnap, int start,
-                   T* scratch, bool row0_above) {
+                   const T* snap_in, T scale, T* scratch, bool row0_above) {
   constexpr int NV = snap_values<R, PD>();
   Lane<T, R, PD> s;
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
@@ -409,11 +379,15 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
   }
   s.w = 0;
 
-  SnapCursor cur;
-  if constexpr (SNAP) cur.init(snaps, nsnap, start, L);
+  auto record = [&](const Cell<T>& u) {
+    if constexpr (SNAP) {
+      const uint32_t sl = s.w >> kSlotShift;
+      if (sl) store_snapshot<T, R, PD>(s, u, scratch + size_t(sl - 1) * NV * 64 + lane);
+    }
+  };
   const int steps = W + L - 1;
-  // Ramp: lane l starts at step l and must keep its column-`start` state until then.  Rounded
-  // up to four steps so the steady loop reads aligned groups of four column words.
+  // Ramp: lane l starts at step l and keeps its column-`start` state until then.  Rounded up to
+  // four steps so the steady loop reads aligned groups of four column words.
   const int ramp = min(steps, (L - 1 + 3) & ~3);
   int t = 0;
   Cell<T> Q;
@@ -423,8 +397,8 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
     if (t >= lane) {
       cells(s, rw, P, Q);
       P = Q;
+      record(P);
     }
-    if constexpr (SNAP) cur.visit(t, lane, s, P, scratch);
   }
   // Steady state: every lane is live; the diagonal and up neighbours alternate between P and Q.
   uint4 q = *reinterpret_cast<const uint4*>(lw + t);
@@ -434,34 +408,35 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
     s.w = shr1(cq.x, s.w);
     shift_up(s, Q);
     cells(s, rw, P, Q);
-    if constexpr (SNAP) cur.visit(t, lane, s, Q, scratch);
+    record(Q);
     s.w = shr1(cq.y, s.w);
     shift_up(s, P);
     cells(s, rw, Q, P);
-    if constexpr (SNAP) cur.visit(t + 1, lane, s, P, scratch);
+    record(P);
     s.w = shr1(cq.z, s.w);
     shift_up(s, Q);
     cells(s, rw, P, Q);
-    if constexpr (SNAP) cur.visit(t + 2, lane, s, Q, scratch);
+    record(Q);
     s.w = shr1(cq.w, s.w);
     shift_up(s, P);
     cells(s, rw, Q, P);
-    if constexpr (SNAP) cur.visit(t + 3, lane, s, P, scratch);
+    record(P);
   }
   for (; t < steps; ++t) {
     s.w = shr1(lw[t], s.w);
     shift_up(s, Q);
     cells(s, rw, P, Q);
     P = Q;
-    if constexpr (SNAP) cur.visit(t, lane, s, P, scratch);
+    record(P);
   }
   return s.acc;
 }
 
-// Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded.
+// Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded, and marks
+// the columns this sweep must record with their scratch slots.
 template <bool PD>
 __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int L,
-                                          int lane, uint32_t* lw) {
+                                          int lane, uint32_t* lw, int nsnap) {
   const int W = p.len - start;
   const int total = W + L + 8;  // the steady loop reads one group of four ahead
   wave_sync_lds();
@@ -469,6 +444,13 @@ __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, 
     uint32_t v = 0;
     if (i < W) v = PD ? a.hap_words[p.off + start + i] : uint32_t(a.hap_bytes[p.off + start + i]);
     lw[i] = v;
+  }
+  if (nsnap) {
+    wave_sync_lds();
+    for (int i = lane; i < nsnap; i += 64) {
+      const SnapEntry e = a.snaps[p.snap0 + i];
+      lw[e.col - start - 1] |= uint32_t(e.slot + 1) << kSlotShift;
+    }
   }
   wave_sync_lds();
 }
@@ -537,14 +519,14 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
           scale = T(double(p.start_src_len) / double(p.len));  // kernel.rs:641
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
-        stage_hap<PD>(a, p, start, L, lane, lw);
+        stage_hap<PD>(a, p, start, L, lane, lw, nsnap);
         // Lane 1 starts below DP row 0 when identity rows fill lane 0 (n a multiple of R).
         const bool row0_above = lane == 1 && pad == R;
         const T acc =
             nsnap ? sweep<T, R, PD, true>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                          a.snaps + p.snap0, nsnap, start, scratch, row0_above)
+                                          scratch, row0_above)
                   : sweep<T, R, PD, false>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                           a.snaps + p.snap0, 0, start, scratch, row0_above);
+                                           scratch, row0_above);
         const double raw = double(readlane(acc, L - 1));
         if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted))
           res = __builtin_nan("");
@@ -899,6 +881,11 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
   if ((rc = d.alloc(&d.counters, 32))) return rc;
   if ((rc = d.alloc(&d.out, size_t(p.n_pairs)))) return rc;
   // LDS per wave: the longest haplotype plus the ramp of the widest read, in words.
+  if (p.max_slots > kMaxSlots) {
+    set_error(kInvalidArgument, "a region holds more than " + std::to_string(kMaxSlots) +
+                                    " distinct haplotype prefix columns");
+    return kInvalidArgument;
+  }
   d.lds_words = ((p.max_hap_len + 64 + 8) + 3) & ~3;
   if (int64_t(d.lds_words) * 4 * kWavesPerBlock > 64 * 1024) {
     set_error(kInvalidArgument, "haplotype of " + std::to_string(p.max_hap_len) +
