@@ -234,6 +234,27 @@ def test_pd_flags_never_lower_a_likelihood_on_a_reference_region():
     assert np.all(w >= wo) and np.sum(np.abs(w - wo) > 1.0) >= 5
 
 
+def test_pd_deletion_equals_skipping_the_span():
+    # A read matching the haplotype with a flagged span removed sees the deletion branch carry
+    # the mass from before the span across it: the PD likelihood equals the plain likelihood
+    # against the physically shortened haplotype, with the row-0 start density C/m of the full
+    # length (a factor (m - s) / m), up to the small mass of paths that do not skip.  Ignoring
+    # the branch values (a wrong diagonal or left index) lands 4-10 log10 units lower.
+    rng = Rng(3)
+    for _ in range(12):
+        m = 80
+        hap = bytes(rng.base() for _ in range(m))
+        a, s = 20 + rng.below(30), 1 + rng.below(8)
+        flags = [0] * m
+        flags[a] |= F.DEL_START
+        flags[a + s - 1] |= F.DEL_END
+        shortened = hap[:a] + hap[a + s:]
+        r = read(shortened[a - 15:a + 15])
+        pd = O.pd_log10_likelihood(hap, flags, r)
+        skip = O.log10_likelihood(shortened, r) + math.log10((m - s) / m)
+        assert 0.0 <= pd - skip < 1e-4, (a, s, pd, skip)
+
+
 def test_pd_single_base_deletion_on_one_column_is_max_of_skip_and_keep():
     # A one-base read against a two-column haplotype whose second column is a one-base deletion:
     # flags only change branch bookkeeping; with a one-row read the result equals the plain one.
