@@ -20,20 +20,23 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def declared_symbols() -> set[str]:
-    text = (ROOT / "include" / "fgkl.h").read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(fgkl_[a-z0-9_]+)\s*\(", text))
+    names = set()
+    for header in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b((?:fgkl_|Java_)[A-Za-z0-9_]+)\s*\(", text))
+    return names
 
 
 def test_library_exports_every_declared_symbol():
     names = declared_symbols()
     assert "fgkl_pairhmm_compute_native" in names and "fgkl_batch_run" in names
+    assert "Java_com_fulcrumgenomics_fgkl_pairhmm_FgklPairHmm_computeNative" in names
     out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
                          text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
     missing = names - exported
     assert not missing, missing
-    assert names <= set(_lib.SIGNATURES)
+    assert {n for n in names if n.startswith("fgkl_")} <= set(_lib.SIGNATURES)
 
 
 def test_backend_name():
