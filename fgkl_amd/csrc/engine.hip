@@ -3,7 +3,7 @@
 // Kernel layout (one work item per wave64):
 //  * A read of n bases is laid out down the lanes: lane l owns R consecutive rows, with
 //    R = ceil((n+1)/64) (1..8) and L = ceil((n+1)/R) live lanes.  The first R*L-n >= 1 rows of
-//    lane 0 are identity rows that reproduce DP row 0 exactly (prior 0, M->I 0, I->I 0, M->D 0,
+//    lane 0 are identity rows that reproduce DP row 0 exactly (prior 0, M->I 0, I->I 1, M->D 0,
 //    D->D 1, left value (0, 0, C/m)), so the read's last row is always the last register of lane
 //    L-1 and lane 0 never needs the row above it.
 //  * The haplotype is swept along anti-diagonals: at step t lane l computes column t-l+1 for its
@@ -131,7 +131,7 @@ __device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
 
 // Per-lane parameters of its R rows (Workspace::prepare, kernel.rs:485-529).
 template <typename T, int R, bool PD> struct Rows {
-  T pm[R], px[R], mm[R], im[R], mi[R], ii[R], md[R], dd[R];
+  T pm[R], px[R], mm[R], im[R], mi[R], md[R], gc[R];
   uint32_t rb[R];
   uint32_t ralt[R];
   uint32_t kind[R];  // 0: row >= 2, 1: row 1, 2: identity row (or a lane past the read)
@@ -153,16 +153,17 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       rw.mm[k] = tab.mm[((iq >= dq ? iq * (iq + 1) : dq * (dq + 1)) >> 1) + (iq >= dq ? dq : iq)];
       rw.mi[k] = tab.ep[iq];
       rw.md[k] = tab.ep[dq];
-      rw.ii[k] = tab.ep[gq];
-      rw.dd[k] = tab.ep[gq];
+      rw.gc[k] = tab.ep[gq];  // insertion and deletion extension alike (model.rs:73-75)
       rw.im[k] = tab.im[gq];
       rw.rb[k] = b;
       rw.ralt[k] = alt_bit(b);
       rw.kind[k] = arow == 0 ? 1u : 0u;
     } else {
+      // Identity rows: D->D 1 keeps row 0's C/m; I->I 1 is harmless because the row above an
+      // identity row is zero.  Lanes past the read: all zero.
       rw.pm[k] = T(0); rw.px[k] = T(0); rw.mm[k] = T(0); rw.im[k] = T(0);
-      rw.mi[k] = T(0); rw.ii[k] = T(0); rw.md[k] = T(0);
-      rw.dd[k] = lane < L ? T(1) : T(0);
+      rw.mi[k] = T(0); rw.md[k] = T(0);
+      rw.gc[k] = lane < L ? T(1) : T(0);
       rw.rb[k] = 0x100u;
       rw.ralt[k] = 0u;
       rw.kind[k] = 2u;
@@ -170,8 +171,11 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
   }
 }
 
-// Number of scratch values a lane keeps per snapshot slot.
+// Number of scratch values a lane keeps per snapshot slot, and the record stride: a slot is
+// 64 per-lane records of snap_stride values, so the one lane that records a column writes one
+// contiguous, 16-byte-aligned block.
 template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 7 : 3 * R + 4; }
+template <int R, bool PD> constexpr int snap_stride() { return (snap_values<R, PD>() + 3) & ~3; }
 
 // DPP wave_shr:1 with lane 0 receiving zero (bound_ctrl): no "old" operand to materialise.
 __device__ __forceinline__ float shr1z(float src) {
@@ -249,7 +253,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
     const T prior = match ? rw.pm[k] : rw.px[k];
     if constexpr (!PD) {
       // kernel.rs:1011-1013
-      const T Y = fma(s.y[k], rw.dd[k], s.m[k] * rw.md[k]);
+      const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[k]);
       s.m[k] = prior * fma(pdy, rw.im[k], fma(pdx, rw.im[k], pdm * rw.mm[k]));
       s.y[k] = Y;
     } else {
@@ -264,7 +268,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       const T lme = A ? fmax(s.bm[k], s.m[k]) : s.m[k];
       const T lxe = A ? fmax(s.bx[k], s.x[k]) : s.x[k];
       const T lye = A ? fmax(s.by[k], s.y[k]) : s.y[k];
-      const T Y = fma(lye, rw.dd[k], lme * rw.md[k]);
+      const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
       const T BM = N ? s.m[k] : (I ? s.bm[k] : lme);
       const T BX = N ? s.x[k] : (I ? s.bx[k] : lxe);
       const T BY = N ? s.y[k] : (I ? s.by[k] : lye);
@@ -280,45 +284,45 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
     const T pum = k ? s.m[k - 1] : u.m;
     const T pux = k ? s.x[k - 1] : u.x;
     if constexpr (!PD) {
-      s.x[k] = fma(pux, rw.ii[k], pum * rw.mi[k]);
+      s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[k]);
     } else {
       // pd_reference.rs:95-100
       const T pubm = k ? s.bm[k - 1] : u.bm;
       const T pubx = k ? s.bx[k - 1] : u.bx;
       const T ume = de[k] ? fmax(pubm, pum) : pum;
       const T uxe = de[k] ? fmax(pubx, pux) : pux;
-      s.x[k] = fma(uxe, rw.ii[k], ume * rw.mi[k]);
+      s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[k]);
     }
   }
   s.acc += s.m[R - 1] + s.x[R - 1];
 }
 
-// Writes a lane's state into snapshot slot memory (layout [value][lane]); `u` is the row above
+// Writes a lane's state into its snapshot record (value by value); `u` is the row above
 // its first row in the column just computed.
 template <typename T, int R, bool PD>
 __device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Cell<T>& u, T* d) {
   constexpr int NV = snap_values<R, PD>();
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    d[(3 * k + 0) * 64] = s.m[k];
-    d[(3 * k + 1) * 64] = s.x[k];
-    d[(3 * k + 2) * 64] = s.y[k];
+    d[(3 * k + 0)] = s.m[k];
+    d[(3 * k + 1)] = s.x[k];
+    d[(3 * k + 2)] = s.y[k];
     if constexpr (PD) {
-      d[(3 * R + 3 * k + 0) * 64] = s.bm[k];
-      d[(3 * R + 3 * k + 1) * 64] = s.bx[k];
-      d[(3 * R + 3 * k + 2) * 64] = s.by[k];
+      d[(3 * R + 3 * k + 0)] = s.bm[k];
+      d[(3 * R + 3 * k + 1)] = s.bx[k];
+      d[(3 * R + 3 * k + 2)] = s.by[k];
     }
   }
   constexpr int U = PD ? 6 * R : 3 * R;
-  d[(U + 0) * 64] = u.m;
-  d[(U + 1) * 64] = u.x;
-  d[(U + 2) * 64] = u.y;
+  d[(U + 0)] = u.m;
+  d[(U + 1)] = u.x;
+  d[(U + 2)] = u.y;
   if constexpr (PD) {
-    d[(U + 3) * 64] = u.bm;
-    d[(U + 4) * 64] = u.bx;
-    d[(U + 5) * 64] = u.by;
+    d[(U + 3)] = u.bm;
+    d[(U + 4)] = u.bx;
+    d[(U + 5)] = u.by;
   }
-  d[(NV - 1) * 64] = s.acc;
+  d[(NV - 1)] = s.acc;
 }
 
 // A staged column word carries, in bits 20..31, one plus the scratch slot that must record the
@@ -337,28 +341,28 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
   Lane<T, R, PD> s;
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
   if (snap_in) {
-    const T* p = snap_in + lane;
+    const T* p = snap_in + lane * snap_stride<R, PD>();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      s.m[k] = p[(3 * k + 0) * 64] * scale;
-      s.x[k] = p[(3 * k + 1) * 64] * scale;
-      s.y[k] = p[(3 * k + 2) * 64] * scale;
+      s.m[k] = p[(3 * k + 0)] * scale;
+      s.x[k] = p[(3 * k + 1)] * scale;
+      s.y[k] = p[(3 * k + 2)] * scale;
       if constexpr (PD) {
-        s.bm[k] = p[(3 * R + 3 * k + 0) * 64] * scale;
-        s.bx[k] = p[(3 * R + 3 * k + 1) * 64] * scale;
-        s.by[k] = p[(3 * R + 3 * k + 2) * 64] * scale;
+        s.bm[k] = p[(3 * R + 3 * k + 0)] * scale;
+        s.bx[k] = p[(3 * R + 3 * k + 1)] * scale;
+        s.by[k] = p[(3 * R + 3 * k + 2)] * scale;
       }
     }
     constexpr int U = PD ? 6 * R : 3 * R;
-    P.m = p[(U + 0) * 64] * scale;
-    P.x = p[(U + 1) * 64] * scale;
-    P.y = p[(U + 2) * 64] * scale;
+    P.m = p[(U + 0)] * scale;
+    P.x = p[(U + 1)] * scale;
+    P.y = p[(U + 2)] * scale;
     if constexpr (PD) {
-      P.bm = p[(U + 3) * 64] * scale;
-      P.bx = p[(U + 4) * 64] * scale;
-      P.by = p[(U + 5) * 64] * scale;
+      P.bm = p[(U + 3)] * scale;
+      P.bx = p[(U + 4)] * scale;
+      P.by = p[(U + 5)] * scale;
     }
-    s.acc = p[(NV - 1) * 64] * scale;
+    s.acc = p[(NV - 1)] * scale;
   } else {
     // Column 0: zeros below row 0; identity rows carry row 0's deletion value C/m.
 #pragma unroll
@@ -382,7 +386,8 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
   auto record = [&](const Cell<T>& u) {
     if constexpr (SNAP) {
       const uint32_t sl = s.w >> kSlotShift;
-      if (sl) store_snapshot<T, R, PD>(s, u, scratch + size_t(sl - 1) * NV * 64 + lane);
+      if (sl)
+        store_snapshot<T, R, PD>(s, u, scratch + (size_t(sl - 1) * 64 + lane) * snap_stride<R, PD>());
     }
   };
   const int steps = W + L - 1;
@@ -515,7 +520,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
         const T* snap_in = nullptr;
         T scale = T(1);
         if (shared && p.start_slot >= 0) {
-          snap_in = scratch + size_t(p.start_slot) * NV * 64;
+          snap_in = scratch + size_t(p.start_slot) * 64 * snap_stride<R, PD>();
           scale = T(double(p.start_src_len) / double(p.len));  // kernel.rs:641
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
@@ -777,12 +782,12 @@ struct DeviceBatch::Impl {
     int per_cu = 0;
     if (resident_blocks<T>(st, pd, R, lds_bytes, &per_cu)) return -1;
     const int64_t waves = int64_t(per_cu) * st->num_cus * kWavesPerBlock;
-    const int nv = pd ? 6 * R + 7 : 3 * R + 4;
+    const int nv = ((pd ? 6 * R + 7 : 3 * R + 4) + 3) & ~3;  // snap_stride
     return waves * int64_t(plan.max_slots) * nv * 64 * int64_t(sizeof(T));
   }
   template <typename T>
   int64_t scratch_stride(int R) const {
-    const int nv = pd ? 6 * R + 7 : 3 * R + 4;
+    const int nv = ((pd ? 6 * R + 7 : 3 * R + 4) + 3) & ~3;  // snap_stride
     return int64_t(plan.max_slots) * nv * 64;
   }
 
