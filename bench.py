@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -46,6 +47,18 @@ def bucket_cells(data) -> np.ndarray:
     for r in range(1, 9):
         out[r - 1] = int(cells[rows == r].sum())
     return out
+
+
+PROFILE = ROOT / "profiles" / "sweep_kernel_batched_uneven_4096_traffic.txt"
+
+
+def measured_traffic(R: int):
+    """HBM bytes per launch of the R-rows-per-lane sweep on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size), or None."""
+    if not PROFILE.exists():
+        return None
+    m = re.search(rf"^traffic_bytes R={R} read=(\d+) write=(\d+)$", PROFILE.read_text(), re.M)
+    return None if m is None else int(m.group(1)) + int(m.group(2))
 
 
 def cpu_baseline(data, max_regions: int, threads: int):
@@ -151,7 +164,8 @@ def main() -> int:
     result["roofline"] = {
         "bound": "valu", "kernel": f"sweep_kernel<float, R={dom + 1}, plain>",
         "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(dom + 1),
+        "traffic_source": f"{PROFILE.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
         "algorithmic_flops_per_launch": int(FLOPS_PER_CELL * bc[dom]),
         "launch_ms": round(bucket_ms[dom], 3),
     }
