@@ -68,6 +68,15 @@ SIGNATURES = {
     "fgkl_synth_free": (None, [C.c_void_p]),
     "fgkl_synth_splitmix": (None, [C.c_uint64, C.c_int64, u64p]),
     "fgkl_selftest_dpp": (C.c_int, [i32p]),
+    "fgkl_selftest_dpp_shl": (C.c_int, [i32p]),
+    "fgkl_grad_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, i64p, i64p, i64p, u8p, i64p,
+                                   u8p]),
+    "fgkl_grad_pairs": (C.c_int64, [C.c_void_p]),
+    "fgkl_grad_read_bases": (C.c_int64, [C.c_void_p]),
+    "fgkl_grad_forward": (C.c_int, [C.c_void_p, f64p, f64p, f64p, C.c_double, f64p, f64p]),
+    "fgkl_grad_backward": (C.c_int, [C.c_void_p, f64p, f64p, f64p, C.c_double, f64p, f64p, f64p,
+                                     f64p, f64p, f64p, f64p]),
+    "fgkl_grad_free": (None, [C.c_void_p]),
 }
 
 _lib = None

@@ -8,6 +8,7 @@
 
 #include "../../include/fgkl.h"
 #include "fgkl_engine.hpp"
+#include "fgkl_grad.hpp"
 #include "fgkl_internal.hpp"
 #include "fgkl_synthetic.hpp"
 
@@ -27,6 +28,14 @@ struct fgkl_batch {
 struct fgkl_synth {
   Batch b;
 };
+
+struct fgkl_grad {
+  std::unique_ptr<GradBatch> grad;
+};
+
+namespace fgkl {
+int dpp_shl_selftest(int32_t* out64);
+}
 
 namespace {
 
@@ -295,5 +304,67 @@ void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out) {
 }
 
 int fgkl_selftest_dpp(int32_t* out64) { return dpp_selftest(out64); }
+
+int fgkl_selftest_dpp_shl(int32_t* out64) { return dpp_shl_selftest(out64); }
+
+int fgkl_grad_create(fgkl_grad** out, int64_t n_regions, const int64_t* region_reads,
+                     const int64_t* region_haps, const int64_t* read_offsets,
+                     const uint8_t* bases, const int64_t* hap_offsets,
+                     const uint8_t* hap_bases) {
+  if (!out) return fail(kInvalidArgument, "null output handle");
+  *out = nullptr;
+  try {
+    GradInput in;
+    in.n_regions = n_regions;
+    in.region_read = region_reads;
+    in.region_hap = region_haps;
+    in.read_off = read_offsets;
+    in.bases = bases;
+    in.hap_off = hap_offsets;
+    in.hap_bases = hap_bases;
+    auto* g = new fgkl_grad;
+    if (int rc = GradBatch::create(in, g->grad)) {
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int64_t fgkl_grad_pairs(const fgkl_grad* g) { return g ? g->grad->n_pairs() : 0; }
+
+int64_t fgkl_grad_read_bases(const fgkl_grad* g) { return g ? g->grad->n_read_bases() : 0; }
+
+int fgkl_grad_forward(fgkl_grad* g, const double* err, const double* ins, const double* del,
+                      double gcp, double* log10_likelihoods, double* kernel_ms) {
+  if (!g) return fail(kInvalidArgument, "null gradient handle");
+  try {
+    GradParams p;
+    p.err = err; p.ins = ins; p.del = del; p.gcp = gcp;
+    return g->grad->forward(p, log10_likelihoods, kernel_ms);
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int fgkl_grad_backward(fgkl_grad* g, const double* err, const double* ins, const double* del,
+                       double gcp, const double* weights, double* log10_likelihoods,
+                       double* grad_err, double* grad_ins, double* grad_del, double* grad_gcp,
+                       double* kernel_ms) {
+  if (!g) return fail(kInvalidArgument, "null gradient handle");
+  try {
+    GradParams p;
+    p.err = err; p.ins = ins; p.del = del; p.gcp = gcp;
+    return g->grad->backward(p, weights, log10_likelihoods, grad_err, grad_ins, grad_del,
+                             grad_gcp, kernel_ms);
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+void fgkl_grad_free(fgkl_grad* g) { delete g; }
 
 }  // extern "C"

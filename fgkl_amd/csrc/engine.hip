@@ -32,6 +32,7 @@
 
 #include "fgkl_internal.hpp"
 #include "fgkl_engine.hpp"
+#include "wave64.hpp"
 
 namespace fgkl {
 
@@ -83,38 +84,6 @@ struct SweepArgs {
   int32_t rescue;
   int32_t lds_words;           // LDS words per wave for the staged haplotype
 };
-
-// ---- wave64 primitives ----------------------------------------------------------------------
-// DPP wave_shr:1 (dpp_ctrl 0x138): lane i receives lane i-1's value; lane 0 keeps `old`.
-__device__ __forceinline__ int shr1i(int old, int src) {
-  return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xF, 0xF, false);
-}
-__device__ __forceinline__ uint32_t shr1(uint32_t old, uint32_t src) {
-  return uint32_t(shr1i(int(old), int(src)));
-}
-__device__ __forceinline__ float shr1(float old, float src) {
-  return __builtin_bit_cast(float, shr1i(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src)));
-}
-__device__ __forceinline__ double shr1(double old, double src) {
-  const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
-  const uint32_t lo = uint32_t(shr1i(int(uint32_t(o)), int(uint32_t(s))));
-  const uint32_t hi = uint32_t(shr1i(int(uint32_t(o >> 32)), int(uint32_t(s >> 32))));
-  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
-__device__ __forceinline__ float readlane(float v, int l) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-__device__ __forceinline__ double readlane(double v, int l) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), l));
-  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), l));
-  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Qualities above 254 are clamped (model.rs:53-64).
 __device__ __forceinline__ int dclamp(uint32_t q) { return q > uint32_t(kMaxQual) ? kMaxQual : int(q); }
@@ -176,19 +145,6 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
 // contiguous, 16-byte-aligned block.
 template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 7 : 3 * R + 4; }
 template <int R, bool PD> constexpr int snap_stride() { return (snap_values<R, PD>() + 3) & ~3; }
-
-// DPP wave_shr:1 with lane 0 receiving zero (bound_ctrl): no "old" operand to materialise.
-__device__ __forceinline__ float shr1z(float src) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, src), 0x138,
-                                                            0xF, 0xF, true));
-}
-__device__ __forceinline__ double shr1z(double src) {
-  const uint64_t s = __builtin_bit_cast(uint64_t, src);
-  const uint32_t lo = uint32_t(__builtin_amdgcn_mov_dpp(int(uint32_t(s)), 0x138, 0xF, 0xF, true));
-  const uint32_t hi =
-      uint32_t(__builtin_amdgcn_mov_dpp(int(uint32_t(s >> 32)), 0x138, 0xF, 0xF, true));
-  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
 
 // The values of one DP row in one column: match, insertion, deletion and (PD) branch values.
 template <typename T> struct Cell {
@@ -472,9 +428,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
   constexpr int NV = snap_values<R, PD>();
 
   for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(a.counter, 1);
-    item = __builtin_amdgcn_readfirstlane(item);
+    // Every lane adds one, so a claim advances the counter by 64 and every lane's old value lies
+    // in [64 i, 64 i + 63].  A lane-dependent branch around the claim, next to the lane-0 stores
+    // at the end of an item, would let the optimiser thread the loop's back edge past the claim
+    // for lanes 1..63, which would then read an item without lane 0.
+    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, 1)) >> 6;
     if (item >= n_items) break;
 
     // Everything below is uniform across the wave; readfirstlane keeps it in scalar registers.
@@ -604,6 +562,7 @@ template <> struct KernelTable<double> {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
 
 struct DeviceState {
   int device = -1;
@@ -879,6 +838,11 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
     d.bucket_n[R - 1] = int32_t(items.size());
     int64_t pairs = 0;
     for (int32_t r : items) pairs += p.region_nhap[size_t(p.read_region[size_t(r)])];
+    if (pairs > kMaxQueueItems) {  // a work queue advances by 64 per claim in an int32 counter
+      set_error(kInvalidArgument, "a batch holds more than " + std::to_string(kMaxQueueItems) +
+                                      " pairs of one read length class; split it");
+      return kInvalidArgument;
+    }
     d.bucket_pairs[R - 1] = pairs;
     if ((rc = d.upload(&d.bucket_items[R - 1], items.data(), items.size()))) return rc;
     if ((rc = d.alloc(&d.rescue_pairs[R - 1], size_t(2 * pairs)))) return rc;

@@ -106,6 +106,32 @@ class BatchData:
                 for h in range(int(self.region_haps[g]), int(self.region_haps[g + 1]))]
 
 
+def batch_from_regions(regions, pd: bool = False) -> BatchData:
+    """A BatchData of explicit regions: each is (reads, haplotypes) with reads as ReadRef and
+    haplotypes as bytes (PdHaplotype when pd)."""
+    rr, hh, ro, ho = [0], [0], [0], [0]
+    cols = {k: [] for k in ("bases", "quals", "ins", "dels", "gcp", "hap", "flags")}
+    for reads, haps in regions:
+        for rd in reads:
+            cols["bases"].append(np.frombuffer(bytes(rd.bases), np.uint8))
+            cols["quals"].append(np.asarray(rd.quals, np.uint8))
+            cols["ins"].append(np.asarray(rd.ins_gop, np.uint8))
+            cols["dels"].append(np.asarray(rd.del_gop, np.uint8))
+            cols["gcp"].append(np.asarray(rd.gcp, np.uint8))
+            ro.append(ro[-1] + len(rd.bases))
+        for h in haps:
+            b = bytes(h.bases) if pd else bytes(h)
+            cols["hap"].append(np.frombuffer(b, np.uint8))
+            cols["flags"].append(np.asarray(h.flags if pd else [0] * len(b), np.uint8))
+            ho.append(ho[-1] + len(b))
+        rr.append(rr[-1] + len(reads))
+        hh.append(hh[-1] + len(haps))
+    cat = lambda k: np.concatenate(cols[k]).astype(np.uint8) if cols[k] else np.zeros(0, np.uint8)
+    return BatchData(len(regions), np.array(rr, np.int64), np.array(hh, np.int64),
+                     np.array(ro, np.int64), cat("bases"), cat("quals"), cat("ins"), cat("dels"),
+                     cat("gcp"), np.array(ho, np.int64), cat("hap"), cat("flags"), pd)
+
+
 def _from_handle(h, pd: bool) -> BatchData:
     L = lib()
     try:

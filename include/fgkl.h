@@ -154,6 +154,40 @@ void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out);
 
 /* Checks the DPP lane shift the kernels rely on: out[i] receives 1000+i-1, out[0] = -1. */
 int fgkl_selftest_dpp(int32_t* out64);
+/* Checks the opposite DPP lane shift (the adjoint's): out[i] receives 1000+i+1, out[63] = -1. */
+int fgkl_selftest_dpp_shl(int32_t* out64);
+/*
+ * Gradient engine (no reference counterpart: fgkl computes likelihoods only).  The model is the
+ * reference's FP64 forward recurrence (crates/pairhmm/src/reference.rs:20-64) with continuous
+ * parameters per read base i: error probability e[i] (priors 1-e match, e/3 mismatch, N matches
+ * everything), insertion and deletion gap-open probabilities d_ins[i], d_del[i] (M->M =
+ * 1 - d_ins - d_del, M->I = d_ins, M->D = d_del) and one global gap-continuation probability g
+ * (I->I = D->D = g, I->M = D->M = 1 - g), the continuous form of the Phred tables
+ * (crates/pairhmm/src/model.rs:66-84).  Regions as fgkl_batch_create; pairs land region after
+ * region, [r * H + h] inside each.
+ *
+ * Host arrays: e, d_ins, d_del and grad_err/ins/del hold one double per read base
+ * (fgkl_grad_read_bases), weights and log10_likelihoods one per pair (fgkl_grad_pairs); g and
+ * *grad_gcp are single values.  Each call copies its inputs to the device, runs, copies the
+ * outputs back and returns when they are written; kernel_ms (NULL to skip) receives the device
+ * time of the kernels (HIP events).  fgkl_grad_backward writes d/dx of sum_p w_p log10 L_p
+ * (w_p = 1 when weights is NULL) and log10 L when log10_likelihoods is not NULL.  A pair with
+ * w_p = 0, or whose FP64 likelihood underflows to 0, contributes nothing.  Calls on one handle
+ * must not overlap (they share its device buffers).
+ */
+typedef struct fgkl_grad fgkl_grad;
+int fgkl_grad_create(fgkl_grad** out, int64_t n_regions, const int64_t* region_reads,
+                     const int64_t* region_haps, const int64_t* read_offsets,
+                     const uint8_t* bases, const int64_t* hap_offsets, const uint8_t* hap_bases);
+int64_t fgkl_grad_pairs(const fgkl_grad* g);
+int64_t fgkl_grad_read_bases(const fgkl_grad* g);
+int fgkl_grad_forward(fgkl_grad* g, const double* err, const double* ins, const double* del,
+                      double gcp, double* log10_likelihoods, double* kernel_ms);
+int fgkl_grad_backward(fgkl_grad* g, const double* err, const double* ins, const double* del,
+                       double gcp, const double* weights, double* log10_likelihoods,
+                       double* grad_err, double* grad_ins, double* grad_del, double* grad_gcp,
+                       double* kernel_ms);
+void fgkl_grad_free(fgkl_grad* g);
 
 #ifdef __cplusplus
 }

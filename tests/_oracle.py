@@ -111,3 +111,52 @@ def priors(q) -> tuple[float, float]:
     p = (C.c_double * 2)()
     oracle().fgklo_priors(q, p)
     return p[0], p[1]
+
+
+def grad_pair(hap: bytes, bases: bytes, e, di, dd, g: float):
+    """Gradient oracle of one pair: (log10 L, d/de, d/dd_ins, d/dd_del, d/dg)."""
+    o = oracle()
+    n = len(bases)
+    e, di, dd = (np.ascontiguousarray(x, dtype=np.float64) for x in (e, di, dd))
+    ge, gi, gd, gg = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(1)
+    h, b = _u8(hap), _u8(bases)
+    f = lambda a: a.ctypes.data_as(f64p)
+    ll = o.fgklo_log10_likelihood_grad(_p(h), len(h), _p(b), n, f(e), f(di), f(dd), g, f(ge),
+                                       f(gi), f(gd), f(gg))
+    return ll, ge, gi, gd, float(gg[0])
+
+
+def cont_pair(hap: bytes, bases: bytes, e, di, dd, g: float) -> float:
+    """log10 L of the continuous-parameter forward (the gradient model)."""
+    o = oracle()
+    e, di, dd = (np.ascontiguousarray(x, dtype=np.float64) for x in (e, di, dd))
+    h, b = _u8(hap), _u8(bases)
+    f = lambda a: a.ctypes.data_as(f64p)
+    return o.fgklo_log10_likelihood_cont(_p(h), len(h), _p(b), len(b), f(e), f(di), f(dd), g)
+
+
+def grad_batch(data, e, di, dd, g: float, weights=None):
+    """The gradient API's outputs from the oracle: per-pair log10 L and the gradients of
+    sum_p w_p log10 L_p per read base and for g."""
+    n_pairs = data.n_pairs()
+    ll = np.zeros(n_pairs)
+    ge, gi, gd = np.zeros(len(data.bases)), np.zeros(len(data.bases)), np.zeros(len(data.bases))
+    gg = 0.0
+    base = data.out_base()
+    for gi_ in range(data.n_regions):
+        r0, r1 = int(data.region_reads[gi_]), int(data.region_reads[gi_ + 1])
+        h0, h1 = int(data.region_haps[gi_]), int(data.region_haps[gi_ + 1])
+        H = h1 - h0
+        for r in range(r0, r1):
+            lo, hi = int(data.read_offsets[r]), int(data.read_offsets[r + 1])
+            for k in range(H):
+                p = int(base[gi_]) + (r - r0) * H + k
+                w = 1.0 if weights is None else float(weights[p])
+                l_, a, b, c, d = grad_pair(data.hap(h0 + k), bytes(data.bases[lo:hi]),
+                                           e[lo:hi], di[lo:hi], dd[lo:hi], g)
+                ll[p] = l_
+                ge[lo:hi] += w * a
+                gi[lo:hi] += w * b
+                gd[lo:hi] += w * c
+                gg += w * d
+    return ll, ge, gi, gd, gg
