@@ -30,6 +30,7 @@ METRIC = "PairHMM GCUPS, batched FP32+FP64-rescue vs plain FP64, 1 GPU, |Δlog10
 FLOPS_PER_CELL = 12          # 4 mul + 4 FMA per cell (reference.rs:49-54)
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed) peak
 PEAK_FP64_TFLOPS = 78.6
+ADJOINT_FLOPS_PER_CELL = 36  # forward recompute 12 + reverse recurrence 12 + gradient terms 12
 
 
 def log(msg: str) -> None:
@@ -80,6 +81,39 @@ def cpu_baseline(data, max_regions: int, threads: int):
     return sub, ref, dt
 
 
+def adjoint_bench(args) -> dict:
+    """The gradient engine on the adjoint workload: FP64 forward recompute, reverse sweep and
+    gradient accumulation, timed by HIP events around its kernels."""
+    from fgkl_amd.grad import GradProblem, probabilities
+    from fgkl_amd.synthetic import workload
+
+    data = workload(4, seed=args.seed, n_regions=args.adjoint_regions)
+    cells = data.cells()
+    e, di, dd, g = probabilities(data)
+    prob = GradProblem(data)
+    prob.backward(e, di, dd, g)  # warm-up
+    runs = [prob.backward(e, di, dd, g, timing=True) for _ in range(2)]
+    ms = min(r[-1] for r in runs)
+    ll, ge, gi, gd, gg, _ = runs[-1]
+    assert np.array_equal(ge, runs[0][1]) and gg == runs[0][4], "adjoint runs differ"
+    fwd_ms = min(prob.forward(e, di, dd, g, timing=True)[1] for _ in range(2))
+    achieved = ADJOINT_FLOPS_PER_CELL * cells / (ms * 1e-3) / 1e12
+    log(f"adjoint: {data.n_regions} regions, {data.n_pairs()} pairs, {cells:.3e} cells; "
+        f"backward {ms:.1f} ms ({cells / ms / 1e6:.1f} GCUPS), forward {fwd_ms:.1f} ms")
+    prob.close()
+    return {
+        "workload": "adjoint regions: R~U(1,200) reads of U(50,250) bp, H~U(2,8) haplotypes of "
+                    "read_max+U(50,400) bp",
+        "regions": data.n_regions, "pairs": data.n_pairs(), "cells": cells, "dtype": "f64",
+        "backward_ms": round(ms, 3), "gcups": round(cells / ms / 1e6, 2),
+        "forward_ms": round(fwd_ms, 3), "forward_gcups": round(cells / fwd_ms / 1e6, 2),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                     "flops_per_cell": ADJOINT_FLOPS_PER_CELL},
+        "finite": bool(np.isfinite(ge).all() and np.isfinite(gi).all() and np.isfinite(gd).all()),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +125,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 per-call run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient workload")
+    ap.add_argument("--adjoint-regions", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -194,6 +230,9 @@ def main() -> int:
                             "pairs_over_1e-4": int((d > 1e-4).sum()),
                             "nonfinite": int((~np.isfinite(fp32_res)).sum())}
         log(f"fp64 per-call: {base_ms:.1f} ms, {base_gcups:.1f} GCUPS; max |d| {d.max():.3g}")
+
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_bench(args)
 
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
