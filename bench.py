@@ -37,19 +37,6 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def bucket_cells(data) -> np.ndarray:
-    """Nominal cells per rows-per-lane bucket (R = ceil((n + 1) / 64): n rows + an identity row)."""
-    lens = np.diff(data.read_offsets)
-    rows = (lens + 64) // 64
-    region_of = np.repeat(np.arange(data.n_regions), np.diff(data.region_reads))
-    hap_sum = np.diff(data.hap_offsets[data.region_haps])
-    cells = lens.astype(np.int64) * hap_sum[region_of].astype(np.int64)
-    out = np.zeros(8, dtype=np.int64)
-    for r in range(1, 9):
-        out[r - 1] = int(cells[rows == r].sum())
-    return out
-
-
 PROFILE = ROOT / "profiles" / "sweep_kernel_batched_uneven_4096_traffic.txt"
 
 
@@ -165,6 +152,7 @@ def main() -> int:
     st = batch.run(F.MODE_FP32_RESCUE, share=True, graph=False)
     eager_ms = st.total_ms
     bucket_ms = list(st.bucket_ms)
+    bc = [int(x) for x in st.bucket_cells]  # cells each rows-per-lane launch covers
     primary_ms, rescue_ms, rescued = st.primary_ms, st.rescue_ms, int(st.rescued_pairs)
     fp32_res = batch.results()
     log(f"eager: total {eager_ms:.1f} ms, fp32 {primary_ms:.1f} ms, fp64 rescue {rescue_ms:.1f} ms, "
@@ -194,7 +182,6 @@ def main() -> int:
     }
 
     # Roofline: the dominant sweep kernel (the bucket with the most kernel time).
-    bc = bucket_cells(data)
     dom = int(np.argmax(bucket_ms))
     achieved = FLOPS_PER_CELL * bc[dom] / (bucket_ms[dom] * 1e-3) / 1e12
     result["roofline"] = {
@@ -208,7 +195,7 @@ def main() -> int:
     result["kernels"] = {
         "fp32_sweep_ms": round(primary_ms, 3), "fp64_rescue_ms": round(rescue_ms, 3),
         "bucket_ms": [round(x, 3) for x in bucket_ms],
-        "bucket_cells": [int(x) for x in bc],
+        "bucket_cells": bc,
         "fp32_sweep_gcups": round(cells / (primary_ms * 1e-3) / 1e9, 2),
     }
     result["rescue"] = {"pairs": rescued, "fraction": round(rescued / n_pairs, 6),
@@ -248,7 +235,7 @@ def main() -> int:
         result["cpu_baseline"] = {
             "value": round(sub_cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
             "kind": "port",
-            "sample": f"first {sub.n_regions} regions of the same workload ({n_pairs and n_sub} "
+            "sample": f"first {sub.n_regions} regions of the same workload ({n_sub} "
                       f"pairs, {sub_cells:.3e} cells), scalar FP64 oracle, one region per thread",
             "seconds": round(dt, 2)}
         result.setdefault("parity", {})["max_abs_dlog10L_vs_oracle_sample"] = float(dgpu.max())

@@ -24,6 +24,7 @@ class RunStats(C.Structure):
         ("rescue_ms", C.c_double),
         ("rescued_pairs", C.c_int64),
         ("bucket_ms", C.c_double * 8),
+        ("bucket_cells", C.c_int64 * 8),
     ]
 
 

@@ -238,7 +238,10 @@ int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_grap
     stats->primary_ms = st.primary_ms;
     stats->rescue_ms = st.rescue_ms;
     stats->rescued_pairs = st.rescued_pairs;
-    for (int i = 0; i < 8; ++i) stats->bucket_ms[i] = st.bucket_ms[i];
+    for (int i = 0; i < 8; ++i) {
+      stats->bucket_ms[i] = st.bucket_ms[i];
+      stats->bucket_cells[i] = st.bucket_cells[i];
+    }
   }
   return rc;
 }

@@ -70,7 +70,7 @@ struct SweepArgs {
   const SnapEntry* snaps;
   const uint8_t* hap_bytes;
   const uint32_t* hap_words;
-  const int32_t* items;        // read indices, or (read, sorted hap) pairs in pair mode
+  const int32_t* items;        // (read, read or -1) per item, or (read, sorted hap) in pair mode
   int32_t n_items;
   const int32_t* n_items_dev;  // when set, the item count is read from device memory
   int32_t* counter;            // work-queue head for this launch
@@ -162,13 +162,26 @@ template <typename T, int R, bool PD> struct Lane {
   uint32_t w;
 };
 
+// Bitwise AND of a value with a per-lane 32-bit mask (both halves of a double).
+__device__ __forceinline__ float and_bits(float v, uint32_t k) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & k);
+}
+__device__ __forceinline__ double and_bits(double v, uint32_t k) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & ((uint64_t(k) << 32) | k));
+}
+
 // The row above a lane's first row in the column it computes next: lane l-1's last row, moved
 // with DPP before either lane updates.  Lane 0 receives zeros; its first row is always an
-// identity row, which ignores them.
+// identity row, which ignores them.  The first lane of a second read in the wave starts with
+// identity rows too; of the values above them only the insertion value would leak through (I->I
+// 1 in an identity row), so it is masked off there (`keep` is zero in that lane, all ones
+// elsewhere).  The mask is applied after the move by an AND on an opaque mask: a select would
+// let the compiler run the move only in the lanes that keep it, and a DPP move reads a lane
+// outside EXEC as zero, which would also cut the insertion value reaching the lane below.
 template <typename T, int R, bool PD>
-__device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u) {
+__device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u, uint32_t keep) {
   u.m = shr1z(s.m[R - 1]);
-  u.x = shr1z(s.x[R - 1]);
+  u.x = and_bits(shr1z(s.x[R - 1]), keep);
   u.y = shr1z(s.y[R - 1]);
   if constexpr (PD) {
     u.bm = shr1z(s.bm[R - 1]);
@@ -289,10 +302,13 @@ constexpr int kMaxSlots = 4095;
 
 // One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
 // column start+1+t (zero beyond W, 16-byte aligned).  Returns the lane's raw scaled sum over the
-// read's last row (meaningful in lane L-1).
+// read's last row (meaningful in lane L-1).  `cap` receives lane cap_lane's sum right after step
+// cap_t: the first read of a two-read wave finishes column W there, and its last lane would go on
+// summing the columns past W while the second read's lanes finish.
 template <typename T, int R, bool PD, bool SNAP>
 __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
-                   const T* snap_in, T scale, T* scratch, bool row0_above) {
+                   const T* snap_in, T scale, T* scratch, bool row0_above, uint32_t keep,
+                   int cap_t, int cap_lane, T& cap) {
   constexpr int NV = snap_values<R, PD>();
   Lane<T, R, PD> s;
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
@@ -354,41 +370,46 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
   Cell<T> Q;
   for (; t < ramp; ++t) {
     s.w = shr1(lw[t], s.w);
-    shift_up(s, Q);
+    shift_up(s, Q, keep);
     if (t >= lane) {
       cells(s, rw, P, Q);
       P = Q;
       record(P);
     }
+    if (t == cap_t) cap = readlane(s.acc, cap_lane);
   }
   // Steady state: every lane is live; the diagonal and up neighbours alternate between P and Q.
+  // It stops by step cap_t; the single steps after it test for the capture.
+  const int steady_end = min(steps, cap_t + 1);
   uint4 q = *reinterpret_cast<const uint4*>(lw + t);
-  for (; t + 4 <= steps; t += 4) {
+  for (; t + 4 <= steady_end; t += 4) {
     const uint4 cq = q;
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
     s.w = shr1(cq.x, s.w);
-    shift_up(s, Q);
+    shift_up(s, Q, keep);
     cells(s, rw, P, Q);
     record(Q);
     s.w = shr1(cq.y, s.w);
-    shift_up(s, P);
+    shift_up(s, P, keep);
     cells(s, rw, Q, P);
     record(P);
     s.w = shr1(cq.z, s.w);
-    shift_up(s, Q);
+    shift_up(s, Q, keep);
     cells(s, rw, P, Q);
     record(Q);
     s.w = shr1(cq.w, s.w);
-    shift_up(s, P);
+    shift_up(s, P, keep);
     cells(s, rw, Q, P);
     record(P);
   }
+  if (t - 1 == cap_t) cap = readlane(s.acc, cap_lane);
   for (; t < steps; ++t) {
     s.w = shr1(lw[t], s.w);
-    shift_up(s, Q);
+    shift_up(s, Q, keep);
     cells(s, rw, P, Q);
     P = Q;
     record(P);
+    if (t == cap_t) cap = readlane(s.acc, cap_lane);
   }
   return s.acc;
 }
@@ -436,30 +457,48 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
     if (item >= n_items) break;
 
     // Everything below is uniform across the wave; readfirstlane keeps it in scalar registers.
-    int r, k_begin, k_end;
+    // A main-pass item is one read, or two reads of one region sharing the wave (read A in lanes
+    // [0, La), read B in [La, La+Lb)); a pair-mode item is one (read, sorted haplotype).
+    int ra, rb, k_begin, k_end;
+    ra = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
     if (a.pair_mode) {
-      r = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
+      rb = -1;
       k_begin = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
       k_end = k_begin + 1;
     } else {
-      r = __builtin_amdgcn_readfirstlane(a.items[item]);
+      rb = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
       k_begin = 0;
       k_end = -1;
     }
-    const int64_t o = a.read_off[r];
-    const int n = __builtin_amdgcn_readfirstlane(int(a.read_off[r + 1] - o));
-    const int L = (n + R) / R;  // n rows plus at least one identity row
-    const int pad = R * L - n;
-    const int g = __builtin_amdgcn_readfirstlane(a.read_region[r]);
+    const bool two = rb >= 0;
+    const int64_t oa = a.read_off[ra];
+    const int na = __builtin_amdgcn_readfirstlane(int(a.read_off[ra + 1] - oa));
+    const int64_t ob = two ? a.read_off[rb] : 0;
+    const int nb = two ? __builtin_amdgcn_readfirstlane(int(a.read_off[rb + 1] - ob)) : 0;
+    const int La = (na + R) / R;  // n rows plus at least one identity row
+    const int Lb = two ? (nb + R) / R : 0;
+    const int L = La + Lb;
+    const int pada = R * La - na, padb = R * Lb - nb;
+    const int g = __builtin_amdgcn_readfirstlane(a.read_region[ra]);
     const int hap0 = __builtin_amdgcn_readfirstlane(a.region_hap0[g]);
     const int H = __builtin_amdgcn_readfirstlane(a.region_nhap[g]);
     if (k_end < 0) k_end = H;
-    const int64_t out_base = a.region_out[g] + int64_t(a.read_local[r]) * H;
+    const int64_t out_a = a.region_out[g] + int64_t(a.read_local[ra]) * H;
+    const int64_t out_b = two ? a.region_out[g] + int64_t(a.read_local[rb]) * H : 0;
 
+    // This lane's read: its offset, lanes, identity rows and lane index within the read.
+    const bool in_b = lane >= La;
     Rows<T, R, PD> rw;
-    load_rows<T, R, PD>(a, tab, o, L, pad, lane, rw);
+    load_rows<T, R, PD>(a, tab, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
+                        in_b ? lane - La : lane, rw);
+    // Each read's second lane starts below DP row 0 when identity rows fill its first lane.
+    const bool row0_above = (lane == 1 && pada == R) || (two && lane == La + 1 && padb == R);
+    // Zero in the first lane of read B, all ones elsewhere; opaque to the optimiser (see
+    // shift_up).
+    uint32_t keep = (two && lane == La) ? 0u : ~0u;
+    asm volatile("" : "+v"(keep));
 
-    double prev = 0.0;
+    double prev_a = 0.0, prev_b = 0.0;
     for (int k = k_begin; k < k_end; ++k) {
       HapPlan p = a.plan[hap0 + k];
       p.len = __builtin_amdgcn_readfirstlane(p.len);
@@ -468,9 +507,10 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
       p.nsnap = __builtin_amdgcn_readfirstlane(p.nsnap);
       p.snap0 = __builtin_amdgcn_readfirstlane(p.snap0);
       p.start_slot = __builtin_amdgcn_readfirstlane(p.start_slot);
-      double res;
+      double res_a, res_b;
       if (!a.pair_mode && p.dup) {
-        res = prev;  // kernel.rs:237-246
+        res_a = prev_a;  // kernel.rs:237-246
+        res_b = prev_b;
       } else {
         const bool shared = a.share && !a.pair_mode;
         const int start = shared ? p.lcp : 0;
@@ -483,28 +523,40 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) 
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
         stage_hap<PD>(a, p, start, L, lane, lw, nsnap);
-        // Lane 1 starts below DP row 0 when identity rows fill lane 0 (n a multiple of R).
-        const bool row0_above = lane == 1 && pad == R;
+        // Read A's last lane finishes column W at step W + La - 2.
+        const int W = p.len - start;
+        T acc_a = T(0);
         const T acc =
-            nsnap ? sweep<T, R, PD, true>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                          scratch, row0_above)
-                  : sweep<T, R, PD, false>(rw, lw, p.len - start, L, lane, init, snap_in, scale,
-                                           scratch, row0_above);
-        const double raw = double(readlane(acc, L - 1));
-        if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted))
-          res = __builtin_nan("");
-        else
-          res = log10(raw) - Num<T>::kLog10Scale;
+            nsnap ? sweep<T, R, PD, true>(rw, lw, W, L, lane, init, snap_in, scale, scratch,
+                                          row0_above, keep, W + La - 2, La - 1, acc_a)
+                  : sweep<T, R, PD, false>(rw, lw, W, L, lane, init, snap_in, scale, scratch,
+                                           row0_above, keep, W + La - 2, La - 1, acc_a);
+        // A read's last row is its last lane's last register (kernel.rs:897-914).
+        auto finish = [](double raw) {
+          if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted)) return __builtin_nan("");
+          return log10(raw) - Num<T>::kLog10Scale;
+        };
+        res_a = finish(double(acc_a));
+        res_b = finish(double(readlane(acc, L - 1)));
       }
       if (lane == 0) {
-        a.out[out_base + p.src] = res;
-        if (a.rescue && res != res) {
+        a.out[out_a + p.src] = res_a;
+        if (a.rescue && res_a != res_a) {
           const int idx = atomicAdd(a.rescue_count, 1);
-          a.rescue_pairs[2 * idx] = r;
+          a.rescue_pairs[2 * idx] = ra;
           a.rescue_pairs[2 * idx + 1] = k;
         }
+        if (two) {
+          a.out[out_b + p.src] = res_b;
+          if (a.rescue && res_b != res_b) {
+            const int idx = atomicAdd(a.rescue_count, 1);
+            a.rescue_pairs[2 * idx] = rb;
+            a.rescue_pairs[2 * idx + 1] = k;
+          }
+        }
       }
-      prev = res;
+      prev_a = res_a;
+      prev_b = res_b;
     }
   }
 }
@@ -670,6 +722,7 @@ struct DeviceBatch::Impl {
   int32_t* bucket_items[kMaxRowsPerLane] = {};
   int32_t bucket_n[kMaxRowsPerLane] = {};
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
+  int64_t bucket_cells[kMaxRowsPerLane] = {};
   int32_t* rescue_pairs[kMaxRowsPerLane] = {};
   int32_t* counters = nullptr;  // [0,8) main, [8,16) rescue, [16,24) rescue counts
   double* out = nullptr;
@@ -834,16 +887,16 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
     if ((rc = d.upload(&d.hap_bytes, in.hap_bases, hb))) return rc;
   }
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
-    const auto& items = p.bucket_reads[R - 1];
-    d.bucket_n[R - 1] = int32_t(items.size());
-    int64_t pairs = 0;
-    for (int32_t r : items) pairs += p.region_nhap[size_t(p.read_region[size_t(r)])];
+    const auto& items = p.bucket_items[R - 1];
+    d.bucket_n[R - 1] = int32_t(items.size() / 2);
+    const int64_t pairs = p.bucket_pairs[R - 1];
     if (pairs > kMaxQueueItems) {  // a work queue advances by 64 per claim in an int32 counter
       set_error(kInvalidArgument, "a batch holds more than " + std::to_string(kMaxQueueItems) +
                                       " pairs of one read length class; split it");
       return kInvalidArgument;
     }
     d.bucket_pairs[R - 1] = pairs;
+    d.bucket_cells[R - 1] = p.bucket_cells[R - 1];
     if ((rc = d.upload(&d.bucket_items[R - 1], items.data(), items.size()))) return rc;
     if ((rc = d.alloc(&d.rescue_pairs[R - 1], size_t(2 * pairs)))) return rc;
   }
@@ -924,6 +977,7 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
     if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue)
       for (int R = 0; R < kMaxRowsPerLane; ++R) rescued += counts[16 + R];
     stats->rescued_pairs = rescued;
+    for (int R = 0; R < kMaxRowsPerLane; ++R) stats->bucket_cells[R] = d.bucket_cells[R];
     if (!use_graph) {
       float first = 0.f;
       HIP_TRY(hipEventElapsedTime(&first, d.ev[0], d.ev[1]));

@@ -21,6 +21,7 @@ struct RunStats {
   double rescue_ms = 0;    // the FP64 rescue kernels; -1 when run as one graph
   int64_t rescued_pairs = 0;
   double bucket_ms[kMaxRowsPerLane] = {};  // primary sweep launch of each rows-per-lane bucket
+  int64_t bucket_cells[kMaxRowsPerLane] = {};  // DP cells (read x haplotype bases) it covers
 };
 
 class DeviceBatch {

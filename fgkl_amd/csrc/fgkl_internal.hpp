@@ -97,8 +97,11 @@ struct HostPlan {
   std::vector<uint32_t> words;      // PD column words (same offsets as the haplotype bytes)
   int32_t max_slots = 0;
   int32_t max_hap_len = 0;
-  // Work: read indices per rows-per-lane bucket (index R-1), longest work first.
-  std::vector<int32_t> bucket_reads[kMaxRowsPerLane];
+  // Work per rows-per-lane bucket (index R-1), costliest first: items of two int32, a read and
+  // a second read of the same region sharing the wave (-1 when the read is alone).
+  std::vector<int32_t> bucket_items[kMaxRowsPerLane];
+  int64_t bucket_pairs[kMaxRowsPerLane] = {};  // read x haplotype pairs of each bucket
+  int64_t bucket_cells[kMaxRowsPerLane] = {};  // their read bases x haplotype bases
 };
 
 // Validates the batch and builds its plan.  Returns kOk or kInvalidArgument with a message
@@ -109,5 +112,11 @@ int build_plan(const BatchInput& in, HostPlan& out);
 // lanes it then occupies.
 inline int rows_per_lane(int64_t n) { return int((n + 64) / 64); }
 inline int64_t lanes_for(int64_t n, int R) { return (n + R) / R; }
+// Rows per lane for two reads sharing a wave (each with its identity row), or 0 if none fits.
+inline int rows_per_lane2(int64_t na, int64_t nb) {
+  for (int R = 1; R <= kMaxRowsPerLane; ++R)
+    if (lanes_for(na, R) + lanes_for(nb, R) <= 64) return R;
+  return 0;
+}
 
 }  // namespace fgkl

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -212,6 +213,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
   RegionHaps rh{&in, &end_state};
 
   int64_t out_base = 0;
+  std::vector<int64_t> item_costs[kMaxRowsPerLane];
   std::vector<int64_t> order;
   std::vector<int64_t> lcp;
   std::map<int64_t, int32_t> counts;
@@ -285,26 +287,88 @@ int build_plan(const BatchInput& in, HostPlan& p) {
   p.region_out[size_t(G)] = out_base;
   p.n_pairs = out_base;
 
-  // Work buckets by rows per lane, costliest first so the persistent grid's tail is short.
-  std::vector<int64_t> cost(size_t(NR), 0);
+  // Work items.  Two reads of a region can share a wave: read A's rows fill lanes [0, L_A), read
+  // B's the lanes after, both swept against the same haplotype columns (B lags L_A steps).  A
+  // step costs about kStepFixed + kStepPerRow * R VALU instructions (wave-wide moves, the column
+  // word, then R cells), and a region's sweeps take sum(W) + H' (L - 1) steps (H' haplotypes
+  // swept, W their columns after the shared prefix).  Reads are paired greedily, longest first,
+  // each with the free read that lowers the modelled cost most; the candidates are, for every R,
+  // the longest free read that fits beside it.
+  constexpr int64_t kStepFixed = 20, kStepPerRow = 19;
   for (int64_t g = 0; g < G; ++g) {
-    int64_t cols = 0;
+    int64_t cols = 0, swept = 0, hap_bases = 0;
     const int32_t H = p.region_nhap[size_t(g)];
     for (int32_t k = 0; k < H; ++k) {
       const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
-      if (!hp.dup) cols += hp.len - hp.lcp;
+      hap_bases += hp.len;
+      if (!hp.dup) {
+        cols += hp.len - hp.lcp;
+        swept += 1;
+      }
     }
-    for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r) {
-      const int64_t n = in.read_off[r + 1] - in.read_off[r];
-      const int R = p.read_rows[size_t(r)];
-      const int64_t L = lanes_for(n, R);
-      cost[size_t(r)] = (cols + int64_t(H) * (L - 1)) * R;
+    auto item_cost = [&](int R, int64_t L) {
+      return (cols + swept * (L - 1)) * (kStepFixed + kStepPerRow * R);
+    };
+    auto single_cost = [&](int64_t n) {
+      const int R = rows_per_lane(n);
+      return item_cost(R, lanes_for(n, R));
+    };
+    auto pair_cost = [&](int64_t na, int64_t nb) {
+      const int R = rows_per_lane2(na, nb);
+      return R ? item_cost(R, lanes_for(na, R) + lanes_for(nb, R)) : INT64_MAX;
+    };
+    const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
+    auto len = [&](int64_t r) { return in.read_off[r + 1] - in.read_off[r]; };
+    std::multiset<std::pair<int64_t, int64_t>> free;  // (length, read)
+    for (int64_t r = r0; r < r1; ++r) free.emplace(len(r), r);
+    while (!free.empty()) {
+      auto ia = std::prev(free.end());
+      const int64_t a = ia->second, na = ia->first;
+      free.erase(ia);
+      const int64_t alone = single_cost(na);
+      int64_t best_gain = 0, b = -1;
+      for (int R = rows_per_lane(na); R <= kMaxRowsPerLane; ++R) {
+        const int64_t spare = 64 - lanes_for(na, R);
+        if (spare <= 0) continue;
+        const int64_t max_n = spare * R - 1;  // its rows plus one identity row
+        auto it = free.upper_bound(std::make_pair(max_n, INT64_MAX));
+        if (it == free.begin()) continue;
+        --it;
+        const int64_t gain = alone + single_cost(it->first) - pair_cost(na, it->first);
+        if (gain > best_gain) {
+          best_gain = gain;
+          b = it->second;
+        }
+      }
+      int64_t nb = 0;
+      if (b >= 0) {
+        nb = len(b);
+        free.erase(free.find(std::make_pair(nb, b)));
+      }
+      const int R = b >= 0 ? rows_per_lane2(na, nb) : rows_per_lane(na);
+      const int64_t L = lanes_for(na, R) + (b >= 0 ? lanes_for(nb, R) : 0);
+      auto& bucket = p.bucket_items[R - 1];
+      bucket.push_back(int32_t(a));
+      bucket.push_back(int32_t(b));
+      p.bucket_pairs[R - 1] += int64_t(H) * (b >= 0 ? 2 : 1);
+      p.bucket_cells[R - 1] += (na + nb) * hap_bases;
+      item_costs[R - 1].push_back(item_cost(R, L));
     }
   }
-  for (int64_t r = 0; r < NR; ++r) p.bucket_reads[p.read_rows[size_t(r)] - 1].push_back(int32_t(r));
-  for (auto& b : p.bucket_reads)
-    std::stable_sort(b.begin(), b.end(),
-                     [&](int32_t a, int32_t c) { return cost[size_t(a)] > cost[size_t(c)]; });
+  // Costliest items first, so the persistent grid's tail is short.
+  for (int R = 0; R < kMaxRowsPerLane; ++R) {
+    auto& items = p.bucket_items[R];
+    const auto& c = item_costs[R];
+    std::vector<int32_t> idx(c.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return c[size_t(x)] > c[size_t(y)]; });
+    std::vector<int32_t> sorted(items.size());
+    for (size_t i = 0; i < idx.size(); ++i) {
+      sorted[2 * i] = items[2 * size_t(idx[i])];
+      sorted[2 * i + 1] = items[2 * size_t(idx[i]) + 1];
+    }
+    items.swap(sorted);
+  }
   return kOk;
 }
 

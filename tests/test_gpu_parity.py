@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import fgkl_amd as F
-from fgkl_amd.synthetic import PdRegion, Region, Rng, workload
+from fgkl_amd.synthetic import PdRegion, Region, Rng, batch_from_regions, workload
 from _oracle import batch_reference, pd_reference_all, reference_all
 
 pytestmark = pytest.mark.gpu
@@ -505,6 +505,63 @@ def test_batch_workloads_match_the_oracle(gpu, wl, regions):
         assert st.rescued_pairs >= 0.25 * data.n_pairs()
     b.run(F.MODE_FP64_PER_REGION)
     assert_rel(b.results(), expected, what=f"workload {wl} per-region")
+    b.close()
+
+
+def _wave_pairs():
+    """(R, A, B): read lengths that fill one wave at exactly R rows per lane, for R = 1..8, with
+    1, about R/2 and R identity rows in read B's first lane (and a varying count in read A's)."""
+    out = []
+    for R in range(1, 9):
+        for pad_b in sorted({1, (R + 1) // 2, R}):
+            lanes_b = 6 + 3 * R
+            pad_a = 1 + (R + pad_b) % R
+            out.append((R, R * (64 - lanes_b) - pad_a, R * lanes_b - pad_b))
+    return out
+
+
+@pytest.mark.parametrize("pd", [False, True])
+def test_two_reads_sharing_a_wave_match_the_oracle(gpu, pd):
+    # One region per length pair, so the planner can only pair its two reads or not; at these
+    # lengths pairing always lowers the modelled cost, so each region's cells land in bucket R.
+    rng = Rng(404 + pd)
+    regions, want = [], np.zeros(8, np.int64)
+    for R, na, nb in _wave_pairs():
+        ref = bytes(rng.base() for _ in range(na + 120))
+        reads = []
+        for n in (na, nb):
+            start = rng.below(60)
+            bases = bytearray(ref[start:start + n])
+            for i in range(rng.below(11), n, 13):
+                bases[i] = rng.base()
+            reads.append(F.ReadRef(bytes(bases), [15 + rng.below(26) for _ in range(n)],
+                                   [30 + rng.below(16) for _ in range(n)],
+                                   [30 + rng.below(16) for _ in range(n)],
+                                   [5 + rng.below(8) for _ in range(n)]))
+        haps = [ref, ref[:na // 2] + ref[na // 2 + 3:], ref[:40] + b"G" + ref[41:]]
+        if pd:
+            flagged = []
+            for h in haps:
+                f = [0] * len(h)
+                for c in range(5 + rng.below(20), len(h), 37):
+                    f[c] = F.SNP | (F.ALT_A, F.ALT_C, F.ALT_G, F.ALT_T)[rng.below(4)]
+                d0 = 10 + rng.below(len(h) - 20)
+                f[d0] |= F.DEL_START
+                f[d0 + 1 + rng.below(4)] |= F.DEL_END
+                flagged.append(F.PdHaplotype(h, f))
+            haps = flagged
+        regions.append((reads, haps))
+        want[R - 1] += (na + nb) * sum(len(h.bases if pd else h) for h in haps)
+    data = batch_from_regions(regions, pd=pd)
+    expected = batch_reference(data)
+    b = F.Batch(data)
+    for share in (True, False):
+        st = b.run(F.MODE_FP64, share=share)
+        assert_rel(b.results(), expected, what=f"paired reads fp64 share={share}")
+        st = b.run(F.MODE_FP32_RESCUE, share=share)
+        assert_close(b.results(), expected, FLOAT_TOL, f"paired reads fp32 share={share}")
+    assert list(st.bucket_cells) == list(want)
+    assert sum(st.bucket_cells) == data.cells()
     b.close()
 
 
