@@ -9,7 +9,8 @@ haplotype lengths), as the reference counts them (synthetic.rs:166-170).
 
 Alongside: the plain per-call FP64 GPU engine on the same inputs (one pair per wave, one
 synchronous launch set per region), parity of every pair against it, the roofline of the
-dominant sweep kernel, and the CPU oracle timed on a bounded sample (cpu_baseline).
+dominant sweep kernel, an end-to-end call from host arrays, the gradient engine, a rescue-stress
+and a PD-HMM workload, and the CPU oracle timed on a bounded sample (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -40,12 +41,14 @@ def log(msg: str) -> None:
 PROFILE = ROOT / "profiles" / "sweep_kernel_batched_uneven_4096_traffic.txt"
 
 
-def measured_traffic(R: int):
-    """HBM bytes per launch of the R-rows-per-lane sweep on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size), or None."""
+def measured_traffic(launch: str):
+    """HBM bytes per launch of one bucket's sweep on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size), when they were taken
+    with the same launch configuration `launch` (R:ROWS:BLOCK:BOOST); otherwise None."""
     if not PROFILE.exists():
         return None
-    m = re.search(rf"^traffic_bytes R={R} read=(\d+) write=(\d+)$", PROFILE.read_text(), re.M)
+    m = re.search(rf"^traffic_bytes launch={re.escape(launch)} read=(\d+) write=(\d+)$",
+                  PROFILE.read_text(), re.M)
     return None if m is None else int(m.group(1)) + int(m.group(2))
 
 
@@ -101,6 +104,58 @@ def adjoint_bench(args) -> dict:
     }
 
 
+SIDE_WORKLOADS = {
+    2: ("rescue_stress", "1024 regions x 64 reads of 250 bp x 8 haplotypes of 600 bp, quals "
+                         "U(30,41), 30% of reads from unrelated sequence"),
+    3: ("pd", "PD-HMM, 1024 regions of the headline shapes, flags at 5% of haplotype positions "
+              "(undetermined SNPs with an alternate, deletion runs of 1-8)"),
+}
+
+
+def side_bench(args, config: int) -> dict:
+    """A secondary workload through the same engine: graph-replayed FP32 + FP64-rescue GCUPS, the
+    rescued fraction and the FP64 rescue's share of device time (eager run, HIP events), and every
+    pair against the engine's FP64 mode."""
+    import fgkl_amd as F
+    from fgkl_amd.synthetic import workload
+
+    name, desc = SIDE_WORKLOADS[config]
+    data = workload(config, seed=args.seed)
+    cells, n_pairs = data.cells(), data.n_pairs()
+    b = F.Batch(data)
+    st = b.run(F.MODE_FP32_RESCUE, share=True, graph=False)
+    eager = b.results()
+    primary_ms, rescue_ms, total_ms = st.primary_ms, st.rescue_ms, st.total_ms
+    rescued = int(st.rescued_pairs)
+    b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+    t0 = time.perf_counter()
+    steps = 3
+    for _ in range(steps):
+        b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+    elapsed = time.perf_counter() - t0
+    assert np.array_equal(b.results(), eager), f"{name}: graph replay differs from the eager run"
+    st64 = b.run(F.MODE_FP64, share=True)
+    d = np.abs(eager - b.results())
+    b.close()
+    gcups = cells * steps / elapsed / 1e9
+    achieved = FLOPS_PER_CELL * cells / (primary_ms * 1e-3) / 1e12
+    log(f"{name}: {cells:.3e} cells, {gcups:.1f} GCUPS, rescued {rescued}/{n_pairs}, "
+        f"fp64 rescue {rescue_ms:.1f} of {total_ms:.1f} ms; max |d| vs fp64 {d.max():.3g}")
+    return {
+        "workload": desc, "regions": data.n_regions, "pairs": n_pairs, "cells": cells,
+        "gcups": round(gcups, 2), "ms_per_step": round(elapsed * 1e3 / steps, 3),
+        "fp32_sweep_ms": round(primary_ms, 3), "fp64_rescue_ms": round(rescue_ms, 3),
+        "rescue": {"pairs": rescued, "fraction": round(rescued / n_pairs, 6),
+                   "fp64_time_share": round(rescue_ms / total_ms, 4) if total_ms else None},
+        "fp64_mode_gcups": round(cells / (st64.total_ms * 1e-3) / 1e9, 2),
+        "roofline_fp32_sweeps": {"bound": "valu", "achieved": round(achieved, 3),
+                                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4)},
+        "parity": {"max_abs_dlog10L_vs_fp64": float(d.max()), "pairs_over_1e-4": int((d > 1e-4).sum()),
+                   "nonfinite": int((~np.isfinite(eager)).sum())},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +169,11 @@ def main() -> int:
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 per-call run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient workload")
     ap.add_argument("--adjoint-regions", type=int, default=4096)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launches")
+    ap.add_argument("--launch", default="", metavar="R:ROWS:BLOCK:BOOST,...",
+                    help="set these bucket launch configurations instead of tuning")
+    ap.add_argument("--no-side", action="store_true",
+                    help="skip the rescue-stress and PD workloads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -147,6 +207,25 @@ def main() -> int:
     batch = F.Batch(data)
     prep_s = time.perf_counter() - t0
     log(f"planned + uploaded in {prep_s:.2f}s")
+
+    # Launch autotuning per rows-per-lane bucket (untimed, like planning), or the configurations
+    # given with --launch (so a profiling run can repeat a tuned run's kernels).
+    tune = None
+    if args.launch:
+        for item in args.launch.split(","):
+            R, rows, block, boost = (int(x) for x in item.split(":"))
+            batch.set_launch(R, rows, block, bool(boost))
+    elif not args.no_tune:
+        tune = batch.tune(share=True)
+    if tune is not None:
+        chosen = {f"R={R}": {"rows": tune.rows[R - 1], "block": tune.block[R - 1],
+                             "boost": bool(tune.boost[R - 1]),
+                             "default_ms": round(tune.default_ms[R - 1], 3),
+                             "tuned_ms": round(tune.tuned_ms[R - 1], 3)}
+                  for R in range(1, 9) if tune.rows[R - 1]}
+        log(f"autotune: {tune.candidates} configurations in {tune.seconds:.2f}s; " +
+            ", ".join(f"{k}: rows {v['rows']} block {v['block']} boost {int(v['boost'])} "
+                      f"{v['default_ms']}->{v['tuned_ms']} ms" for k, v in chosen.items()))
 
     # Eager run: per-bucket kernel times (HIP events on the engine's stream) and results.
     st = batch.run(F.MODE_FP32_RESCUE, share=True, graph=False)
@@ -184,14 +263,24 @@ def main() -> int:
     # Roofline: the dominant sweep kernel (the bucket with the most kernel time).
     dom = int(np.argmax(bucket_ms))
     achieved = FLOPS_PER_CELL * bc[dom] / (bucket_ms[dom] * 1e-3) / 1e12
+    d_rows, d_block, d_boost = batch.get_launch(dom + 1)
+    launches = ",".join(f"{R}:" + ":".join(str(int(x)) for x in batch.get_launch(R))
+                        for R in range(1, 9) if bc[R - 1])
     result["roofline"] = {
-        "bound": "valu", "kernel": f"sweep_kernel<float, R={dom + 1}, plain>",
+        "bound": "valu",
+        "kernel": f"sweep_kernel<float, {d_rows}, false, {str(d_boost).lower()}> "
+                  f"(bucket R={dom + 1}, {d_block} threads per workgroup)",
         "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(dom + 1),
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(
+            f"{dom + 1}:{d_rows}:{d_block}:{int(d_boost)}"),
         "traffic_source": f"{PROFILE.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
         "algorithmic_flops_per_launch": int(FLOPS_PER_CELL * bc[dom]),
         "launch_ms": round(bucket_ms[dom], 3),
+        "launches": launches,
     }
+    if tune is not None:
+        result["autotune"] = {"seconds": round(tune.seconds, 3), "candidates": tune.candidates,
+                              "buckets": chosen}
     result["kernels"] = {
         "fp32_sweep_ms": round(primary_ms, 3), "fp64_rescue_ms": round(rescue_ms, 3),
         "bucket_ms": [round(x, 3) for x in bucket_ms],
@@ -218,8 +307,24 @@ def main() -> int:
                             "nonfinite": int((~np.isfinite(fp32_res)).sum())}
         log(f"fp64 per-call: {base_ms:.1f} ms, {base_gcups:.1f} GCUPS; max |d| {d.max():.3g}")
 
+    # End to end: one call from host arrays to host results (plan, upload, one graph run, copy
+    # back), PCIe included.  Not the headline, which times resident inputs.
+    t0 = time.perf_counter()
+    e2e = F.Batch(data)
+    e2e.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+    e2e_res = e2e.results()
+    e2e_s = time.perf_counter() - t0
+    e2e.close()
+    assert np.array_equal(e2e_res, fp32_res), "a fresh batch gave different results"
+    result["end_to_end"] = {"seconds": round(e2e_s, 4), "gcups": round(cells / e2e_s / 1e9, 2),
+                            "includes": "plan on the host, upload, graph build + run, download"}
+    log(f"end to end: {e2e_s * 1e3:.1f} ms, {cells / e2e_s / 1e9:.1f} GCUPS")
+
     if not args.no_adjoint:
         result["adjoint"] = adjoint_bench(args)
+    if not args.no_side:
+        for config in SIDE_WORKLOADS:
+            result[SIDE_WORKLOADS[config][0]] = side_bench(args, config)
 
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)

@@ -300,6 +300,23 @@ class Batch:
         _check(lib().fgkl_batch_run(self._h, mode, int(share), int(graph), C.byref(st)))
         return st
 
+    def tune(self, share: bool = True):
+        """Times each FP32 bucket's launch configurations and keeps the fastest
+        (fgkl_batch_tune); returns the report."""
+        rep = _lib.TuneReport()
+        _check(lib().fgkl_batch_tune(self._h, int(share), C.byref(rep)))
+        return rep
+
+    def set_launch(self, bucket_rows: int, rows: int = 0, block: int = 256, boost: bool = False):
+        _check(lib().fgkl_batch_set_launch(self._h, bucket_rows, rows, block, int(boost)))
+
+    def get_launch(self, bucket_rows: int) -> tuple[int, int, bool]:
+        """(rows per lane, threads per workgroup, occupancy-raised build) of bucket R."""
+        r, b, o = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().fgkl_batch_get_launch(self._h, bucket_rows, C.byref(r), C.byref(b),
+                                           C.byref(o)))
+        return r.value, b.value, bool(o.value)
+
     def results(self) -> np.ndarray:
         n = int(lib().fgkl_batch_pairs(self._h))
         out = np.empty(n, dtype=np.float64)

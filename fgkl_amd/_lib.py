@@ -28,6 +28,18 @@ class RunStats(C.Structure):
     ]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int32 * 8),
+        ("block", C.c_int32 * 8),
+        ("boost", C.c_int32 * 8),
+        ("default_ms", C.c_double * 8),
+        ("tuned_ms", C.c_double * 8),
+        ("candidates", C.c_int32),
+        ("seconds", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/fgkl.h declares.
 SIGNATURES = {
     "fgkl_last_error": (C.c_char_p, []),
@@ -59,6 +71,10 @@ SIGNATURES = {
     "fgkl_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "fgkl_batch_results": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "fgkl_batch_pairs": (C.c_int64, [C.c_void_p]),
+    "fgkl_batch_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TuneReport)]),
+    "fgkl_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fgkl_batch_get_launch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fgkl_batch_free": (None, [C.c_void_p]),
     "fgkl_synth_region": (C.c_void_p, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                        C.c_int]),

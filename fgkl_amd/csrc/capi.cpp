@@ -253,6 +253,47 @@ int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoo
 
 int64_t fgkl_batch_pairs(const fgkl_batch* batch) { return batch ? batch->batch->n_pairs() : 0; }
 
+int fgkl_batch_tune(fgkl_batch* batch, int share_prefixes, fgkl_tune_report* report) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  try {
+    TuneReport rep;
+    const int rc = batch->batch->tune(share_prefixes, &rep);
+    if (rc == kOk && report) {
+      *report = fgkl_tune_report{};
+      for (int i = 0; i < 8; ++i) {
+        const bool used = rep.cfg[i].rows != 0;
+        report->rows[i] = used ? rep.cfg[i].rows : 0;
+        report->block[i] = used ? rep.cfg[i].block : 0;
+        report->boost[i] = used && rep.cfg[i].boost ? 1 : 0;
+        report->default_ms[i] = rep.default_ms[i];
+        report->tuned_ms[i] = rep.tuned_ms[i];
+      }
+      report->candidates = rep.candidates;
+      report->seconds = rep.seconds;
+    }
+    return rc;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int fgkl_batch_set_launch(fgkl_batch* batch, int bucket_rows, int rows, int block, int boost) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  if (boost != 0 && boost != 1) return fail(kInvalidArgument, "boost must be 0 or 1");
+  return batch->batch->set_launch(bucket_rows, LaunchCfg{rows, block, boost != 0});
+}
+
+int fgkl_batch_get_launch(const fgkl_batch* batch, int bucket_rows, int* rows, int* block,
+                          int* boost) {
+  if (!batch || !rows || !block || !boost) return fail(kInvalidArgument, "null argument");
+  LaunchCfg c;
+  if (int rc = batch->batch->get_launch(bucket_rows, &c)) return rc;
+  *rows = c.rows;
+  *block = c.block;
+  *boost = c.boost ? 1 : 0;
+  return kOk;
+}
+
 void fgkl_batch_free(fgkl_batch* batch) { delete batch; }
 
 fgkl_synth* fgkl_synth_region(uint64_t seed, int64_t num_reads, int64_t read_len,

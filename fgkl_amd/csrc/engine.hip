@@ -1,28 +1,10 @@
-// The MI355X (gfx950) PairHMM / PD-HMM engine: device kernels and the batch runner.
-//
-// Kernel layout (one work item per wave64):
-//  * A read of n bases is laid out down the lanes: lane l owns R consecutive rows, with
-//    R = ceil((n+1)/64) (1..8) and L = ceil((n+1)/R) live lanes.  The first R*L-n >= 1 rows of
-//    lane 0 are identity rows that reproduce DP row 0 exactly (prior 0, M->I 0, I->I 1, M->D 0,
-//    D->D 1, left value (0, 0, C/m)), so the read's last row is always the last register of lane
-//    L-1 and lane 0 never needs the row above it.
-//  * The haplotype is swept along anti-diagonals: at step t lane l computes column t-l+1 for its
-//    R rows.  The values of the row above a lane's first row arrive from lane l-1 with a DPP
-//    wave_shr:1 move of the registers that lane computed one step earlier.  The haplotype column
-//    word travels the same way, lane 0 reading it from an LDS copy of the haplotype.
-//  * A work item is one read against every sorted haplotype of its region.  A haplotype starts
-//    at the column where it diverges from the previous sorted haplotype, from a snapshot of that
-//    column that an earlier sweep wrote into the wave's HBM scratch slot, rescaled by the ratio
-//    of haplotype lengths (the recurrence is linear in C/m; kernel.rs:610-678 does the same on
-//    the CPU).  Rescue and per-pair work items run a single haplotype from column 0.
-//  * FP32 sweeps scale by 2^120 and flag a pair whose raw sum is below 1e-28 or NaN
-//    (simd.rs:102-114, kernel.rs:897-914): it is appended to a device list that the FP64 kernel
-//    drains in the same stream.  FP64 sweeps scale by 2^1020 and are never rescued.
-//  * The recurrence is not a dense contraction, so MFMA is not used: every cell is 4 multiplies
-//    and 4 FMAs on the vector ALU (kernel.rs:1011-1013 operation order).
+// The MI355X (gfx950) PairHMM / PD-HMM engine's batch runner: device state and tables, batch
+// upload, the per-bucket launches (eagerly or as one captured hipGraph), the FP64 rescue, the
+// per-region FP64 baseline and the launch autotuner.  The kernels are in sweep.hpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -30,536 +12,11 @@
 #include <string>
 #include <vector>
 
-#include "fgkl_internal.hpp"
 #include "fgkl_engine.hpp"
-#include "wave64.hpp"
+#include "fgkl_internal.hpp"
+#include "sweep.hpp"
 
 namespace fgkl {
-
-// ---------------------------------------------------------------------------------------------
-// Numeric constants (simd.rs:102-141).
-template <typename T> struct Num;
-template <> struct Num<float> {
-  static constexpr double kScale = 0x1p120;                       // 2^120
-  static constexpr double kLog10Scale = 120.0 * 0.301029995663981195213738894724493027;
-  static constexpr double kMinAccepted = 1e-28;
-};
-template <> struct Num<double> {
-  static constexpr double kScale = 0x1p1020;                      // 2^1020
-  static constexpr double kLog10Scale = 1020.0 * 0.301029995663981195213738894724493027;
-  static constexpr double kMinAccepted = 0.0;                     // never rescued
-};
-
-template <typename T> struct DevTab {
-  const T* pm;  // 1 - e(q)
-  const T* px;  // e(q) / 3
-  const T* ep;  // e(q)
-  const T* im;  // 1 - e(gcp)
-  const T* mm;  // quantized match-to-match, triangular
-};
-
-struct SweepArgs {
-  const uint8_t *bases, *quals, *ins, *del, *gcp;
-  const int64_t* read_off;
-  const int32_t* read_region;
-  const int32_t* read_local;
-  const int64_t* region_out;
-  const int32_t* region_hap0;
-  const int32_t* region_nhap;
-  const HapPlan* plan;
-  const SnapEntry* snaps;
-  const uint8_t* hap_bytes;
-  const uint32_t* hap_words;
-  const int32_t* items;        // (read, read or -1) per item, or (read, sorted hap) in pair mode
-  int32_t n_items;
-  const int32_t* n_items_dev;  // when set, the item count is read from device memory
-  int32_t* counter;            // work-queue head for this launch
-  int32_t pair_mode;
-  int32_t share;
-  void* scratch;
-  int64_t scratch_wave_stride;  // elements of T per wave
-  double* out;
-  int32_t* rescue_pairs;
-  int32_t* rescue_count;
-  int32_t rescue;
-  int32_t lds_words;           // LDS words per wave for the staged haplotype
-};
-
-// Qualities above 254 are clamped (model.rs:53-64).
-__device__ __forceinline__ int dclamp(uint32_t q) { return q > uint32_t(kMaxQual) ? kMaxQual : int(q); }
-
-__device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
-  switch (b) {
-    case 'A': case 'a': return 1u;
-    case 'C': case 'c': return 2u;
-    case 'G': case 'g': return 4u;
-    case 'T': case 't': return 8u;
-    default: return 0u;
-  }
-}
-
-// Per-lane parameters of its R rows (Workspace::prepare, kernel.rs:485-529).
-template <typename T, int R, bool PD> struct Rows {
-  T pm[R], px[R], mm[R], im[R], mi[R], md[R], gc[R];
-  uint32_t rb[R];
-  uint32_t ralt[R];
-  uint32_t kind[R];  // 0: row >= 2, 1: row 1, 2: identity row (or a lane past the read)
-};
-
-template <typename T, int R, bool PD>
-__device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& tab, int64_t o,
-                                          int L, int pad, int lane, Rows<T, R, PD>& rw) {
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int arow = R * lane + k - pad;
-    if (lane < L && arow >= 0) {
-      const int64_t i = o + arow;
-      const uint32_t b = a.bases[i];
-      const int q = dclamp(a.quals[i]), iq = dclamp(a.ins[i]), dq = dclamp(a.del[i]);
-      const int gq = dclamp(a.gcp[i]);
-      rw.pm[k] = tab.pm[q];
-      rw.px[k] = b == 'N' ? rw.pm[k] : tab.px[q];  // an N read base matches everything
-      rw.mm[k] = tab.mm[((iq >= dq ? iq * (iq + 1) : dq * (dq + 1)) >> 1) + (iq >= dq ? dq : iq)];
-      rw.mi[k] = tab.ep[iq];
-      rw.md[k] = tab.ep[dq];
-      rw.gc[k] = tab.ep[gq];  // insertion and deletion extension alike (model.rs:73-75)
-      rw.im[k] = tab.im[gq];
-      rw.rb[k] = b;
-      rw.ralt[k] = alt_bit(b);
-      rw.kind[k] = arow == 0 ? 1u : 0u;
-    } else {
-      // Identity rows: D->D 1 keeps row 0's C/m; I->I 1 is harmless because the row above an
-      // identity row is zero.  Lanes past the read: all zero.
-      rw.pm[k] = T(0); rw.px[k] = T(0); rw.mm[k] = T(0); rw.im[k] = T(0);
-      rw.mi[k] = T(0); rw.md[k] = T(0);
-      rw.gc[k] = lane < L ? T(1) : T(0);
-      rw.rb[k] = 0x100u;
-      rw.ralt[k] = 0u;
-      rw.kind[k] = 2u;
-    }
-  }
-}
-
-// Number of scratch values a lane keeps per snapshot slot, and the record stride: a slot is
-// 64 per-lane records of snap_stride values, so the one lane that records a column writes one
-// contiguous, 16-byte-aligned block.
-template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 7 : 3 * R + 4; }
-template <int R, bool PD> constexpr int snap_stride() { return (snap_values<R, PD>() + 3) & ~3; }
-
-// The values of one DP row in one column: match, insertion, deletion and (PD) branch values.
-template <typename T> struct Cell {
-  T m, x, y, bm, bx, by;
-};
-
-// What a lane carries from one anti-diagonal step to the next: its R rows' values in the column
-// it computed last (the left neighbours of its next column), its running last-row sum, and the
-// word of the column it is on.  The row above its first row (diagonal and up neighbours) lives
-// in Cell values the sweep passes in.
-template <typename T, int R, bool PD> struct Lane {
-  T m[R], x[R], y[R];
-  T bm[PD ? R : 1], bx[PD ? R : 1], by[PD ? R : 1];
-  T acc;
-  uint32_t w;
-};
-
-// Bitwise AND of a value with a per-lane 32-bit mask (both halves of a double).
-__device__ __forceinline__ float and_bits(float v, uint32_t k) {
-  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & k);
-}
-__device__ __forceinline__ double and_bits(double v, uint32_t k) {
-  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & ((uint64_t(k) << 32) | k));
-}
-
-// The row above a lane's first row in the column it computes next: lane l-1's last row, moved
-// with DPP before either lane updates.  Lane 0 receives zeros; its first row is always an
-// identity row, which ignores them.  The first lane of a second read in the wave starts with
-// identity rows too; of the values above them only the insertion value would leak through (I->I
-// 1 in an identity row), so it is masked off there (`keep` is zero in that lane, all ones
-// elsewhere).  The mask is applied after the move by an AND on an opaque mask: a select would
-// let the compiler run the move only in the lanes that keep it, and a DPP move reads a lane
-// outside EXEC as zero, which would also cut the insertion value reaching the lane below.
-template <typename T, int R, bool PD>
-__device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u, uint32_t keep) {
-  u.m = shr1z(s.m[R - 1]);
-  u.x = and_bits(shr1z(s.x[R - 1]), keep);
-  u.y = shr1z(s.y[R - 1]);
-  if constexpr (PD) {
-    u.bm = shr1z(s.bm[R - 1]);
-    u.bx = shr1z(s.bx[R - 1]);
-    u.by = shr1z(s.by[R - 1]);
-  }
-}
-
-// The R cells of one column.  Rows are updated in place: match, deletion and branch values
-// bottom-up (each row still reads the old column of the row above), then insertion values
-// top-down (each row reads the new column of the row above), so no value is copied.
-template <typename T, int R, bool PD>
-__device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
-                                      const Cell<T>& d, const Cell<T>& u) {
-  const uint32_t w = s.w;
-  const uint32_t hb = w & 0xFFu;
-  const bool hN = hb == uint32_t('N');
-  uint32_t st[PD ? R : 1], de[PD ? R : 1];
-  if constexpr (PD) {
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      if (rw.kind[k] == 2u) {
-        st[k] = kInsideDel;  // identity rows reproduce row 0, branch values included
-        de[k] = 0u;
-      } else {
-        st[k] = rw.kind[k] == 1u ? ((w >> 12) & 3u) : ((w >> 14) & 3u);
-        de[k] = (w >> 16) & 1u;
-      }
-    }
-  }
-#pragma unroll
-  for (int k = R - 1; k >= 0; --k) {
-    const T pdm = k ? s.m[k - 1] : d.m;
-    const T pdx = k ? s.x[k - 1] : d.x;
-    const T pdy = k ? s.y[k - 1] : d.y;
-    bool match = hN || hb == rw.rb[k];
-    if constexpr (PD) match = match || (((w >> 8) & rw.ralt[k]) != 0u);
-    const T prior = match ? rw.pm[k] : rw.px[k];
-    if constexpr (!PD) {
-      // kernel.rs:1011-1013
-      const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[k]);
-      s.m[k] = prior * fma(pdy, rw.im[k], fma(pdx, rw.im[k], pdm * rw.mm[k]));
-      s.y[k] = Y;
-    } else {
-      // pd_reference.rs:62-94 for the column's deletion state.
-      const bool A = st[k] == kAfterDel, I = st[k] == kInsideDel, N = st[k] == kNormal;
-      const T pdbm = k ? s.bm[k - 1] : d.bm;
-      const T pdbx = k ? s.bx[k - 1] : d.bx;
-      const T pdby = k ? s.by[k - 1] : d.by;
-      const T dme = A ? fmax(pdbm, pdm) : pdm;
-      const T dxe = A ? fmax(pdbx, pdx) : pdx;
-      const T dye = A ? fmax(pdby, pdy) : pdy;
-      const T lme = A ? fmax(s.bm[k], s.m[k]) : s.m[k];
-      const T lxe = A ? fmax(s.bx[k], s.x[k]) : s.x[k];
-      const T lye = A ? fmax(s.by[k], s.y[k]) : s.y[k];
-      const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
-      const T BM = N ? s.m[k] : (I ? s.bm[k] : lme);
-      const T BX = N ? s.x[k] : (I ? s.bx[k] : lxe);
-      const T BY = N ? s.y[k] : (I ? s.by[k] : lye);
-      s.m[k] = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
-      s.y[k] = Y;
-      s.bm[k] = BM;
-      s.bx[k] = BX;
-      s.by[k] = BY;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const T pum = k ? s.m[k - 1] : u.m;
-    const T pux = k ? s.x[k - 1] : u.x;
-    if constexpr (!PD) {
-      s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[k]);
-    } else {
-      // pd_reference.rs:95-100
-      const T pubm = k ? s.bm[k - 1] : u.bm;
-      const T pubx = k ? s.bx[k - 1] : u.bx;
-      const T ume = de[k] ? fmax(pubm, pum) : pum;
-      const T uxe = de[k] ? fmax(pubx, pux) : pux;
-      s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[k]);
-    }
-  }
-  s.acc += s.m[R - 1] + s.x[R - 1];
-}
-
-// Writes a lane's state into its snapshot record (value by value); `u` is the row above
-// its first row in the column just computed.
-template <typename T, int R, bool PD>
-__device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Cell<T>& u, T* d) {
-  constexpr int NV = snap_values<R, PD>();
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    d[(3 * k + 0)] = s.m[k];
-    d[(3 * k + 1)] = s.x[k];
-    d[(3 * k + 2)] = s.y[k];
-    if constexpr (PD) {
-      d[(3 * R + 3 * k + 0)] = s.bm[k];
-      d[(3 * R + 3 * k + 1)] = s.bx[k];
-      d[(3 * R + 3 * k + 2)] = s.by[k];
-    }
-  }
-  constexpr int U = PD ? 6 * R : 3 * R;
-  d[(U + 0)] = u.m;
-  d[(U + 1)] = u.x;
-  d[(U + 2)] = u.y;
-  if constexpr (PD) {
-    d[(U + 3)] = u.bm;
-    d[(U + 4)] = u.bx;
-    d[(U + 5)] = u.by;
-  }
-  d[(NV - 1)] = s.acc;
-}
-
-// A staged column word carries, in bits 20..31, one plus the scratch slot that must record the
-// column's state (0: none).  The word travels down the lanes with the column, so each lane sees
-// the mark exactly when it has just computed that column.
-constexpr uint32_t kSlotShift = 20;
-constexpr int kMaxSlots = 4095;
-
-// One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
-// column start+1+t (zero beyond W, 16-byte aligned).  Returns the lane's raw scaled sum over the
-// read's last row (meaningful in lane L-1).  `cap` receives lane cap_lane's sum right after step
-// cap_t: the first read of a two-read wave finishes column W there, and its last lane would go on
-// summing the columns past W while the second read's lanes finish.
-template <typename T, int R, bool PD, bool SNAP>
-__device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
-                   const T* snap_in, T scale, T* scratch, bool row0_above, uint32_t keep,
-                   int cap_t, int cap_lane, T& cap) {
-  constexpr int NV = snap_values<R, PD>();
-  Lane<T, R, PD> s;
-  Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
-  if (snap_in) {
-    const T* p = snap_in + lane * snap_stride<R, PD>();
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      s.m[k] = p[(3 * k + 0)] * scale;
-      s.x[k] = p[(3 * k + 1)] * scale;
-      s.y[k] = p[(3 * k + 2)] * scale;
-      if constexpr (PD) {
-        s.bm[k] = p[(3 * R + 3 * k + 0)] * scale;
-        s.bx[k] = p[(3 * R + 3 * k + 1)] * scale;
-        s.by[k] = p[(3 * R + 3 * k + 2)] * scale;
-      }
-    }
-    constexpr int U = PD ? 6 * R : 3 * R;
-    P.m = p[(U + 0)] * scale;
-    P.x = p[(U + 1)] * scale;
-    P.y = p[(U + 2)] * scale;
-    if constexpr (PD) {
-      P.bm = p[(U + 3)] * scale;
-      P.bx = p[(U + 4)] * scale;
-      P.by = p[(U + 5)] * scale;
-    }
-    s.acc = p[(NV - 1)] * scale;
-  } else {
-    // Column 0: zeros below row 0; identity rows carry row 0's deletion value C/m.
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      s.m[k] = T(0);
-      s.x[k] = T(0);
-      s.y[k] = (rw.kind[k] == 2u && lane < L) ? init : T(0);
-      if constexpr (PD) {
-        s.bm[k] = T(0);
-        s.bx[k] = T(0);
-        s.by[k] = T(0);
-      }
-    }
-    s.acc = T(0);
-    // When identity rows fill lane 0, read row 1 is lane 1's first row and its diagonal
-    // neighbour in column 1 is DP row 0's deletion value (Y[0][0] = C/m, reference.rs:27-28).
-    if (row0_above) P.y = init;
-  }
-  s.w = 0;
-
-  auto record = [&](const Cell<T>& u) {
-    if constexpr (SNAP) {
-      const uint32_t sl = s.w >> kSlotShift;
-      if (sl)
-        store_snapshot<T, R, PD>(s, u, scratch + (size_t(sl - 1) * 64 + lane) * snap_stride<R, PD>());
-    }
-  };
-  const int steps = W + L - 1;
-  // Ramp: lane l starts at step l and keeps its column-`start` state until then.  Rounded up to
-  // four steps so the steady loop reads aligned groups of four column words.
-  const int ramp = min(steps, (L - 1 + 3) & ~3);
-  int t = 0;
-  Cell<T> Q;
-  for (; t < ramp; ++t) {
-    s.w = shr1(lw[t], s.w);
-    shift_up(s, Q, keep);
-    if (t >= lane) {
-      cells(s, rw, P, Q);
-      P = Q;
-      record(P);
-    }
-    if (t == cap_t) cap = readlane(s.acc, cap_lane);
-  }
-  // Steady state: every lane is live; the diagonal and up neighbours alternate between P and Q.
-  // It stops by step cap_t; the single steps after it test for the capture.
-  const int steady_end = min(steps, cap_t + 1);
-  uint4 q = *reinterpret_cast<const uint4*>(lw + t);
-  for (; t + 4 <= steady_end; t += 4) {
-    const uint4 cq = q;
-    q = *reinterpret_cast<const uint4*>(lw + t + 4);
-    s.w = shr1(cq.x, s.w);
-    shift_up(s, Q, keep);
-    cells(s, rw, P, Q);
-    record(Q);
-    s.w = shr1(cq.y, s.w);
-    shift_up(s, P, keep);
-    cells(s, rw, Q, P);
-    record(P);
-    s.w = shr1(cq.z, s.w);
-    shift_up(s, Q, keep);
-    cells(s, rw, P, Q);
-    record(Q);
-    s.w = shr1(cq.w, s.w);
-    shift_up(s, P, keep);
-    cells(s, rw, Q, P);
-    record(P);
-  }
-  if (t - 1 == cap_t) cap = readlane(s.acc, cap_lane);
-  for (; t < steps; ++t) {
-    s.w = shr1(lw[t], s.w);
-    shift_up(s, Q, keep);
-    cells(s, rw, P, Q);
-    P = Q;
-    record(P);
-    if (t == cap_t) cap = readlane(s.acc, cap_lane);
-  }
-  return s.acc;
-}
-
-// Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded, and marks
-// the columns this sweep must record with their scratch slots.
-template <bool PD>
-__device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int L,
-                                          int lane, uint32_t* lw, int nsnap) {
-  const int W = p.len - start;
-  const int total = W + L + 8;  // the steady loop reads one group of four ahead
-  wave_sync_lds();
-  for (int i = lane; i < total; i += 64) {
-    uint32_t v = 0;
-    if (i < W) v = PD ? a.hap_words[p.off + start + i] : uint32_t(a.hap_bytes[p.off + start + i]);
-    lw[i] = v;
-  }
-  if (nsnap) {
-    wave_sync_lds();
-    for (int i = lane; i < nsnap; i += 64) {
-      const SnapEntry e = a.snaps[p.snap0 + i];
-      lw[e.col - start - 1] |= uint32_t(e.slot + 1) << kSlotShift;
-    }
-  }
-  wave_sync_lds();
-}
-
-template <typename T, int R, bool PD>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a, DevTab<T> tab) {
-  extern __shared__ __attribute__((aligned(16))) uint32_t lds_all[];
-  const int lane = threadIdx.x & 63;
-  const int wave_in_block = threadIdx.x >> 6;
-  const int64_t wave_global = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave_in_block;
-  uint32_t* lw = lds_all + wave_in_block * a.lds_words;
-  T* scratch = a.scratch ? static_cast<T*>(a.scratch) + wave_global * a.scratch_wave_stride : nullptr;
-  const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
-  constexpr int NV = snap_values<R, PD>();
-
-  for (;;) {
-    // Every lane adds one, so a claim advances the counter by 64 and every lane's old value lies
-    // in [64 i, 64 i + 63].  A lane-dependent branch around the claim, next to the lane-0 stores
-    // at the end of an item, would let the optimiser thread the loop's back edge past the claim
-    // for lanes 1..63, which would then read an item without lane 0.
-    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, 1)) >> 6;
-    if (item >= n_items) break;
-
-    // Everything below is uniform across the wave; readfirstlane keeps it in scalar registers.
-    // A main-pass item is one read, or two reads of one region sharing the wave (read A in lanes
-    // [0, La), read B in [La, La+Lb)); a pair-mode item is one (read, sorted haplotype).
-    int ra, rb, k_begin, k_end;
-    ra = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
-    if (a.pair_mode) {
-      rb = -1;
-      k_begin = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
-      k_end = k_begin + 1;
-    } else {
-      rb = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
-      k_begin = 0;
-      k_end = -1;
-    }
-    const bool two = rb >= 0;
-    const int64_t oa = a.read_off[ra];
-    const int na = __builtin_amdgcn_readfirstlane(int(a.read_off[ra + 1] - oa));
-    const int64_t ob = two ? a.read_off[rb] : 0;
-    const int nb = two ? __builtin_amdgcn_readfirstlane(int(a.read_off[rb + 1] - ob)) : 0;
-    const int La = (na + R) / R;  // n rows plus at least one identity row
-    const int Lb = two ? (nb + R) / R : 0;
-    const int L = La + Lb;
-    const int pada = R * La - na, padb = R * Lb - nb;
-    const int g = __builtin_amdgcn_readfirstlane(a.read_region[ra]);
-    const int hap0 = __builtin_amdgcn_readfirstlane(a.region_hap0[g]);
-    const int H = __builtin_amdgcn_readfirstlane(a.region_nhap[g]);
-    if (k_end < 0) k_end = H;
-    const int64_t out_a = a.region_out[g] + int64_t(a.read_local[ra]) * H;
-    const int64_t out_b = two ? a.region_out[g] + int64_t(a.read_local[rb]) * H : 0;
-
-    // This lane's read: its offset, lanes, identity rows and lane index within the read.
-    const bool in_b = lane >= La;
-    Rows<T, R, PD> rw;
-    load_rows<T, R, PD>(a, tab, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
-                        in_b ? lane - La : lane, rw);
-    // Each read's second lane starts below DP row 0 when identity rows fill its first lane.
-    const bool row0_above = (lane == 1 && pada == R) || (two && lane == La + 1 && padb == R);
-    // Zero in the first lane of read B, all ones elsewhere; opaque to the optimiser (see
-    // shift_up).
-    uint32_t keep = (two && lane == La) ? 0u : ~0u;
-    asm volatile("" : "+v"(keep));
-
-    double prev_a = 0.0, prev_b = 0.0;
-    for (int k = k_begin; k < k_end; ++k) {
-      HapPlan p = a.plan[hap0 + k];
-      p.len = __builtin_amdgcn_readfirstlane(p.len);
-      p.lcp = __builtin_amdgcn_readfirstlane(p.lcp);
-      p.dup = __builtin_amdgcn_readfirstlane(p.dup);
-      p.nsnap = __builtin_amdgcn_readfirstlane(p.nsnap);
-      p.snap0 = __builtin_amdgcn_readfirstlane(p.snap0);
-      p.start_slot = __builtin_amdgcn_readfirstlane(p.start_slot);
-      double res_a, res_b;
-      if (!a.pair_mode && p.dup) {
-        res_a = prev_a;  // kernel.rs:237-246
-        res_b = prev_b;
-      } else {
-        const bool shared = a.share && !a.pair_mode;
-        const int start = shared ? p.lcp : 0;
-        const int nsnap = shared ? p.nsnap : 0;
-        const T* snap_in = nullptr;
-        T scale = T(1);
-        if (shared && p.start_slot >= 0) {
-          snap_in = scratch + size_t(p.start_slot) * 64 * snap_stride<R, PD>();
-          scale = T(double(p.start_src_len) / double(p.len));  // kernel.rs:641
-        }
-        const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
-        stage_hap<PD>(a, p, start, L, lane, lw, nsnap);
-        // Read A's last lane finishes column W at step W + La - 2.
-        const int W = p.len - start;
-        T acc_a = T(0);
-        const T acc =
-            nsnap ? sweep<T, R, PD, true>(rw, lw, W, L, lane, init, snap_in, scale, scratch,
-                                          row0_above, keep, W + La - 2, La - 1, acc_a)
-                  : sweep<T, R, PD, false>(rw, lw, W, L, lane, init, snap_in, scale, scratch,
-                                           row0_above, keep, W + La - 2, La - 1, acc_a);
-        // A read's last row is its last lane's last register (kernel.rs:897-914).
-        auto finish = [](double raw) {
-          if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted)) return __builtin_nan("");
-          return log10(raw) - Num<T>::kLog10Scale;
-        };
-        res_a = finish(double(acc_a));
-        res_b = finish(double(readlane(acc, L - 1)));
-      }
-      if (lane == 0) {
-        a.out[out_a + p.src] = res_a;
-        if (a.rescue && res_a != res_a) {
-          const int idx = atomicAdd(a.rescue_count, 1);
-          a.rescue_pairs[2 * idx] = ra;
-          a.rescue_pairs[2 * idx + 1] = k;
-        }
-        if (two) {
-          a.out[out_b + p.src] = res_b;
-          if (a.rescue && res_b != res_b) {
-            const int idx = atomicAdd(a.rescue_count, 1);
-            a.rescue_pairs[2 * idx] = rb;
-            a.rescue_pairs[2 * idx + 1] = k;
-          }
-        }
-      }
-      prev_a = res_a;
-      prev_b = res_b;
-    }
-  }
-}
 
 __global__ void fill_i32(int32_t* p, int n, int32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -586,34 +43,17 @@ __global__ void dpp_selftest_kernel(const int32_t* in, int32_t* out) {
 
 namespace {
 
-using KernelF = void (*)(SweepArgs, DevTab<float>);
-using KernelD = void (*)(SweepArgs, DevTab<double>);
-
-template <typename T> struct KernelTable;
-template <> struct KernelTable<float> {
-  static constexpr KernelF plain[kMaxRowsPerLane] = {
-      sweep_kernel<float, 1, false>, sweep_kernel<float, 2, false>, sweep_kernel<float, 3, false>,
-      sweep_kernel<float, 4, false>, sweep_kernel<float, 5, false>, sweep_kernel<float, 6, false>,
-      sweep_kernel<float, 7, false>, sweep_kernel<float, 8, false>};
-  static constexpr KernelF pd[kMaxRowsPerLane] = {
-      sweep_kernel<float, 1, true>, sweep_kernel<float, 2, true>, sweep_kernel<float, 3, true>,
-      sweep_kernel<float, 4, true>, sweep_kernel<float, 5, true>, sweep_kernel<float, 6, true>,
-      sweep_kernel<float, 7, true>, sweep_kernel<float, 8, true>};
+template <typename T> struct Kernels;
+template <> struct Kernels<float> {
+  static KernelF get(bool pd, bool boost, int rows) { return kSweepF32[pd][boost][rows - 1]; }
 };
-template <> struct KernelTable<double> {
-  static constexpr KernelD plain[kMaxRowsPerLane] = {
-      sweep_kernel<double, 1, false>, sweep_kernel<double, 2, false>,
-      sweep_kernel<double, 3, false>, sweep_kernel<double, 4, false>,
-      sweep_kernel<double, 5, false>, sweep_kernel<double, 6, false>,
-      sweep_kernel<double, 7, false>, sweep_kernel<double, 8, false>};
-  static constexpr KernelD pd[kMaxRowsPerLane] = {
-      sweep_kernel<double, 1, true>, sweep_kernel<double, 2, true>, sweep_kernel<double, 3, true>,
-      sweep_kernel<double, 4, true>, sweep_kernel<double, 5, true>, sweep_kernel<double, 6, true>,
-      sweep_kernel<double, 7, true>, sweep_kernel<double, 8, true>};
+template <> struct Kernels<double> {
+  static KernelD get(bool pd, bool, int rows) { return kSweepF64[pd][rows - 1]; }
 };
 
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kBlocks[3] = {64, 128, 256};
+int block_index(int block) { return block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1; }
+
 constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
 
 struct DeviceState {
@@ -621,9 +61,12 @@ struct DeviceState {
   int num_cus = 0;
   DevTab<float> tf{};
   DevTab<double> td{};
-  // Resident blocks per CU for each kernel, at the LDS size it was queried with.
-  int occ[2][2][kMaxRowsPerLane] = {};
-  int occ_lds[2][2][kMaxRowsPerLane] = {};
+  // Resident blocks per CU of each kernel [fp64][pd][boost][rows-1][block], at the LDS size it
+  // was queried with.
+  int occ[2][2][2][kMaxRowsPerLane][3] = {};
+  int occ_lds[2][2][2][kMaxRowsPerLane][3] = {};
+  // Launch configurations the autotuner chose for FP32 buckets [pd][R-1] (rows 0: none yet).
+  LaunchCfg tuned[2][kMaxRowsPerLane];
 };
 
 std::mutex g_dev_mutex;
@@ -681,14 +124,17 @@ int device_state(DeviceState** out) {
 }
 
 template <typename T>
-int resident_blocks(DeviceState* st, bool pd, int R, int lds_bytes, int* blocks) {
+int resident_blocks(DeviceState* st, bool pd, bool boost, int rows, int block, int lds_bytes,
+                    int* blocks) {
   const int ti = std::is_same<T, double>::value ? 1 : 0;
-  int& occ = st->occ[ti][pd][R - 1];
-  int& occ_lds = st->occ_lds[ti][pd][R - 1];
+  const int bi = block_index(block);
+  std::lock_guard<std::mutex> lock(g_dev_mutex);
+  int& occ = st->occ[ti][pd][boost][rows - 1][bi];
+  int& occ_lds = st->occ_lds[ti][pd][boost][rows - 1][bi];
   if (occ == 0 || occ_lds != lds_bytes) {
     int nb = 0;
-    auto kern = pd ? KernelTable<T>::pd[R - 1] : KernelTable<T>::plain[R - 1];
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, size_t(lds_bytes)));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, Kernels<T>::get(pd, boost, rows),
+                                                         block, size_t(lds_bytes)));
     occ = std::max(nb, 1);
     occ_lds = lds_bytes;
   }
@@ -737,10 +183,17 @@ struct DeviceBatch::Impl {
   hipEvent_t bev[2 * kMaxRowsPerLane] = {};  // around each primary bucket launch
   hipGraphExec_t graph[8] = {};  // by mode * 2 + share
   int lds_words = 0;
+  LaunchCfg cfg[kMaxRowsPerLane];  // FP32 primary launch of each bucket
+
+  void drop_graphs() {
+    for (auto& g : graph) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
 
   ~Impl() {
-    for (auto& g : graph)
-      if (g) (void)hipGraphExecDestroy(g);
+    drop_graphs();
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : bev)
@@ -774,33 +227,81 @@ struct DeviceBatch::Impl {
     return a;
   }
 
+  // Waves a launch of bucket R's `max_items` items runs with: the resident limit, or fewer when
+  // the bucket is small.
   template <typename T>
-  int launch(int R, SweepArgs a, int64_t max_items) {
-    if (max_items <= 0) return kOk;
-    const int lds_bytes = lds_words * 4 * kWavesPerBlock;
+  int grid_waves(const LaunchCfg& c, int rows, int64_t max_items, int* grid) {
+    const int wpb = c.block / 64;
     int per_cu = 0;
-    if (int rc = resident_blocks<T>(st, pd, R, lds_bytes, &per_cu)) return rc;
-    const int64_t want = (max_items + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int grid = int(std::min<int64_t>(int64_t(per_cu) * st->num_cus, want));
-    auto kern = pd ? KernelTable<T>::pd[R - 1] : KernelTable<T>::plain[R - 1];
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds_bytes, stream, a, dev_tab<T>(st));
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, lds_words * 4 * wpb, &per_cu))
+      return rc;
+    const int64_t want = (max_items + wpb - 1) / wpb;
+    *grid = int(std::min<int64_t>(int64_t(per_cu) * st->num_cus, want));
+    return kOk;
+  }
+
+  // Launches bucket R's sweep with configuration `c` (FP64 launches ignore `boost`).  The
+  // scratch stride follows the kernel's rows per lane.
+  template <typename T>
+  int launch(int R, SweepArgs a, int64_t max_items, LaunchCfg c = LaunchCfg{}) {
+    if (max_items <= 0) return kOk;
+    const int rows = c.rows ? c.rows : R;
+    if (!std::is_same<T, float>::value) c.boost = false;
+    int grid = 0;
+    if (int rc = grid_waves<T>(c, rows, max_items, &grid)) return rc;
+    if (a.scratch) a.scratch_wave_stride = scratch_stride<T>(rows);
+    hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows), dim3(grid), dim3(c.block),
+                       lds_words * 4 * (c.block / 64), stream, a, dev_tab<T>(st));
     HIP_TRY(hipGetLastError());
     return kOk;
   }
 
+  // Scratch bytes a launch of bucket R with configuration `c` may touch.
   template <typename T>
-  int64_t scratch_need(int R) {
-    const int lds_bytes = lds_words * 4 * kWavesPerBlock;
-    int per_cu = 0;
-    if (resident_blocks<T>(st, pd, R, lds_bytes, &per_cu)) return -1;
-    const int64_t waves = int64_t(per_cu) * st->num_cus * kWavesPerBlock;
-    const int nv = ((pd ? 6 * R + 7 : 3 * R + 4) + 3) & ~3;  // snap_stride
-    return waves * int64_t(plan.max_slots) * nv * 64 * int64_t(sizeof(T));
+  int64_t scratch_need(int R, LaunchCfg c) {
+    const int rows = c.rows ? c.rows : R;
+    int grid = 0;
+    if (grid_waves<T>(c, rows, bucket_n[R - 1], &grid)) return -1;
+    return int64_t(grid) * (c.block / 64) * scratch_stride<T>(rows) * int64_t(sizeof(T));
   }
   template <typename T>
-  int64_t scratch_stride(int R) const {
-    const int nv = ((pd ? 6 * R + 7 : 3 * R + 4) + 3) & ~3;  // snap_stride
+  int64_t scratch_stride(int rows) const {
+    const int nv = ((pd ? 6 * rows + 7 : 3 * rows + 4) + 3) & ~3;  // snap_stride
     return int64_t(plan.max_slots) * nv * 64;
+  }
+
+  // Largest rows per lane bucket R may run at.  PD batches keep R: with prefix sharing a PD
+  // read's last bits can change with its lane layout (seen as 4e-8 in log10 on 2 of 7444
+  // pairs), while plain reads are bitwise the same at R and R + 1.
+  int max_rows(int R) const { return pd ? R : std::min(R + 1, kMaxRowsPerLane); }
+
+  // The FP32 configurations of bucket R that the tuner times (the default first), or, with
+  // `all`, every one set_launch accepts.
+  std::vector<LaunchCfg> candidates(int R, bool all = false) const {
+    std::vector<LaunchCfg> out;
+    for (int rows = R; rows <= max_rows(R); ++rows)
+      for (int boost = 0; boost < 2; ++boost)
+        for (int block : {256, 64, 128})
+          if (all || block != 128) out.push_back(LaunchCfg{rows, block, boost != 0});
+    return out;
+  }
+  bool valid(int R, const LaunchCfg& c) const {
+    const int rows = c.rows ? c.rows : R;
+    return rows >= R && rows <= max_rows(R) && block_index(c.block) >= 0;
+  }
+
+  SweepArgs primary_args(int R, int share, bool fp32) const {
+    SweepArgs a = base_args();
+    a.items = bucket_items[R - 1];
+    a.n_items = bucket_n[R - 1];
+    a.counter = counters + (R - 1);
+    a.share = share;
+    if (fp32) {
+      a.rescue = 1;
+      a.rescue_pairs = rescue_pairs[R - 1];
+      a.rescue_count = counters + 16 + (R - 1);
+    }
+    return a;
   }
 
   int enqueue(int mode, int share, bool timed);
@@ -815,20 +316,11 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
   // run is not captured into a graph.
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
     if (!bucket_n[R - 1]) continue;
-    SweepArgs a = base_args();
-    a.items = bucket_items[R - 1];
-    a.n_items = bucket_n[R - 1];
-    a.counter = counters + (R - 1);
-    a.share = share;
+    const SweepArgs a = primary_args(R, share, fp32);
     if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1)], stream));
     if (fp32) {
-      a.scratch_wave_stride = scratch_stride<float>(R);
-      a.rescue = 1;
-      a.rescue_pairs = rescue_pairs[R - 1];
-      a.rescue_count = counters + 16 + (R - 1);
-      if (int rc = launch<float>(R, a, bucket_n[R - 1])) return rc;
+      if (int rc = launch<float>(R, a, bucket_n[R - 1], cfg[R - 1])) return rc;
     } else {
-      a.scratch_wave_stride = scratch_stride<double>(R);
       if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
     }
     if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1) + 1], stream));
@@ -909,17 +401,24 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
     return kInvalidArgument;
   }
   d.lds_words = ((p.max_hap_len + 64 + 8) + 3) & ~3;
-  if (int64_t(d.lds_words) * 4 * kWavesPerBlock > 64 * 1024) {
+  if (int64_t(d.lds_words) * 4 * (256 / 64) > 64 * 1024) {  // four waves per 256-thread block
     set_error(kInvalidArgument, "haplotype of " + std::to_string(p.max_hap_len) +
                                     " bases exceeds the engine's LDS staging capacity");
     return kInvalidArgument;
   }
+  {
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) d.cfg[R - 1] = d.st->tuned[d.pd][R - 1];
+  }
+  // Scratch for the largest configuration any launch may use, so tuning never reallocates.
   int64_t need = 0;
   if (p.max_slots > 0) {
     for (int R = 1; R <= kMaxRowsPerLane; ++R) {
       if (!d.bucket_n[R - 1]) continue;
-      need = std::max(need, d.scratch_need<float>(R));
-      need = std::max(need, d.scratch_need<double>(R));
+      for (const LaunchCfg& c : d.candidates(R, true))
+        need = std::max(need, d.scratch_need<float>(R, c));
+      need = std::max(need, d.scratch_need<double>(R, LaunchCfg{}));
+      if (need < 0) break;
     }
   }
   if (need < 0) return kInternal;
@@ -1062,6 +561,73 @@ int DeviceBatch::run_per_region(RunStats* stats) {
   return kOk;
 }
 
+int DeviceBatch::tune(int share, TuneReport* report) {
+  Impl& d = *impl_;
+  const auto t0 = std::chrono::steady_clock::now();
+  TuneReport rep;
+  for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+    if (!d.bucket_n[R - 1]) continue;
+    const SweepArgs a = d.primary_args(R, share, true);
+    const std::vector<LaunchCfg> cands = d.candidates(R);
+    double best_ms = 0;
+    LaunchCfg best = cands[0];
+    for (size_t i = 0; i < cands.size(); ++i) {
+      double ms = 1e30;
+      for (int rep_i = 0; rep_i < 2; ++rep_i) {
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(d.ev[0], d.stream));
+        if (int rc = d.launch<float>(R, a, d.bucket_n[R - 1], cands[i])) return rc;
+        HIP_TRY(hipEventRecord(d.ev[1], d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, d.ev[0], d.ev[1]));
+        ms = std::min(ms, double(t));
+      }
+      if (i == 0) {
+        rep.default_ms[R - 1] = best_ms = ms;
+      } else if (ms < 0.99 * best_ms) {  // a candidate must win by 1% to replace the default
+        best_ms = ms;
+        best = cands[i];
+      }
+    }
+    rep.candidates += int(cands.size());
+    d.cfg[R - 1] = best;
+    rep.cfg[R - 1] = best;
+    rep.tuned_ms[R - 1] = best_ms;
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    d.st->tuned[d.pd][R - 1] = best;
+  }
+  d.drop_graphs();
+  rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (report) *report = rep;
+  return kOk;
+}
+
+int DeviceBatch::set_launch(int R, const LaunchCfg& cfg) {
+  Impl& d = *impl_;
+  if (R < 1 || R > kMaxRowsPerLane || !d.valid(R, cfg)) {
+    set_error(kInvalidArgument, "launch configuration outside the tuned space: bucket rows " +
+                                    std::to_string(R) + ", rows " + std::to_string(cfg.rows) +
+                                    ", block " + std::to_string(cfg.block));
+    return kInvalidArgument;
+  }
+  d.cfg[R - 1] = cfg;
+  d.cfg[R - 1].rows = cfg.rows ? cfg.rows : R;
+  d.drop_graphs();
+  return kOk;
+}
+
+int DeviceBatch::get_launch(int R, LaunchCfg* cfg) const {
+  if (R < 1 || R > kMaxRowsPerLane) {
+    set_error(kInvalidArgument, "bucket rows per lane must be 1..8");
+    return kInvalidArgument;
+  }
+  *cfg = impl_->cfg[R - 1];
+  if (!cfg->rows) cfg->rows = R;
+  return kOk;
+}
+
 int DeviceBatch::results(double* out, int64_t n) {
   Impl& d = *impl_;
   if (n != d.plan.n_pairs) {
@@ -1101,3 +667,4 @@ int device_available() {
 }
 
 }  // namespace fgkl
+

@@ -24,6 +24,23 @@ struct RunStats {
   int64_t bucket_cells[kMaxRowsPerLane] = {};  // DP cells (read x haplotype bases) it covers
 };
 
+// How the FP32 sweep of one rows-per-lane bucket is launched.  Every configuration computes
+// the same bits: a read's rows, whichever lanes hold them, see the same operands in the same
+// order, and identity rows reproduce DP row 0 exactly.
+struct LaunchCfg {
+  int rows = 0;        // kernel rows per lane, R or R + 1 for bucket R (0: R)
+  int block = 256;     // threads per workgroup: 64, 128 or 256
+  bool boost = false;  // the build compiled for one more wave per SIMD
+};
+
+struct TuneReport {
+  LaunchCfg cfg[kMaxRowsPerLane];           // chosen per bucket (rows 0: empty bucket)
+  double default_ms[kMaxRowsPerLane] = {};  // rows R, 256 threads, natural build
+  double tuned_ms[kMaxRowsPerLane] = {};
+  int candidates = 0;                       // configurations timed
+  double seconds = 0;                       // wall time of the tuning
+};
+
 class DeviceBatch {
  public:
   ~DeviceBatch();
@@ -36,6 +53,14 @@ class DeviceBatch {
   int results(double* out, int64_t n);
   int64_t n_pairs() const;
   int64_t scratch_bytes() const;
+  // Times every launch configuration of each FP32 bucket (rows R or R+1, 256 or 64 threads,
+  // natural or occupancy-raised build), keeps the fastest for this batch and for later batches
+  // of the same kind on this device, and drops captured graphs.
+  int tune(int share, TuneReport* report);
+  // Sets (or reads) bucket R's configuration explicitly; kInvalidArgument outside the tuned
+  // space.
+  int set_launch(int R, const LaunchCfg& cfg);
+  int get_launch(int R, LaunchCfg* cfg) const;
 
  private:
   DeviceBatch();

@@ -131,6 +131,32 @@ int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_grap
                    fgkl_run_stats* stats);
 int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoods_len);
 int64_t fgkl_batch_pairs(const fgkl_batch* batch);
+
+/*
+ * Launch autotuner (no reference counterpart: fgkl picks lanes per batch from the CPU's SIMD
+ * width, crates/pairhmm/src/lib.rs:408-413, simd.rs:317-328).  For every rows-per-lane bucket
+ * R of the FP32 pass it times rows per lane (R or R+1; R only for PD batches, whose last bits
+ * may depend on the lane layout under prefix sharing), threads per workgroup (256 or 64) and
+ * the natural or occupancy-raised kernel build on this batch, keeps the fastest (a candidate
+ * must beat the default by 1%) for this batch and for batches of the same kind (plain / PD)
+ * created later in the process on this device, and drops captured graphs.  Every
+ * configuration computes the same bits.  set_launch sets one bucket by hand (rows 0 = R;
+ * blocks 64, 128 or 256); out-of-range values are rejected with FGKL_INVALID_ARGUMENT.
+ */
+typedef struct fgkl_tune_report {
+  int32_t rows[8];       /* per bucket R=1..8: kernel rows per lane chosen (0: empty bucket) */
+  int32_t block[8];      /* threads per workgroup chosen */
+  int32_t boost[8];      /* 1 when the occupancy-raised build won */
+  double default_ms[8];  /* the bucket's launch at rows R, 256 threads, natural build */
+  double tuned_ms[8];    /* with the configuration chosen */
+  int32_t candidates;    /* configurations timed (each twice, best kept) */
+  double seconds;        /* wall time of the tuning */
+} fgkl_tune_report;
+
+int fgkl_batch_tune(fgkl_batch* batch, int share_prefixes, fgkl_tune_report* report);
+int fgkl_batch_set_launch(fgkl_batch* batch, int bucket_rows, int rows, int block, int boost);
+int fgkl_batch_get_launch(const fgkl_batch* batch, int bucket_rows, int* rows, int* block,
+                          int* boost);
 void fgkl_batch_free(fgkl_batch* batch);
 
 /*

@@ -565,6 +565,71 @@ def test_two_reads_sharing_a_wave_match_the_oracle(gpu, pd):
     b.close()
 
 
+def _launch_space(R, pd=False):
+    # PD buckets keep R rows per lane (their last bits may depend on the lane layout).
+    rows = [R] + ([R + 1] if R < 8 and not pd else [])
+    return [(r, b, o) for r in rows for b in (256, 128, 64) for o in (False, True)]
+
+
+@pytest.mark.parametrize("wl", [1, 3])
+def test_every_launch_configuration_gives_the_same_bits(gpu, wl):
+    data = workload(wl, seed=808 + wl, n_regions=5)
+    b = F.Batch(data)
+    for R in range(1, 9):
+        b.set_launch(R)  # the default, whatever an earlier tuning chose
+    b.run(F.MODE_FP32_RESCUE, share=True)
+    ref = b.results()
+    assert_close(ref, batch_reference(data), FLOAT_TOL, f"workload {wl}")
+    st = b.run(F.MODE_FP32_RESCUE, share=True)
+    used = [R for R in range(1, 9) if st.bucket_cells[R - 1] > 0]
+    assert len(used) >= 3
+    for R in used:
+        for rows, block, boost in _launch_space(R, pd=data.pd):
+            b.set_launch(R, rows, block, boost)
+            assert b.get_launch(R) == (rows, block, boost)
+            b.run(F.MODE_FP32_RESCUE, share=True)
+            got = b.results()
+            assert np.array_equal(got, ref), f"bucket {R}: rows {rows}, block {block}, boost {boost}"
+        b.set_launch(R)
+    b.close()
+
+
+def test_tuning_keeps_results_and_is_inherited_by_later_batches(gpu):
+    data = workload(1, seed=4242, n_regions=6)
+    b = F.Batch(data)
+    b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+    before = b.results()
+    rep = b.tune(share=True)
+    st = b.run(F.MODE_FP32_RESCUE, share=True)
+    assert rep.candidates > 0 and rep.seconds > 0
+    for R in range(1, 9):
+        if st.bucket_cells[R - 1] == 0:
+            continue
+        assert rep.rows[R - 1] in (R, min(R + 1, 8))
+        assert rep.block[R - 1] in (64, 256)
+        assert 0 < rep.tuned_ms[R - 1] <= rep.default_ms[R - 1]
+        assert b.get_launch(R) == (rep.rows[R - 1], rep.block[R - 1], bool(rep.boost[R - 1]))
+    assert np.array_equal(b.results(), before)
+    for _ in range(2):  # graphs captured after tuning use the tuned launches
+        b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+        assert np.array_equal(b.results(), before)
+        b.run(F.MODE_FP64, share=True)
+    later = F.Batch(workload(1, seed=4243, n_regions=6))
+    for R in range(1, 9):
+        if st.bucket_cells[R - 1]:
+            assert later.get_launch(R) == b.get_launch(R)
+    with pytest.raises(F.InvalidArgument):
+        b.set_launch(3, rows=5)
+    with pytest.raises(F.InvalidArgument):
+        b.set_launch(2, block=512)
+    pd = F.Batch(workload(3, seed=4244, n_regions=2))
+    with pytest.raises(F.InvalidArgument):
+        pd.set_launch(1, rows=2)
+    pd.close()
+    later.close()
+    b.close()
+
+
 def test_batch_graph_replays_match_eager_runs(gpu):
     data = workload(1, seed=99, n_regions=5)
     b = F.Batch(data)
