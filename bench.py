@@ -10,7 +10,8 @@ haplotype lengths), as the reference counts them (synthetic.rs:166-170).
 Alongside: the plain per-call FP64 GPU engine on the same inputs (one pair per wave, one
 synchronous launch set per region), parity of every pair against it, the roofline of the
 dominant sweep kernel, an end-to-end call from host arrays, the gradient engine, a rescue-stress
-and a PD-HMM workload, and the CPU oracle timed on a bounded sample (cpu_baseline).
+and a PD-HMM workload, and fgkl's lane-batched CPU backend (restated in C) timed on the host
+cores over a bounded sample (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -52,10 +53,8 @@ def measured_traffic(launch: str):
     return None if m is None else int(m.group(1)) + int(m.group(2))
 
 
-def cpu_baseline(data, max_regions: int, threads: int):
-    """The CPU oracle (test infrastructure, the checker) timed on the first regions."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from _oracle import batch_reference
+def first_regions(data, max_regions: int):
+    """The first regions of a BatchData, as a BatchData."""
     from fgkl_amd.synthetic import BatchData
 
     g = min(max_regions, data.n_regions)
@@ -65,10 +64,64 @@ def cpu_baseline(data, max_regions: int, threads: int):
                     data.read_offsets[: r1 + 1].copy(), data.bases[:rb], data.quals[:rb],
                     data.ins[:rb], data.dels[:rb], data.gcp[:rb], data.hap_offsets[: h1 + 1].copy(),
                     data.hap_bases[:hb], data.hap_flags[:hb], data.pd)
+    return sub
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(data, args, fp32_res) -> dict:
+    """fgkl's lane-batched CPU backend, restated (oracle/port.c), timed on this host's cores: one
+    region per worker thread, best of three runs over the first regions of the workload, sized
+    by a probe to about --cpu-seconds per run.  Also the scalar FP64 oracle on a few regions, as
+    the parity check of the GPU results."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _oracle import batch_reference, port_backend, port_batch
+
+    try:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+    except AttributeError:
+        threads = min(os.cpu_count() or 1, 16)
+    probe = first_regions(data, max(args.cpu_regions, threads))
     t0 = time.perf_counter()
-    ref = batch_reference(sub, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return sub, ref, dt
+    port_batch(probe, nthreads=threads)
+    per_region = (time.perf_counter() - t0) / probe.n_regions
+    want = int(min(data.n_regions, max(probe.n_regions, args.cpu_seconds / max(per_region, 1e-9))))
+    sub = first_regions(data, want)
+    times, got = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        got = port_batch(sub, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    n_sub = sub.n_pairs()
+    d_port = np.abs(fp32_res[:n_sub] - got)
+    small = first_regions(data, args.cpu_regions)
+    ref = batch_reference(small, nthreads=threads)
+    d_oracle = np.abs(fp32_res[: small.n_pairs()] - ref)
+    log(f"cpu baseline ({port_backend()}): {sub.n_regions} regions, best of 3 {dt:.2f}s on "
+        f"{threads} threads = {sub.cells() / dt / 1e9:.2f} GCUPS; gpu vs port max |d| "
+        f"{d_port.max():.3g}, gpu vs oracle max |d| {d_oracle.max():.3g}")
+    return {
+        "value": round(sub.cells() / dt / 1e9, 3), "unit": "GCUPS", "cores": threads,
+        "kind": "port",
+        "sample": f"first {sub.n_regions} regions of the same workload ({n_sub} pairs, "
+                  f"{sub.cells():.3e} cells), best of 3",
+        "implementation": f"fgkl's lane-batched FP32 backend restated in C, {port_backend()}, "
+                          "row sweep with prefix sharing, FTZ/DAZ, lost pairs recomputed in "
+                          "FP64; one region per thread",
+        "cpu_model": cpu_model(), "seconds": [round(t, 3) for t in times],
+        "max_abs_dlog10L_gpu_vs_port": float(d_port.max()),
+        "max_abs_dlog10L_gpu_vs_oracle": float(d_oracle.max()),
+        "oracle_sample_pairs": small.n_pairs(),
+    }
 
 
 def adjoint_bench(args) -> dict:
@@ -163,8 +216,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--regions", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--cpu-regions", type=int, default=16, help="regions in the CPU probe")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample time")
+    ap.add_argument("--cpu-regions", type=int, default=16,
+                    help="regions in the CPU probe and in the oracle parity sample")
+    ap.add_argument("--cpu-seconds", type=float, default=5.0,
+                    help="target time of one CPU baseline run (three are timed)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 per-call run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient workload")
@@ -327,24 +382,10 @@ def main() -> int:
             result[SIDE_WORKLOADS[config][0]] = side_bench(args, config)
 
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        # A probe sizes the sample to about --cpu-seconds of oracle work.
-        sub, ref, dt = cpu_baseline(data, args.cpu_regions, threads)
-        per_region = dt / max(sub.n_regions, 1)
-        want = int(min(data.n_regions, max(sub.n_regions, args.cpu_seconds / max(per_region, 1e-6))))
-        if want > sub.n_regions:
-            sub, ref, dt = cpu_baseline(data, want, threads)
-        sub_cells = sub.cells()
-        n_sub = sub.n_pairs()
-        dgpu = np.abs(fp32_res[:n_sub] - ref)
-        result["cpu_baseline"] = {
-            "value": round(sub_cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
-            "kind": "port",
-            "sample": f"first {sub.n_regions} regions of the same workload ({n_sub} "
-                      f"pairs, {sub_cells:.3e} cells), scalar FP64 oracle, one region per thread",
-            "seconds": round(dt, 2)}
-        result.setdefault("parity", {})["max_abs_dlog10L_vs_oracle_sample"] = float(dgpu.max())
-        log(f"cpu oracle: {dt:.1f}s on {threads} threads; gpu vs oracle max |d| {dgpu.max():.3g}")
+        cb = cpu_baseline(data, args, fp32_res)
+        result["cpu_baseline"] = cb
+        result.setdefault("parity", {})["max_abs_dlog10L_vs_oracle_sample"] = \
+            cb["max_abs_dlog10L_gpu_vs_oracle"]
 
     print(json.dumps(result), flush=True)
     batch.close()

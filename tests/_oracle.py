@@ -37,6 +37,10 @@ def oracle() -> C.CDLL:
         o.fgklo_initial_condition_log10.restype = C.c_double
         o.fgklo_log10_likelihood.restype = C.c_double
         o.fgklo_log10_likelihood.argtypes = [u8p, C.c_int64, u8p, u8p, u8p, u8p, u8p, C.c_int64]
+        o.fgklo_port_batch.restype = C.c_int
+        o.fgklo_port_batch.argtypes = [C.c_int64, i64p, i64p, u8p, u8p, u8p, u8p, u8p, i64p,
+                                       i64p, u8p, f64p, C.c_int, C.c_int]
+        o.fgklo_port_backend.restype = C.c_char_p
         o.fgklo_base_matches_pd.restype = C.c_int
         o.fgklo_base_matches_pd.argtypes = [C.c_uint8, C.c_uint8]
         o.fgklo_pd_log10_likelihood.restype = C.c_double
@@ -99,6 +103,27 @@ def batch_reference(data, nthreads: int = 8) -> np.ndarray:
         _p(data.region_haps, C.c_int64), _p(data.hap_offsets, C.c_int64), _p(data.hap_bases),
         _p(data.hap_flags) if data.pd else None, _p(out, C.c_double), nthreads)
     return out
+
+
+def port_batch(data, nthreads: int = 8, force_avx2: bool = False) -> np.ndarray:
+    """The lane-batched CPU baseline (oracle/port.c) over a plain BatchData, same layout as
+    batch_reference."""
+    assert not data.pd, "the CPU baseline covers the plain PairHMM"
+    out = np.full(data.n_pairs(), np.nan, dtype=np.float64)
+    if len(out) == 0:
+        return out
+    rc = oracle().fgklo_port_batch(
+        data.n_regions, _p(data.region_reads, C.c_int64), _p(data.read_offsets, C.c_int64),
+        _p(data.bases), _p(data.quals), _p(data.ins), _p(data.dels), _p(data.gcp),
+        _p(data.region_haps, C.c_int64), _p(data.hap_offsets, C.c_int64), _p(data.hap_bases),
+        _p(out, C.c_double), nthreads, int(force_avx2))
+    if rc != 0:
+        raise RuntimeError("the CPU baseline needs AVX-512 or AVX2")
+    return out
+
+
+def port_backend() -> str:
+    return oracle().fgklo_port_backend().decode()
 
 
 def transitions(ins, dele, gcp) -> list[float]:
