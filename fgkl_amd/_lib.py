@@ -83,6 +83,11 @@ SIGNATURES = {
     "fgkl_synth_copy": (None, [C.c_void_p, i64p, i64p, i64p, u8p, u8p, u8p, u8p, u8p, i64p,
                                u8p, u8p]),
     "fgkl_synth_free": (None, [C.c_void_p]),
+    "fgkl_replay_load": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.POINTER(C.c_void_p)]),
+    "fgkl_synth_expected": (C.c_int64, [C.c_void_p, f64p, C.c_int64]),
+    "fgkl_replay_stats": (None, [C.c_void_p, i64p, i64p, i64p]),
+    "fgkl_replay_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, i64p, i64p, i64p, u8p, u8p,
+                                    u8p, u8p, u8p, i64p, u8p, u8p, f64p, C.c_int64]),
     "fgkl_synth_splitmix": (None, [C.c_uint64, C.c_int64, u64p]),
     "fgkl_selftest_dpp": (C.c_int, [i32p]),
     "fgkl_selftest_dpp_shl": (C.c_int, [i32p]),

@@ -1,5 +1,6 @@
 // The C-ABI of include/fgkl.h: argument validation with the reference's error classes and
 // messages, and the calls into the device engine.
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <memory>
@@ -10,6 +11,7 @@
 #include "fgkl_engine.hpp"
 #include "fgkl_grad.hpp"
 #include "fgkl_internal.hpp"
+#include "fgkl_replay.hpp"
 #include "fgkl_synthetic.hpp"
 
 using namespace fgkl;
@@ -27,6 +29,8 @@ struct fgkl_batch {
 
 struct fgkl_synth {
   Batch b;
+  std::vector<double> expected;  // recorded likelihoods of a replayed dump
+  int64_t stats[3] = {};         // dump lines, skipped lines, dropped regions
 };
 
 struct fgkl_grad {
@@ -341,6 +345,75 @@ void fgkl_synth_copy(const fgkl_synth* s, int64_t* region_reads, int64_t* region
 }
 
 void fgkl_synth_free(fgkl_synth* s) { delete s; }
+
+int fgkl_replay_load(const char* path, int pd, int64_t max_regions, fgkl_synth** out) {
+  if (!path || !out) return fail(kInvalidArgument, "null argument");
+  *out = nullptr;
+  try {
+    ReplayData d;
+    std::string err;
+    if (replay_read(path, pd != 0, max_regions, d, err)) return fail(kInvalidArgument, err);
+    auto* s = new fgkl_synth;
+    s->b = std::move(d.batch);
+    s->expected = std::move(d.expected);
+    s->stats[0] = d.lines;
+    s->stats[1] = d.skipped_lines;
+    s->stats[2] = d.dropped_regions;
+    *out = s;
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int64_t fgkl_synth_expected(const fgkl_synth* s, double* out, int64_t n) {
+  if (!s) return 0;
+  const int64_t have = int64_t(s->expected.size());
+  if (out) std::memcpy(out, s->expected.data(), size_t(std::min(n, have)) * sizeof(double));
+  return have;
+}
+
+void fgkl_replay_stats(const fgkl_synth* s, int64_t* lines, int64_t* skipped_lines,
+                       int64_t* dropped_regions) {
+  *lines = s ? s->stats[0] : 0;
+  *skipped_lines = s ? s->stats[1] : 0;
+  *dropped_regions = s ? s->stats[2] : 0;
+}
+
+int fgkl_replay_write(const char* path, int pd, int64_t n_regions, const int64_t* region_reads,
+                      const int64_t* region_haps, const int64_t* read_offsets,
+                      const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
+                      const uint8_t* del_gop, const uint8_t* gcp, const int64_t* hap_offsets,
+                      const uint8_t* hap_bases, const uint8_t* hap_flags, const double* expected,
+                      int64_t n_expected) {
+  if (!path || n_regions < 0 || (n_regions && (!region_reads || !region_haps)))
+    return fail(kInvalidArgument, "null argument");
+  try {
+    Batch b;
+    b.region_reads.assign(region_reads, region_reads + n_regions + 1);
+    b.region_haps.assign(region_haps, region_haps + n_regions + 1);
+    const int64_t nr = region_reads[n_regions], nh = region_haps[n_regions];
+    b.read_offsets.assign(read_offsets, read_offsets + nr + 1);
+    b.hap_offsets.assign(hap_offsets, hap_offsets + nh + 1);
+    const size_t rb = size_t(read_offsets[nr]), hb = size_t(hap_offsets[nh]);
+    b.bases.assign(bases, bases + rb);
+    b.quals.assign(quals, quals + rb);
+    b.ins.assign(ins_gop, ins_gop + rb);
+    b.del.assign(del_gop, del_gop + rb);
+    b.gcp.assign(gcp, gcp + rb);
+    b.hap_bases.assign(hap_bases, hap_bases + hb);
+    if (hap_flags)
+      b.hap_flags.assign(hap_flags, hap_flags + hb);
+    else
+      b.hap_flags.assign(hb, 0);
+    std::vector<double> e(expected, expected + (expected ? n_expected : 0));
+    std::string err;
+    if (replay_write(b, e, path, pd != 0, err)) return fail(kInvalidArgument, err);
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
 
 void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out) {
   Rng rng(seed);

@@ -176,6 +176,28 @@ void fgkl_synth_copy(const fgkl_synth* s, int64_t* region_reads, int64_t* region
                      uint8_t* del_gop, uint8_t* gcp, int64_t* hap_offsets, uint8_t* hap_bases,
                      uint8_t* hap_flags);
 void fgkl_synth_free(fgkl_synth* s);
+
+/*
+ * GATK results dumps (no C-ABI counterpart: fgkl reads them in its replay tools,
+ * crates/pairhmm/src/bin/replay.rs:83-159 and pdreplay.rs:34-115).  fgkl_replay_load reads a
+ * --pair-hmm-results-file (pd = 0: space-separated hap, read, Phred+33 quals, ins, del, gcp,
+ * log10L) or --pdhmm-results-file (pd = 1: tab-separated, flags as Java's Arrays.toString) dump
+ * into a batch handle, regions reconstructed from the repeating haplotype lists, at most
+ * max_regions of them (all when <= 0); read it back with fgkl_synth_sizes / fgkl_synth_copy and
+ * the recorded likelihoods with fgkl_synth_expected (returns their count).  fgkl_replay_stats
+ * reports data lines, skipped malformed lines and dropped incomplete regions.
+ * fgkl_replay_write writes a batch and per-pair values in the same format.
+ */
+int fgkl_replay_load(const char* path, int pd, int64_t max_regions, fgkl_synth** out);
+int64_t fgkl_synth_expected(const fgkl_synth* s, double* out, int64_t n);
+void fgkl_replay_stats(const fgkl_synth* s, int64_t* lines, int64_t* skipped_lines,
+                       int64_t* dropped_regions);
+int fgkl_replay_write(const char* path, int pd, int64_t n_regions, const int64_t* region_reads,
+                      const int64_t* region_haps, const int64_t* read_offsets,
+                      const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
+                      const uint8_t* del_gop, const uint8_t* gcp, const int64_t* hap_offsets,
+                      const uint8_t* hap_bases, const uint8_t* hap_flags, const double* expected,
+                      int64_t n_expected);
 /* SplitMix64 (synthetic.rs:19-40): n successive next_u64 draws from seed. */
 void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out);
 

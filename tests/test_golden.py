@@ -14,7 +14,7 @@ from _oracle import batch_reference
 from fgkl_amd.synthetic import PdRegion, Region, workload
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-NAMES = sorted(p.stem for p in GOLDEN.glob("*.json"))
+NAMES = sorted(p.stem for p in GOLDEN.glob("*.json") if not p.name.endswith(".layout.json"))
 
 
 def load(name):
