@@ -87,15 +87,16 @@ double fgklo_log10_likelihood_cont(const uint8_t* hap, int64_t m,
 
 /*
  * The CPU baseline (port.c): fgkl's lane-batched FP32 backend restated with AVX-512 (16 lanes)
- * or AVX2 (8 lanes) vectors, one region per worker thread, FTZ/DAZ, lost pairs recomputed in
- * FP64.  Same inputs and output layout as fgklo_batch (no PD).  Returns 0, or -1 when the CPU
+ * or AVX2 (8 lanes) vectors, one region per worker thread, FTZ/DAZ, prefix sharing and (with
+ * share_suffixes, fgkl's default) suffix sharing, lost pairs recomputed in FP64.  Same inputs
+ * and output layout as fgklo_batch (no PD).  Returns 0, or -1 when the CPU
  * has neither AVX-512 nor AVX2.  force_avx2 selects the 8-lane build.
  */
 int fgklo_port_batch(int64_t n_regions, const int64_t* read_region, const int64_t* read_off,
                      const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
                      const uint8_t* del_gop, const uint8_t* gcp, const int64_t* hap_region,
                      const int64_t* hap_off, const uint8_t* hap_bases, double* out, int nthreads,
-                     int force_avx2);
+                     int force_avx2, int share_suffixes);
 /* The vector width fgklo_port_batch runs with on this CPU. */
 const char* fgklo_port_backend(void);
 

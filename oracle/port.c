@@ -33,11 +33,20 @@ struct port_job {
     const uint8_t* hap_bases;
     double* out;
     const int64_t* out_base;
+    int share_suffixes;
     atomic_long next;
 };
 
-/* One region's haplotypes in sorted order with their LCPs, duplicates and byte codes, and its
- * reads by length, longest first (kernel.rs:53-107, lib.rs:377-384). */
+/* One backward sweep of the suffix plan (plan.rs:14-31): along sorted haplotype `hap`, depths
+ * resume+1..need (depth d is column len - d), starting from node `start` (depth resume) and
+ * storing node node_of[d] (>= 0) at each depth a cut or a later sweep reads. */
+struct port_sweep {
+    int hap, resume, need, start;
+    int* node_of; /* [need + 1] */
+};
+
+/* One region's haplotypes in sorted order with their LCPs, duplicates and byte codes, its reads
+ * by length, longest first (kernel.rs:53-107, lib.rs:377-384), and its suffix plan. */
 struct port_plan {
     int64_t H, nr, r0;
     const uint8_t** hap;
@@ -46,15 +55,19 @@ struct port_plan {
     int64_t* read_order;
     int max_len, ncodes;
     uint8_t code_of[256], code_byte[256];
+    /* suffix sharing: per sorted haplotype the cut column (0: none) and node it joins */
+    int32_t *cut_col, *cut_node;
+    struct port_sweep* sweeps;
+    int nsweeps, nodes;
 };
 
 /* Per-thread buffers, grown on demand. */
 struct port_ws {
-    float *trans, *prior, *endm, *row_a, *row_b, *snap, *snap_acc;
+    float *trans, *prior, *endm, *row_a, *row_b, *snap, *snap_acc, *nodes;
     uint8_t* end_row;
     int32_t *cnt, *slot_of, *snap_cols, *snap_len;
     double* res;
-    size_t cap[12];
+    size_t cap[13];
 };
 
 static void* grow(void* p, size_t* cap, size_t want) {
@@ -84,10 +97,16 @@ static void port_ws_reserve(struct port_ws* w, int rows, int max_len, int ncodes
     w->res = grow(w->res, &w->cap[11], sizeof(double) * (size_t)L * (size_t)H);
 }
 
+/* Backward nodes: per node B_M and B_Y of rows 0..rows, then the row-0 start sum S. */
+static void port_ws_reserve_nodes(struct port_ws* w, int rows, int nodes, int L) {
+    w->nodes = grow(w->nodes, &w->cap[12],
+                    sizeof(float) * (size_t)nodes * (size_t)(2 * (rows + 1) + 1) * L);
+}
+
 static void port_ws_free(struct port_ws* w) {
     free(w->trans); free(w->prior); free(w->endm); free(w->row_a); free(w->row_b);
     free(w->snap); free(w->snap_acc); free(w->end_row); free(w->cnt); free(w->slot_of);
-    free(w->snap_cols); free(w->res);
+    free(w->snap_cols); free(w->res); free(w->nodes);
 }
 
 #define PORT_LANES 16
@@ -183,10 +202,128 @@ static void plan_region(const struct port_job* J, int64_t g, struct port_plan* P
     for (int64_t k = 1; k < H; ++k)
         if (!P->dup[k]) P->lcp_count[P->lcp[k]] += 1;
     free(hidx);
+    P->sweeps = NULL;
+    P->nsweeps = 0;
+    P->nodes = 1;
+    P->cut_col = NULL;
 }
 
 static void plan_free(struct port_plan* P) {
     free(P->hap); free(P->hap_len); free(P->dup); free(P->read_order); free(P->lcp_count);
+    free(P->cut_col);
+    for (int s = 0; s < P->nsweeps; ++s) free(P->sweeps[s].node_of);
+    free(P->sweeps);
+}
+
+static int common_suffix(const uint8_t* a, int na, const uint8_t* b, int nb) {
+    int n = 0;
+    while (n < na && n < nb && a[na - 1 - n] == b[nb - 1 - n]) ++n;
+    return n;
+}
+
+static const struct port_plan* g_rev_plan;
+
+static int rev_cmp(const void* x, const void* y) {  /* reversed bases, then sorted index */
+    const int a = *(const int*)x, b = *(const int*)y;
+    const struct port_plan* P = g_rev_plan;
+    const uint8_t *ha = P->hap[a], *hb = P->hap[b];
+    const int na = P->hap_len[a], nb = P->hap_len[b];
+    for (int i = 0; i < na && i < nb; ++i) {
+        const uint8_t ca = ha[na - 1 - i], cb = hb[nb - 1 - i];
+        if (ca != cb) return ca < cb ? -1 : 1;
+    }
+    if (na != nb) return na < nb ? -1 : 1;
+    return a < b ? -1 : (a > b);
+}
+
+/* The node holding depth `depth` of the suffix at position r of the reversed order: the one the
+ * latest sweep at or before r wrote for a suffix r shares that deep (plan.rs:205-230). */
+static int node_for(struct port_plan* P, const int* sweep_at, const int* lcs, int r, int depth) {
+    if (depth == 0) return 0;
+    int shared = 1 << 30;
+    for (int p = r; p >= 0; --p) {
+        const int s = sweep_at[p];
+        if (s >= 0 && P->sweeps[s].resume < depth && depth <= P->sweeps[s].need) {
+            int* id = &P->sweeps[s].node_of[depth];
+            if (*id < 0) *id = P->nodes++;
+            return *id;
+        }
+        if (lcs[p] < shared) shared = lcs[p];
+        if (p == 0 || shared < depth) break;
+    }
+    abort();  /* the plan never reads a depth no sweep computes */
+}
+
+/* Suffix sharing (plan.rs:55-104, 146-203): cut each distinct haplotype at depth
+ * min(longest shared suffix, len - longest shared prefix - 1) when that saves at least 4
+ * columns, and plan the backward sweeps in reversed-bases order, each resuming from the deepest
+ * depth already computed along its suffix. */
+static void plan_suffixes(struct port_plan* P) {
+    const int H = (int)P->H;
+    P->cut_col = calloc((size_t)(2 * H + 1), sizeof(int32_t));
+    P->cut_node = P->cut_col + H;
+    P->sweeps = calloc((size_t)(H + 1), sizeof(struct port_sweep));
+    P->nsweeps = 0;
+    P->nodes = 1;
+    int* distinct = malloc(sizeof(int) * (size_t)(H + 1) * 6);
+    int* order = distinct + H + 1;
+    int* lcs = order + H + 1;
+    int* pos = lcs + H + 1;
+    int* depth = pos + H + 1;
+    int* sweep_at = depth + H + 1;
+    int nd = 0;
+    for (int k = 0; k < H; ++k)
+        if (!P->dup[k]) distinct[nd++] = k;
+    memcpy(order, distinct, sizeof(int) * (size_t)nd);
+    pthread_mutex_lock(&g_sort_mutex);
+    g_rev_plan = P;
+    qsort(order, (size_t)nd, sizeof(int), rev_cmp);
+    pthread_mutex_unlock(&g_sort_mutex);
+    for (int r = 0; r < nd; ++r) {
+        lcs[r] = r ? common_suffix(P->hap[order[r - 1]], P->hap_len[order[r - 1]],
+                                   P->hap[order[r]], P->hap_len[order[r]]) : 0;
+        pos[order[r]] = r;
+        sweep_at[r] = -1;
+    }
+    for (int t = 0; t < nd; ++t) {
+        const int k = distinct[t];
+        depth[k] = 0;
+        const int next_lcp = t + 1 < nd ? P->lcp[distinct[t + 1]] : 0;
+        const int shared_prefix = P->lcp[k] > next_lcp ? P->lcp[k] : next_lcp;
+        const int len = P->hap_len[k];
+        if (shared_prefix < len) {
+            const int r = pos[k];
+            int s = lcs[r];
+            if (r + 1 < nd && lcs[r + 1] > s) s = lcs[r + 1];
+            const int d = s < len - shared_prefix - 1 ? s : len - shared_prefix - 1;
+            if (d >= 4) depth[k] = d;
+        }
+    }
+    int reach = 0;
+    for (int r = 0; r < nd; ++r) {
+        const int k = order[r];
+        const int resume = lcs[r] < reach ? lcs[r] : reach;
+        if (depth[k] > resume) {
+            struct port_sweep* sw = &P->sweeps[P->nsweeps];
+            sw->hap = k;
+            sw->resume = resume;
+            sw->need = depth[k];
+            sw->node_of = malloc(sizeof(int) * (size_t)(depth[k] + 1));
+            for (int d = 0; d <= depth[k]; ++d) sw->node_of[d] = -1;
+            sweep_at[r] = P->nsweeps++;
+        }
+        reach = resume > depth[k] ? resume : depth[k];
+    }
+    for (int s = 0; s < P->nsweeps; ++s)
+        P->sweeps[s].start = node_for(P, sweep_at, lcs, pos[P->sweeps[s].hap], P->sweeps[s].resume);
+    for (int t = 0; t < nd; ++t) {
+        const int k = distinct[t];
+        if (depth[k] > 0) {
+            P->cut_col[k] = P->hap_len[k] - depth[k];
+            P->cut_node[k] = node_for(P, sweep_at, lcs, pos[k], depth[k]);
+        }
+    }
+    free(distinct);
 }
 
 static int g_use_avx512 = -1;
@@ -201,6 +338,7 @@ static void* port_worker(void* arg) {
         if (g >= J->n_regions) break;
         struct port_plan P;
         plan_region(J, g, &P);
+        if (J->share_suffixes) plan_suffixes(&P);
         if (P.H > 0 && P.nr > 0) {
             if (g_use_avx512)
                 port_region_avx512(J, &P, &ws, J->out + J->out_base[g]);
@@ -225,7 +363,7 @@ int fgklo_port_batch(int64_t n_regions, const int64_t* read_region, const int64_
                      const uint8_t* bases, const uint8_t* quals, const uint8_t* ins_gop,
                      const uint8_t* del_gop, const uint8_t* gcp, const int64_t* hap_region,
                      const int64_t* hap_off, const uint8_t* hap_bases, double* out, int nthreads,
-                     int force_avx2) {
+                     int force_avx2, int share_suffixes) {
     fgklo_port_backend();
     if (force_avx2) g_use_avx512 = 0;
     if (!g_use_avx512 && !__builtin_cpu_supports("avx2")) return -1;
@@ -235,7 +373,7 @@ int fgklo_port_batch(int64_t n_regions, const int64_t* read_region, const int64_
         base[g + 1] = base[g] + (read_region[g + 1] - read_region[g]) *
                                     (hap_region[g + 1] - hap_region[g]);
     struct port_job J = {n_regions, read_region, read_off, bases, quals, ins_gop, del_gop, gcp,
-                         hap_region, hap_off, hap_bases, out, base, 0};
+                         hap_region, hap_off, hap_bases, out, base, share_suffixes != 0, 0};
     atomic_init(&J.next, 0);
     if (nthreads < 1) nthreads = 1;
     pthread_t* th = malloc(sizeof(pthread_t) * (size_t)nthreads);

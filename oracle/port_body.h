@@ -10,6 +10,10 @@
  * columns after the shared prefix, starting from the column snapshot an earlier haplotype left,
  * rescaled by the length ratio (kernel.rs:220-278, 610-759); the cell recurrence in the FMA
  * order of kernel.rs:1011-1013; results through finish_lane's FP32 rule (kernel.rs:897-914).
+ * With a suffix plan, backward sweeps first compute, once per shared suffix, the scaled mass
+ * from each cell to the end of the read (kernel.rs:579-602, 761-891, 1029-1126), and each cut
+ * haplotype stops its forward sweep one column before its cut and joins there in FP64
+ * (kernel.rs:744-791).
  */
 
 PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan* P,
@@ -60,6 +64,104 @@ PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan
         const size_t col_stride = (size_t)(rows + 1) * L;  /* one state of one snapshot */
         double* res = ws->res;  /* [lane][sorted hap] */
 
+        /* Backward nodes: node n holds, at its column, B_M and B_Y of rows 0..rows and the sum S
+         * of the weights of paths starting in row 0 at or after that column, all scaled by 2^120
+         * (the forward initial value's scale, simd.rs:109). */
+        const size_t nstride = (size_t)(2 * (rows + 1) + 1) * L;
+        const V bscale = (V){} + 0x1p120f;
+        if (P->nsweeps > 0 || P->nodes > 1 || (P->cut_col && H > 0)) {
+            port_ws_reserve_nodes(ws, rows, P->nodes, L);
+            /* Node 0, the last column: only trailing insertions lead on (kernel.rs:579-602). */
+            float* n0 = ws->nodes;
+            memset(n0, 0, sizeof(float) * nstride);
+            V x_below = (V){};
+            for (int i = rows; i >= 1; --i) {
+                const V sink = bscale * *(const V*)(endm + (size_t)(i - 1) * L);
+                V mi = (V){}, ii = (V){};
+                if (i < rows) {
+                    mi = *(const V*)(trans + ((size_t)i * 6 + 2) * L);
+                    ii = *(const V*)(trans + ((size_t)i * 6 + 3) * L);
+                }
+                *(V*)(n0 + (size_t)i * L) = x_below * mi + sink;
+                x_below = x_below * ii + sink;
+            }
+            /* The sweeps, rows bottom-up, each row right to left over depths resume+1..need. */
+            float* bprev = ws->row_a;  /* [column][B_M, B_X][L] of row i + 1 */
+            float* bcur = ws->row_b;
+            for (int w = 0; w < P->nsweeps; ++w) {
+                const struct port_sweep* sw = &P->sweeps[w];
+                const uint8_t* hap = P->hap[sw->hap];
+                const int len = P->hap_len[sw->hap];
+                const int right = len - sw->resume, left = len - sw->need;
+                const float* st = ws->nodes + (size_t)sw->start * nstride;
+                for (int j = left; j < right; ++j) {
+                    *(V*)(bprev + (size_t)j * 2 * L) = (V){};
+                    *(V*)(bprev + (size_t)j * 2 * L + L) = (V){};
+                }
+                V starts = *(const V*)(st + (size_t)2 * (rows + 1) * L);
+                for (int i = rows; i >= 0; --i) {
+                    V mm = (V){}, im = (V){}, mi = (V){}, ii = (V){}, md = (V){}, dd = (V){};
+                    if (i < rows) {
+                        const float* t = trans + (size_t)i * 6 * L;
+                        mm = *(const V*)(t); im = *(const V*)(t + L);
+                        mi = *(const V*)(t + 2 * L); ii = *(const V*)(t + 3 * L);
+                    }
+                    if (i >= 1) {
+                        const float* t = trans + (size_t)(i - 1) * 6 * L;
+                        md = *(const V*)(t + 4 * L); dd = *(const V*)(t + 5 * L);
+                    }
+                    const float* prow = prior + (size_t)(i < rows ? i : rows - 1) * P->ncodes * L;
+                    const int has_sink = i >= 1 && ws->end_row[i - 1];
+                    V sink = (V){};
+                    if (has_sink) sink = bscale * *(const V*)(endm + (size_t)(i - 1) * L);
+                    V bm_diag = i < rows ? *(const V*)(st + (size_t)(i + 1) * L) : (V){};
+                    V by_right = *(const V*)(st + (size_t)(rows + 1 + i) * L);
+                    V bm_last = (V){};
+                    int hi = right;
+                    for (int d = sw->resume + 1;; ++d) {
+                        const int write = d <= sw->need && sw->node_of[d] >= 0;
+                        if (d <= sw->need && !write) continue;
+                        const int lo = d <= sw->need ? len - d : left;
+                        for (int j = hi - 1; j >= lo; --j) {
+                            const V pri = *(const V*)(prow + (size_t)P->code_of[hap[j]] * L);
+                            if (i == 0) {  /* row 0: paths start here and enter M of row 1 */
+                                const V by = pri * bm_diag * im;
+                                starts += by;
+                                bm_diag = *(const V*)(bprev + (size_t)j * 2 * L);
+                                by_right = by;
+                            } else {
+                                const V bm_below = *(const V*)(bprev + (size_t)j * 2 * L);
+                                const V bx_below = *(const V*)(bprev + (size_t)j * 2 * L + L);
+                                const V into = pri * bm_diag;
+                                const V ind = into * im;
+                                V bx = bx_below * ii + ind;
+                                const V by = by_right * dd + ind;
+                                V bm = by_right * md + (bx_below * mi + into * mm);
+                                bm += sink;
+                                bx += sink;
+                                *(V*)(bcur + (size_t)j * 2 * L) = bm;
+                                *(V*)(bcur + (size_t)j * 2 * L + L) = bx;
+                                bm_diag = bm_below;
+                                by_right = by;
+                                bm_last = bm;
+                            }
+                        }
+                        hi = lo;
+                        if (!write) break;
+                        float* nd = ws->nodes + (size_t)sw->node_of[d] * nstride;
+                        *(V*)(nd + (size_t)(rows + 1 + i) * L) = by_right;
+                        if (i == 0)
+                            *(V*)(nd + (size_t)2 * (rows + 1) * L) = starts;
+                        else
+                            *(V*)(nd + (size_t)i * L) = bm_last;
+                    }
+                    if (i > 0) {
+                        float* tmp = bprev; bprev = bcur; bcur = tmp;
+                    }
+                }
+            }
+        }
+
         for (int64_t k = 0; k < H; ++k) {
             if (P->dup[k]) {
                 for (int l = 0; l < nl; ++l) res[l * H + k] = res[l * H + k - 1];
@@ -69,8 +171,13 @@ PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan
             if (k) cnt[start] -= 1;
             const uint8_t* hap = P->hap[k];
             const int len = P->hap_len[k];
+            const int cut = P->cut_col ? P->cut_col[k] : 0;  /* 0: sweep to the end */
+            const int stop = cut ? cut - 1 : len;
+            const float* node = cut ? ws->nodes + (size_t)P->cut_node[k] * nstride : NULL;
+            double cross[L];
+            for (int l = 0; l < L; ++l) cross[l] = 0.0;
             int nq = 0;
-            for (int q = start + 1; q <= len; ++q) {
+            for (int q = start + 1; q <= stop; ++q) {
                 if (cnt[q] > 0) {
                     if (ws->slot_of[q] < 0) ws->slot_of[q] = nslots++;
                     ws->snap_cols[nq++] = q;
@@ -95,7 +202,7 @@ PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan
             }
             float* prev = ws->row_a;
             float* cur = ws->row_b;
-            for (int j = start; j <= len; ++j) {  /* DP row 0: (0, 0, C/m) */
+            for (int j = start; j <= stop; ++j) {  /* DP row 0: (0, 0, C/m) */
                 *(V*)(prev + (size_t)j * 3 * L) = zero;
                 *(V*)(prev + (size_t)j * 3 * L + L) = zero;
                 *(V*)(prev + (size_t)j * 3 * L + 2 * L) = init;
@@ -131,7 +238,7 @@ PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan
                 const V e = *(const V*)(endm + (size_t)r * L);
                 V rowsum = zero;
                 int a = 0;
-                for (int j = start + 1; j <= len; ++j) {
+                for (int j = start + 1; j <= stop; ++j) {
                     const float* up = prev + (size_t)j * 3 * L;
                     const V um = *(const V*)(up), ux = *(const V*)(up + L);
                     const V uy = *(const V*)(up + 2 * L);
@@ -155,6 +262,17 @@ PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan
                     lm = m; ly = y;
                 }
                 if (end) acc = e * rowsum + acc;
+                if (node) {
+                    /* The values entering the cut column (kernel.rs:744-755): match from the
+                     * diagonal, deletion from the left; insertions stay within a column. */
+                    const V pri = *(const V*)(prow + (size_t)P->code_of[hap[cut - 1]] * L);
+                    const V jm = pri * (dy * im + (dx * im + dm * mm));
+                    const V jy = ly * dd + lm * md;
+                    const V nm = *(const V*)(node + (size_t)i * L);
+                    const V ny = *(const V*)(node + (size_t)(rows + 1 + i) * L);
+                    for (int l = 0; l < L; ++l)
+                        cross[l] += (double)jm[l] * (double)nm[l] + (double)jy[l] * (double)ny[l];
+                }
                 for (int q = 0; q < nq; ++q) {
                     const int col = ws->snap_cols[q];
                     float* d = ws->snap + (size_t)ws->slot_of[col] * 3 * col_stride;
@@ -165,7 +283,14 @@ PORT_TARGET static void PORT_FN(const struct port_job* J, const struct port_plan
                 }
                 float* tmp = prev; prev = cur; cur = tmp;
             }
-            for (int l = 0; l < nl; ++l) res[l * H + k] = (double)acc[l];
+            if (node) {  /* the join (kernel.rs:767-791), in FP64 */
+                const double init_d = (double)(float)(0x1p120 / (double)len);
+                const float* ns = node + (size_t)2 * (rows + 1) * L;
+                for (int l = 0; l < nl; ++l)
+                    res[l * H + k] = (double)acc[l] + (cross[l] + init_d * (double)ns[l]) / 0x1p120;
+            } else {
+                for (int l = 0; l < nl; ++l) res[l * H + k] = (double)acc[l];
+            }
         }
         /* finish_lane (kernel.rs:897-914) and the FP64 recomputation of lost pairs. */
         for (int l = 0; l < nl; ++l) {

@@ -39,7 +39,7 @@ def oracle() -> C.CDLL:
         o.fgklo_log10_likelihood.argtypes = [u8p, C.c_int64, u8p, u8p, u8p, u8p, u8p, C.c_int64]
         o.fgklo_port_batch.restype = C.c_int
         o.fgklo_port_batch.argtypes = [C.c_int64, i64p, i64p, u8p, u8p, u8p, u8p, u8p, i64p,
-                                       i64p, u8p, f64p, C.c_int, C.c_int]
+                                       i64p, u8p, f64p, C.c_int, C.c_int, C.c_int]
         o.fgklo_port_backend.restype = C.c_char_p
         o.fgklo_base_matches_pd.restype = C.c_int
         o.fgklo_base_matches_pd.argtypes = [C.c_uint8, C.c_uint8]
@@ -105,9 +105,10 @@ def batch_reference(data, nthreads: int = 8) -> np.ndarray:
     return out
 
 
-def port_batch(data, nthreads: int = 8, force_avx2: bool = False) -> np.ndarray:
+def port_batch(data, nthreads: int = 8, force_avx2: bool = False,
+               share_suffixes: bool = True) -> np.ndarray:
     """The lane-batched CPU baseline (oracle/port.c) over a plain BatchData, same layout as
-    batch_reference."""
+    batch_reference; suffix sharing on by default, as in fgkl's JNI configuration."""
     assert not data.pd, "the CPU baseline covers the plain PairHMM"
     out = np.full(data.n_pairs(), np.nan, dtype=np.float64)
     if len(out) == 0:
@@ -116,7 +117,7 @@ def port_batch(data, nthreads: int = 8, force_avx2: bool = False) -> np.ndarray:
         data.n_regions, _p(data.region_reads, C.c_int64), _p(data.read_offsets, C.c_int64),
         _p(data.bases), _p(data.quals), _p(data.ins), _p(data.dels), _p(data.gcp),
         _p(data.region_haps, C.c_int64), _p(data.hap_offsets, C.c_int64), _p(data.hap_bases),
-        _p(out, C.c_double), nthreads, int(force_avx2))
+        _p(out, C.c_double), nthreads, int(force_avx2), int(share_suffixes))
     if rc != 0:
         raise RuntimeError("the CPU baseline needs AVX-512 or AVX2")
     return out

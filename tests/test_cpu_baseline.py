@@ -1,14 +1,15 @@
 """The CPU baseline (oracle/port.c, fgkl's lane-batched backend restated) against the scalar
 oracle.  The baseline is only timed by bench.py; these checks keep its numbers honest: every pair
-within the FP32 bound of the FP64 oracle, in both vector widths, with prefix sharing, duplicate
-haplotypes and reads of mixed lengths in one lane batch."""
+within the FP32 bound of the FP64 oracle, in both vector widths, with prefix sharing alone and with
+suffix sharing (backward sweeps and joins), duplicate haplotypes and reads of mixed lengths in one
+lane batch."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
 import fgkl_amd as F
-from fgkl_amd.synthetic import batch_from_regions, workload
+from fgkl_amd.synthetic import Rng, batch_from_regions, workload
 from _oracle import batch_reference, port_backend, port_batch
 
 FLOAT_TOL = 1e-4
@@ -16,12 +17,48 @@ FLOAT_TOL = 1e-4
 
 @pytest.mark.parametrize("wl,regions", [(0, 1), (1, 6), (2, 2), (4, 6)])
 @pytest.mark.parametrize("avx2", [False, True])
-def test_cpu_baseline_matches_the_oracle(wl, regions, avx2):
+@pytest.mark.parametrize("suffixes", [True, False])
+def test_cpu_baseline_matches_the_oracle(wl, regions, avx2, suffixes):
     data = workload(wl, seed=99 + wl, n_regions=regions)
-    got = port_batch(data, nthreads=4, force_avx2=avx2)
+    got = port_batch(data, nthreads=4, force_avx2=avx2, share_suffixes=suffixes)
     ref = batch_reference(data)
     assert np.all(np.isfinite(got))
     assert np.max(np.abs(got - ref)) <= FLOAT_TOL
+
+
+def test_suffix_sharing_joins_match_the_oracle_on_edited_haplotype_sets():
+    # Haplotype sets built like the reference's plan tests (plan.rs:300-340): random edits of a
+    # base sequence over small alphabets, with copies, so cuts, shared suffixes of every depth,
+    # resumed backward sweeps and duplicate haplotypes all occur.
+    rng = Rng(77)
+    for case in range(40):
+        alphabet = b"AC" if case % 2 == 0 else b"ACGT"
+        base = bytes(alphabet[rng.below(len(alphabet))] for _ in range(20 + rng.below(60)))
+        haps = [base]
+        for _ in range(rng.below(12)):
+            src = haps[rng.below(len(haps))]
+            if rng.chance(0.15):
+                haps.append(src)
+                continue
+            h = bytearray(src)
+            at = rng.below(len(h))
+            h[at:at + rng.below(4)] = bytes(alphabet[rng.below(len(alphabet))]
+                                             for _ in range(rng.below(4)))
+            haps.append(bytes(h) if h else src)
+        reads = []
+        for _ in range(1 + rng.below(20)):
+            src = haps[rng.below(len(haps))]
+            n = 1 + rng.below(len(src))
+            s0 = rng.below(len(src) - n + 1)
+            reads.append(F.ReadRef(src[s0:s0 + n], [10 + rng.below(31) for _ in range(n)],
+                                   [20 + rng.below(26) for _ in range(n)],
+                                   [20 + rng.below(26) for _ in range(n)],
+                                   [1 + rng.below(10) for _ in range(n)]))
+        data = batch_from_regions([(reads, haps)])
+        ref = batch_reference(data)
+        for suffixes in (True, False):
+            got = port_batch(data, nthreads=1, share_suffixes=suffixes)
+            assert np.max(np.abs(got - ref)) <= FLOAT_TOL, (case, suffixes)
 
 
 def test_cpu_baseline_handles_duplicates_prefixes_and_underflow():
