@@ -300,6 +300,13 @@ class Batch:
         _check(lib().fgkl_batch_run(self._h, mode, int(share), int(graph), C.byref(st)))
         return st
 
+    def work(self) -> dict:
+        """DP cells of the plan: nominal, prefix sharing alone, and forward / backward with
+        suffix sharing (fgkl_batch_work)."""
+        w = (C.c_int64 * 4)()
+        _check(lib().fgkl_batch_work(self._h, w))
+        return dict(zip(("nominal", "prefix", "forward", "backward"), (int(x) for x in w)))
+
     def tune(self, share: bool = True):
         """Times each FP32 bucket's launch configurations and keeps the fastest
         (fgkl_batch_tune); returns the report."""

@@ -71,6 +71,7 @@ SIGNATURES = {
     "fgkl_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "fgkl_batch_results": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "fgkl_batch_pairs": (C.c_int64, [C.c_void_p]),
+    "fgkl_batch_work": (C.c_int, [C.c_void_p, i64p]),
     "fgkl_batch_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TuneReport)]),
     "fgkl_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "fgkl_batch_get_launch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),

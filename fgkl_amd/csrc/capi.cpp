@@ -257,6 +257,12 @@ int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoo
 
 int64_t fgkl_batch_pairs(const fgkl_batch* batch) { return batch ? batch->batch->n_pairs() : 0; }
 
+int fgkl_batch_work(const fgkl_batch* batch, int64_t* cells4) {
+  if (!batch || !cells4) return fail(kInvalidArgument, "null argument");
+  batch->batch->work(cells4);
+  return kOk;
+}
+
 int fgkl_batch_tune(fgkl_batch* batch, int share_prefixes, fgkl_tune_report* report) {
   if (!batch) return fail(kInvalidArgument, "null batch");
   try {

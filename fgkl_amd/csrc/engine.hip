@@ -170,6 +170,9 @@ struct DeviceBatch::Impl {
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
   int64_t bucket_cells[kMaxRowsPerLane] = {};
   int32_t* rescue_pairs[kMaxRowsPerLane] = {};
+  BwdSweep* bsweeps = nullptr;
+  int32_t* bwrites = nullptr;
+  int32_t* region_bw = nullptr;
   int32_t* counters = nullptr;  // [0,8) main, [8,16) rescue, [16,24) rescue counts
   double* out = nullptr;
   void* scratch = nullptr;
@@ -224,6 +227,7 @@ struct DeviceBatch::Impl {
     a.region_out = region_out; a.region_hap0 = region_hap0; a.region_nhap = region_nhap;
     a.plan = hplan; a.snaps = snaps; a.hap_bytes = hap_bytes; a.hap_words = hap_words;
     a.scratch = scratch; a.out = out; a.lds_words = lds_words;
+    a.bsweeps = bsweeps; a.bwrites = bwrites; a.region_bw = region_bw;
     return a;
   }
 
@@ -249,7 +253,10 @@ struct DeviceBatch::Impl {
     if (!std::is_same<T, float>::value) c.boost = false;
     int grid = 0;
     if (int rc = grid_waves<T>(c, rows, max_items, &grid)) return rc;
-    if (a.scratch) a.scratch_wave_stride = scratch_stride<T>(rows);
+    if (a.scratch) {
+      a.scratch_wave_stride = scratch_stride<T>(rows);
+      a.node_off = snap_elems<T>(rows);
+    }
     hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows), dim3(grid), dim3(c.block),
                        lds_words * 4 * (c.block / 64), stream, a, dev_tab<T>(st));
     HIP_TRY(hipGetLastError());
@@ -264,10 +271,17 @@ struct DeviceBatch::Impl {
     if (grid_waves<T>(c, rows, bucket_n[R - 1], &grid)) return -1;
     return int64_t(grid) * (c.block / 64) * scratch_stride<T>(rows) * int64_t(sizeof(T));
   }
+  // Per wave: the snapshot slots, then (FP32 plain) the suffix-sharing node records.
   template <typename T>
-  int64_t scratch_stride(int rows) const {
+  int64_t snap_elems(int rows) const {
     const int nv = ((pd ? 6 * rows + 7 : 3 * rows + 4) + 3) & ~3;  // snap_stride
     return int64_t(plan.max_slots) * nv * 64;
+  }
+  template <typename T>
+  int64_t scratch_stride(int rows) const {
+    const bool nodes = std::is_same<T, float>::value && !pd;
+    const int ns = (2 * rows + 1 + 3) & ~3;  // node_stride
+    return snap_elems<T>(rows) + (nodes ? int64_t(plan.max_nodes) * ns * 64 : 0);
   }
 
   // Largest rows per lane bucket R may run at.  PD batches keep R: with prefix sharing a PD
@@ -296,6 +310,7 @@ struct DeviceBatch::Impl {
     a.n_items = bucket_n[R - 1];
     a.counter = counters + (R - 1);
     a.share = share;
+    a.suffix = fp32 && share && !pd && !plan.bsweeps.empty();
     if (fp32) {
       a.rescue = 1;
       a.rescue_pairs = rescue_pairs[R - 1];
@@ -371,7 +386,10 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
       (rc = d.upload(&d.region_hap0, p.region_hap0.data(), size_t(G))) ||
       (rc = d.upload(&d.region_nhap, p.region_nhap.data(), size_t(G))) ||
       (rc = d.upload(&d.hplan, p.plan.data(), size_t(NH))) ||
-      (rc = d.upload(&d.snaps, p.snaps.data(), p.snaps.size())))
+      (rc = d.upload(&d.snaps, p.snaps.data(), p.snaps.size())) ||
+      (rc = d.upload(&d.bsweeps, p.bsweeps.data(), p.bsweeps.size())) ||
+      (rc = d.upload(&d.bwrites, p.bwrites.data(), p.bwrites.size())) ||
+      (rc = d.upload(&d.region_bw, p.region_bw.data(), p.region_bw.size())))
     return rc;
   if (in.pd) {
     if ((rc = d.upload(&d.hap_words, p.words.data(), p.words.size()))) return rc;
@@ -394,13 +412,16 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
   }
   if ((rc = d.alloc(&d.counters, 32))) return rc;
   if ((rc = d.alloc(&d.out, size_t(p.n_pairs)))) return rc;
-  // LDS per wave: the longest haplotype plus the ramp of the widest read, in words.
-  if (p.max_slots > kMaxSlots) {
+  if (p.max_slots > kMaxSlots || p.max_nodes > kMaxSlots) {
     set_error(kInvalidArgument, "a region holds more than " + std::to_string(kMaxSlots) +
-                                    " distinct haplotype prefix columns");
+                                    " distinct haplotype prefix columns or shared suffixes");
     return kInvalidArgument;
   }
-  d.lds_words = ((p.max_hap_len + 64 + 8) + 3) & ~3;
+  // LDS per wave: the longest haplotype (plus the column a backward sweep adds past its end) and
+  // the ramp of the widest read, in words.
+  // A join reduces up to 512 row terms and 66 chunk sums in FP64 there.
+  d.lds_words = ((p.max_hap_len + 1 + 64 + 8) + 3) & ~3;
+  if (!p.bsweeps.empty()) d.lds_words = std::max(d.lds_words, 2 * (512 + 72));
   if (int64_t(d.lds_words) * 4 * (256 / 64) > 64 * 1024) {  // four waves per 256-thread block
     set_error(kInvalidArgument, "haplotype of " + std::to_string(p.max_hap_len) +
                                     " bases exceeds the engine's LDS staging capacity");
@@ -412,7 +433,7 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
   }
   // Scratch for the largest configuration any launch may use, so tuning never reallocates.
   int64_t need = 0;
-  if (p.max_slots > 0) {
+  if (p.max_slots > 0 || p.max_nodes > 0) {
     for (int R = 1; R <= kMaxRowsPerLane; ++R) {
       if (!d.bucket_n[R - 1]) continue;
       for (const LaunchCfg& c : d.candidates(R, true))
@@ -642,6 +663,14 @@ int DeviceBatch::results(double* out, int64_t n) {
 
 int64_t DeviceBatch::n_pairs() const { return impl_->plan.n_pairs; }
 int64_t DeviceBatch::scratch_bytes() const { return int64_t(impl_->scratch_bytes); }
+
+void DeviceBatch::work(int64_t out[4]) const {
+  const HostPlan& p = impl_->plan;
+  out[0] = p.work_nominal;
+  out[1] = p.work_prefix;
+  out[2] = p.work_forward;
+  out[3] = p.work_backward;
+}
 
 int dpp_selftest(int32_t* out64) {
   DeviceState* st = nullptr;

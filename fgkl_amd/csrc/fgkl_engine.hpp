@@ -53,6 +53,8 @@ class DeviceBatch {
   int results(double* out, int64_t n);
   int64_t n_pairs() const;
   int64_t scratch_bytes() const;
+  // DP cells: nominal, with prefix sharing alone, forward and backward with suffix sharing.
+  void work(int64_t out[4]) const;
   // Times every launch configuration of each FP32 bucket (rows R or R+1, 256 or 64 threads,
   // natural or occupancy-raised build), keeps the fastest for this batch and for later batches
   // of the same kind on this device, and drops captured graphs.

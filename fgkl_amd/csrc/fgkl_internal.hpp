@@ -60,8 +60,24 @@ struct HapPlan {        // one per haplotype, region-contiguous, in sorted order
   int32_t start_src_len;  // length of the haplotype that wrote that snapshot
   int32_t snap0;        // first entry in the snapshot list
   int32_t nsnap;        // columns this haplotype must snapshot
+  int32_t cut;          // suffix sharing: forward sweep stops at column cut - 1 (0: no cut)
+  int32_t cut_node;     // backward node the cut joins (record index)
 };
-static_assert(sizeof(HapPlan) == 40, "HapPlan layout");
+static_assert(sizeof(HapPlan) == 48, "HapPlan layout");
+
+// One backward sweep of a region's suffix plan (plan.rs:14-31): over columns right-1 down to
+// left of sorted haplotype `hap` (a global plan index), starting from node `start_node`'s record
+// at column `right` or, when -1, from the zero state beyond the last column (then right = len+1
+// and the sweep also computes column len, which only trailing insertions leave).  Records node
+// bwrites[w0 + i] = (column, node) as it passes those columns, in descending column order.
+struct BwdSweep {
+  int32_t hap;
+  int32_t right;
+  int32_t left;
+  int32_t start_node;
+  int32_t w0;
+  int32_t nw;
+};
 
 struct SnapEntry {
   int32_t col;   // 1-based haplotype column
@@ -97,6 +113,16 @@ struct HostPlan {
   std::vector<uint32_t> words;      // PD column words (same offsets as the haplotype bytes)
   int32_t max_slots = 0;
   int32_t max_hap_len = 0;
+  // Suffix sharing (plain batches only): the backward sweeps of region g are
+  // bsweeps[region_bw[g] .. region_bw[g + 1]), in the order they must run; their node writes
+  // are pairs (column, node) in bwrites; a region uses at most max_nodes node records.
+  std::vector<BwdSweep> bsweeps;
+  std::vector<int32_t> bwrites;
+  std::vector<int32_t> region_bw;
+  int32_t max_nodes = 0;
+  // DP cells of the batch: nominal (every read base x haplotype base), computed with prefix
+  // sharing alone, and with suffix sharing (forward, backward) in the FP32 pass.
+  int64_t work_nominal = 0, work_prefix = 0, work_forward = 0, work_backward = 0;
   // Work per rows-per-lane bucket (index R-1), costliest first: items of two int32, a read and
   // a second read of the same region sharing the wave (-1 when the read is alone).
   std::vector<int32_t> bucket_items[kMaxRowsPerLane];

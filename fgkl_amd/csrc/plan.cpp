@@ -82,6 +82,100 @@ uint32_t column_states(const uint8_t* flags, int64_t m, uint32_t state, uint32_t
   return state;
 }
 
+// ---- plan.rs:55-230: suffix sharing ----------------------------------------------------------
+// A distinct sorted haplotype whose longest prefix shared with another is p and longest suffix
+// shared with another is s stops its forward sweep at column len - d - 1 with
+// d = min(s, len - p - 1) (when d >= 4), and joins there the backward values of depth d, which
+// backward sweeps compute once per shared suffix: in reversed-bases order, each resuming from
+// the deepest depth computed along its suffix so far.
+struct SuffixPlan {
+  std::vector<int32_t> depth;  // per sorted haplotype (0: no cut)
+  std::vector<int32_t> node;   // per sorted haplotype: the node its cut joins
+  struct Sweep {
+    int32_t hap, resume, need, start;
+    std::map<int32_t, int32_t> node_of;  // depth -> node, for the depths it records
+  };
+  std::vector<Sweep> sweeps;
+  int32_t nodes = 1;  // node 0: depth 0 (never stored)
+};
+
+SuffixPlan plan_suffixes(const std::vector<const uint8_t*>& bases, const std::vector<int32_t>& len,
+                         const std::vector<int64_t>& lcp, const std::vector<char>& dup) {
+  constexpr int32_t kMinSaving = 4;  // plan.rs:9
+  const int32_t H = int32_t(bases.size());
+  SuffixPlan plan;
+  plan.depth.assign(size_t(H), 0);
+  plan.node.assign(size_t(H), 0);
+  std::vector<int32_t> distinct;
+  for (int32_t k = 0; k < H; ++k)
+    if (!dup[size_t(k)]) distinct.push_back(k);
+  auto common_suffix = [&](int32_t a, int32_t b) {
+    int32_t n = 0;
+    while (n < len[size_t(a)] && n < len[size_t(b)] &&
+           bases[size_t(a)][len[size_t(a)] - 1 - n] == bases[size_t(b)][len[size_t(b)] - 1 - n])
+      ++n;
+    return n;
+  };
+  std::vector<int32_t> order = distinct;  // by reversed bases, then sorted index
+  std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    const int32_t n = std::min(len[size_t(a)], len[size_t(b)]);
+    for (int32_t i = 0; i < n; ++i) {
+      const uint8_t x = bases[size_t(a)][len[size_t(a)] - 1 - i];
+      const uint8_t y = bases[size_t(b)][len[size_t(b)] - 1 - i];
+      if (x != y) return x < y;
+    }
+    return len[size_t(a)] != len[size_t(b)] ? len[size_t(a)] < len[size_t(b)] : a < b;
+  });
+  const int32_t D = int32_t(order.size());
+  std::vector<int32_t> lcs(size_t(D), 0), pos(size_t(H), 0), sweep_at(size_t(D), -1);
+  for (int32_t r = 0; r < D; ++r) {
+    if (r) lcs[size_t(r)] = common_suffix(order[size_t(r - 1)], order[size_t(r)]);
+    pos[size_t(order[size_t(r)])] = r;
+  }
+  for (size_t t = 0; t < distinct.size(); ++t) {
+    const int32_t k = distinct[t];
+    const int64_t next = t + 1 < distinct.size() ? lcp[size_t(distinct[t + 1])] : 0;
+    const int32_t shared_prefix = int32_t(std::max<int64_t>(lcp[size_t(k)], next));
+    const int32_t L = len[size_t(k)];
+    if (shared_prefix >= L) continue;
+    const int32_t r = pos[size_t(k)];
+    const int32_t s = std::max(lcs[size_t(r)], r + 1 < D ? lcs[size_t(r + 1)] : 0);
+    const int32_t d = std::min(s, L - shared_prefix - 1);
+    if (d >= kMinSaving) plan.depth[size_t(k)] = d;
+  }
+  int32_t reach = 0;
+  for (int32_t r = 0; r < D; ++r) {
+    const int32_t k = order[size_t(r)];
+    const int32_t resume = std::min(lcs[size_t(r)], reach);
+    if (plan.depth[size_t(k)] > resume) {
+      sweep_at[size_t(r)] = int32_t(plan.sweeps.size());
+      plan.sweeps.push_back(SuffixPlan::Sweep{k, resume, plan.depth[size_t(k)], 0, {}});
+    }
+    reach = std::max(resume, plan.depth[size_t(k)]);
+  }
+  // The node holding depth d of the suffix at position r: the one the latest sweep at or before
+  // r wrote for a suffix r shares at least that deep (plan.rs:205-230).
+  auto node_for = [&](int32_t r, int32_t d) -> int32_t {
+    if (d == 0) return 0;
+    int32_t shared = INT32_MAX;
+    for (int32_t q = r; q >= 0; --q) {
+      const int32_t si = sweep_at[size_t(q)];
+      if (si >= 0 && plan.sweeps[size_t(si)].resume < d && d <= plan.sweeps[size_t(si)].need) {
+        auto ins = plan.sweeps[size_t(si)].node_of.emplace(d, plan.nodes);
+        if (ins.second) ++plan.nodes;
+        return ins.first->second;
+      }
+      shared = std::min(shared, lcs[size_t(q)]);
+      if (shared < d) break;
+    }
+    return -1;  // unreachable for a consistent plan
+  };
+  for (auto& sw : plan.sweeps) sw.start = node_for(pos[size_t(sw.hap)], sw.resume);
+  for (int32_t k : distinct)
+    if (plan.depth[size_t(k)] > 0) plan.node[size_t(k)] = node_for(pos[size_t(k)], plan.depth[size_t(k)]);
+  return plan;
+}
+
 struct RegionHaps {
   const BatchInput* in;
   const std::vector<uint32_t>* end_state;
@@ -221,6 +315,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
     const int64_t h0 = in.region_hap[g], h1 = in.region_hap[g + 1], H = h1 - h0;
     p.region_out[size_t(g)] = out_base;
+    p.region_bw.push_back(int32_t(p.bsweeps.size()));
     p.region_hap0[size_t(g)] = int32_t(h0);
     p.region_nhap[size_t(g)] = int32_t(H);
     out_base += (r1 - r0) * H;
@@ -265,6 +360,8 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       hp.start_src_len = 0;
       hp.snap0 = int32_t(p.snaps.size());
       hp.nsnap = 0;
+      hp.cut = 0;
+      hp.cut_node = 0;
       p.max_hap_len = std::max(p.max_hap_len, hp.len);
       if (hp.dup) continue;
       const int64_t start = k == 0 ? 0 : lcp[size_t(k)];
@@ -283,15 +380,54 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         }
       }
     }
+    if (!in.pd && H > 1) {
+      // Suffix sharing for plain haplotypes (the reference shares suffixes in its plain kernel
+      // only, kernel.rs:53-81); the kernel applies it in the FP32 pass with prefix sharing.
+      std::vector<const uint8_t*> hb(static_cast<size_t>(H));
+      std::vector<int32_t> hl(static_cast<size_t>(H));
+      std::vector<char> dp(static_cast<size_t>(H));
+      for (int64_t k = 0; k < H; ++k) {
+        const HapPlan& hp = p.plan[size_t(h0 + k)];
+        hb[size_t(k)] = in.hap_bases + hp.off;
+        hl[size_t(k)] = hp.len;
+        dp[size_t(k)] = char(hp.dup);
+      }
+      const SuffixPlan sp = plan_suffixes(hb, hl, lcp, dp);
+      for (int64_t k = 0; k < H; ++k) {
+        const int32_t d = sp.depth[size_t(k)];
+        if (d > 0) {
+          HapPlan& hp = p.plan[size_t(h0 + k)];
+          hp.cut = hp.len - d;
+          hp.cut_node = sp.node[size_t(k)] - 1;  // node records exclude node 0
+        }
+      }
+      for (const auto& sw : sp.sweeps) {
+        const int32_t L = hl[size_t(sw.hap)];
+        BwdSweep b;
+        b.hap = int32_t(h0 + sw.hap);
+        b.right = sw.resume > 0 ? L - sw.resume : L + 1;
+        b.left = L - sw.need;
+        b.start_node = sw.resume > 0 ? sw.start - 1 : -1;
+        b.w0 = int32_t(p.bwrites.size() / 2);
+        b.nw = int32_t(sw.node_of.size());
+        for (const auto& dn : sw.node_of) {  // ascending depth: descending column
+          p.bwrites.push_back(L - dn.first);
+          p.bwrites.push_back(dn.second - 1);
+        }
+        p.bsweeps.push_back(b);
+      }
+      p.max_nodes = std::max(p.max_nodes, sp.nodes - 1);
+    }
   }
+  p.region_bw.push_back(int32_t(p.bsweeps.size()));
   p.region_out[size_t(G)] = out_base;
   p.n_pairs = out_base;
 
   // Work items.  Two reads of a region can share a wave: read A's rows fill lanes [0, L_A), read
   // B's the lanes after, both swept against the same haplotype columns (B lags L_A steps).  A
   // step costs about kStepFixed + kStepPerRow * R VALU instructions (wave-wide moves, the column
-  // word, then R cells), and a region's sweeps take sum(W) + H' (L - 1) steps (H' haplotypes
-  // swept, W their columns after the shared prefix).  Reads are paired greedily, longest first,
+  // word, then R cells), and a region's sweeps take sum(W) + H' (L - 1) steps (H' sweeps, forward
+  // over the columns between the shared prefix and the cut, and backward over shared suffixes).  Reads are paired greedily, longest first,
   // each with the free read that lowers the modelled cost most; the candidates are, for every R,
   // the longest free read that fits beside it.
   constexpr int64_t kStepFixed = 20, kStepPerRow = 19;
@@ -302,9 +438,28 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
       hap_bases += hp.len;
       if (!hp.dup) {
-        cols += hp.len - hp.lcp;
+        cols += (hp.cut ? hp.cut - 1 : hp.len) - hp.lcp;  // the FP32 pass stops at its cut
         swept += 1;
       }
+    }
+    int64_t prefix_cols = 0, fwd_cols = cols, bwd_cols = 0;
+    for (int32_t k = 0; k < H; ++k) {
+      const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
+      if (!hp.dup) prefix_cols += hp.len - hp.lcp;
+    }
+    for (int32_t w = p.region_bw[size_t(g)]; w < p.region_bw[size_t(g) + 1]; ++w) {
+      cols += p.bsweeps[size_t(w)].right - p.bsweeps[size_t(w)].left;
+      bwd_cols += p.bsweeps[size_t(w)].right - p.bsweeps[size_t(w)].left;
+      swept += 1;
+    }
+    {
+      int64_t read_bases = 0;
+      for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r)
+        read_bases += in.read_off[r + 1] - in.read_off[r];
+      p.work_nominal += read_bases * hap_bases;
+      p.work_prefix += read_bases * prefix_cols;
+      p.work_forward += read_bases * fwd_cols;
+      p.work_backward += read_bases * bwd_cols;
     }
     auto item_cost = [&](int R, int64_t L) {
       return (cols + swept * (L - 1)) * (kStepFixed + kStepPerRow * R);

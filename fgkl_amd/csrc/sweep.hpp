@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "fgkl_internal.hpp"
 #include "wave64.hpp"
@@ -79,6 +80,14 @@ struct SweepArgs {
   int32_t* rescue_count;
   int32_t rescue;
   int32_t lds_words;           // LDS words per wave for the staged haplotype
+  // Suffix sharing (FP32 plain main pass only, when `suffix` is set): the backward sweeps of
+  // region g are bsweeps[region_bw[g] .. region_bw[g + 1]) with node writes in bwrites; node
+  // records sit `node_off` elements into the wave's scratch, after its snapshot slots.
+  const BwdSweep* bsweeps;
+  const int32_t* bwrites;
+  const int32_t* region_bw;
+  int32_t suffix;
+  int64_t node_off;
 };
 
 // Qualities above 254 are clamped (model.rs:53-64).
@@ -302,16 +311,17 @@ constexpr uint32_t kSlotShift = 20;
 constexpr int kMaxSlots = 4095;
 
 // One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
-// column start+1+t (zero beyond W, 16-byte aligned).  Returns the lane's raw scaled sum over the
-// read's last row (meaningful in lane L-1).  `cap` receives lane cap_lane's sum right after step
-// cap_t: the first read of a two-read wave finishes column W there, and its last lane would go on
-// summing the columns past W while the second read's lanes finish.
+// column start+1+t (zero beyond W, 16-byte aligned).  Leaves in `s` each lane's state; s.acc is
+// the raw scaled sum over the read's last row (meaningful in lane L-1).  `cap` receives lane
+// cap_lane's sum right after step cap_t: the first read of a two-read wave finishes column W
+// there, and its last lane would go on summing the columns past W while the second read's lanes
+// finish.  With `freeze`, a lane stops once it has computed column W instead, so that every
+// lane ends holding its rows of column W (a cut haplotype joins from there) and cap is unused.
 template <typename T, int R, bool PD, bool SNAP>
-__device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
-                   const T* snap_in, T scale, T* scratch, bool row0_above, uint32_t keep,
-                   int cap_t, int cap_lane, T& cap) {
+__device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
+                      const T* snap_in, T scale, T* scratch, bool row0_above, uint32_t keep,
+                      int cap_t, int cap_lane, T& cap, bool freeze, Lane<T, R, PD>& s) {
   constexpr int NV = snap_values<R, PD>();
-  Lane<T, R, PD> s;
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
   if (snap_in) {
     const T* p = snap_in + lane * snap_stride<R, PD>();
@@ -369,10 +379,12 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
   const int ramp = min(steps, (L - 1 + 3) & ~3);
   int t = 0;
   Cell<T> Q;
+  // Lane l computes column t - l + 1 at step t; once frozen, only columns up to W.
+  const int last = freeze ? W : steps;
   for (; t < ramp; ++t) {
     s.w = shr1(lw[t], s.w);
     shift_up(s, Q, keep);
-    if (t >= lane) {
+    if (t >= lane && t - lane < last) {
       cells(s, rw, P, Q);
       P = Q;
       record(P);
@@ -380,8 +392,9 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
     if (t == cap_t) cap = readlane(s.acc, cap_lane);
   }
   // Steady state: every lane is live; the diagonal and up neighbours alternate between P and Q.
-  // It stops by step cap_t; the single steps after it test for the capture.
-  const int steady_end = min(steps, cap_t + 1);
+  // It stops by step cap_t (frozen: before lane 0 passes column W); the single steps after it
+  // test for the capture and, frozen, which lanes still compute.
+  const int steady_end = freeze ? W : min(steps, cap_t + 1);
   uint4 q = *reinterpret_cast<const uint4*>(lw + t);
   for (; t + 4 <= steady_end; t += 4) {
     const uint4 cq = q;
@@ -407,20 +420,20 @@ __device__ T sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, i
   for (; t < steps; ++t) {
     s.w = shr1(lw[t], s.w);
     shift_up(s, Q, keep);
-    cells(s, rw, P, Q);
-    P = Q;
-    record(P);
+    if (t - lane < last) {
+      cells(s, rw, P, Q);
+      P = Q;
+      record(P);
+    }
     if (t == cap_t) cap = readlane(s.acc, cap_lane);
   }
-  return s.acc;
 }
 
 // Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded, and marks
 // the columns this sweep must record with their scratch slots.
 template <bool PD>
-__device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int L,
-                                          int lane, uint32_t* lw, int nsnap) {
-  const int W = p.len - start;
+__device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int W,
+                                          int L, int lane, uint32_t* lw, int nsnap) {
   const int total = W + L + 8;  // the steady loop reads one group of four ahead
   wave_sync_lds();
   for (int i = lane; i < total; i += 64) {
@@ -432,10 +445,136 @@ __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, 
     wave_sync_lds();
     for (int i = lane; i < nsnap; i += 64) {
       const SnapEntry e = a.snaps[p.snap0 + i];
-      lw[e.col - start - 1] |= uint32_t(e.slot + 1) << kSlotShift;
+      if (e.col - start <= W) lw[e.col - start - 1] |= uint32_t(e.slot + 1) << kSlotShift;
     }
   }
   wave_sync_lds();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Suffix sharing (FP32 plain): backward sweeps and the join (kernel.rs:579-602, 744-891,
+// 1029-1126).  B_M, B_X, B_Y of a cell are the scaled mass of the paths from that cell to the end
+// of the read (2^120 per path that ends, simd.rs:109).  The backward recurrence mirrors the
+// forward one: row i at column j reads row i+1 at column j+1 (match, through the prior and
+// transitions of read position i+1), row i+1 at column j (insertion) and row i at column j+1
+// (deletion).  In the lane layout rows run bottom-up inside a lane and the row below a lane's
+// last row arrives from lane l+1 by DPP wave_shl:1; columns run right to left, each read's last
+// lane first.  The identity row that stands for DP row 0 keeps D->D 1, so its deletion value
+// becomes the running sum S of the weights of paths that start in row 0 at or after the column
+// (kernel.rs:1104-1126); identity rows above it stay zero.
+
+// A node record: per lane its rows' B_M and B_Y at the node's column and B_M of the row below
+// the lane's last row there (the next column's diagonal), padded to four values.
+template <int R> constexpr int node_stride() { return (2 * R + 1 + 3) & ~3; }
+
+// Parameters of the row below a lane's last row (lane l+1's first row), zero for a read's last
+// lane (nothing lies below its last row).
+struct EdgeRow {
+  float pm, px, mm, im, mi, gc;
+  uint32_t rb;
+};
+
+// Stages the words of backward sweep `b` into the wave's LDS: word t is for column right-1-t and
+// holds the haplotype base of column right-t (the column its match moves into; none for the
+// last column) and, at the columns the sweep records, the node record index plus one.
+__device__ __forceinline__ void stage_bwd(const SweepArgs& a, const BwdSweep& b, const HapPlan& h,
+                                          int L, int lane, uint32_t* lw) {
+  const int Wb = b.right - b.left;
+  const int total = Wb + L + 8;
+  wave_sync_lds();
+  for (int i = lane; i < total; i += 64) {
+    uint32_t v = 0x100u;  // no base: matches nothing
+    const int j = b.right - 1 - i;
+    if (i < Wb && j < h.len) v = uint32_t(a.hap_bytes[h.off + j]);
+    lw[i] = v;
+  }
+  wave_sync_lds();
+  for (int i = lane; i < b.nw; i += 64) {
+    const int col = a.bwrites[2 * (b.w0 + i)], node = a.bwrites[2 * (b.w0 + i) + 1];
+    lw[b.right - 1 - col] |= uint32_t(node + 1) << kSlotShift;
+  }
+  wave_sync_lds();
+}
+
+// One backward sweep of Wb columns.  A lane starts `delay` steps after its read's last lane (the
+// `bottom` lanes read the words from LDS; the others take them from the lane below).  `sink` is
+// 2^120 in a read's last lane (whose last register holds the read's last row), zero elsewhere.
+// Starts from node record `rec_in` (this lane's values at column `right`) or from zeros.
+// `from_lds` is all ones in the bottom lanes, zero elsewhere, and opaque to the optimiser: words
+// are merged with bit operations rather than a select, which the compiler could turn into a DPP
+// move run only in the lanes that keep its result (a DPP move reads a lane outside EXEC as zero).
+// Lanes past the reads (lane >= L) start from zeros: their records were never written, and the
+// upward DPP moves lane L's values into lane L-1, where a stale NaN times the zero parameters
+// below a read's last row would still be NaN.
+template <int R>
+__device__ void bsweep(const Rows<float, R, false>& rw, const EdgeRow& eg, const uint32_t* lw,
+                       int Wb, int steps, int lane, int L, int delay, uint32_t from_lds,
+                       float sink, const float* rec_in, float* nodes) {
+  constexpr int NS = node_stride<R>();
+  float bm[R], bx[R], by[R];
+  float P = 0.f;  // B_M of the row below the last row, one column to the right
+  if (rec_in) {
+    const float* r = rec_in + lane * NS;
+    const bool live = lane < L;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      bm[k] = live ? r[k] : 0.f;
+      by[k] = live ? r[R + k] : 0.f;
+      bx[k] = 0.f;
+    }
+    P = live ? r[2 * R] : 0.f;
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) bm[k] = bx[k] = by[k] = 0.f;
+  }
+  uint32_t w = 0;
+  for (int t = 0; t < steps; ++t) {
+    w = (shl1z(w) & ~from_lds) | (lw[t] & from_lds);
+    const float qm = shl1z(bm[0]), qx = shl1z(bx[0]);  // the row below, this column
+    const int c = t - delay;
+    if (c >= 0 && c < Wb) {
+      const uint32_t hb = w & 0x1FFu;
+      const bool hN = hb == uint32_t('N');
+      float below_x = qx, diag = P;
+#pragma unroll
+      for (int k = R - 1; k >= 0; --k) {
+        const bool edge = k == R - 1;
+        const float pmb = edge ? eg.pm : rw.pm[k < R - 1 ? k + 1 : k];
+        const float pxb = edge ? eg.px : rw.px[k < R - 1 ? k + 1 : k];
+        const uint32_t rbb = edge ? eg.rb : rw.rb[k < R - 1 ? k + 1 : k];
+        const float mmb = edge ? eg.mm : rw.mm[k < R - 1 ? k + 1 : k];
+        const float imb = edge ? eg.im : rw.im[k < R - 1 ? k + 1 : k];
+        const float mib = edge ? eg.mi : rw.mi[k < R - 1 ? k + 1 : k];
+        const float iib = edge ? eg.gc : rw.gc[k < R - 1 ? k + 1 : k];
+        const bool match = hN | (hb == rbb);
+        const float into = (match ? pmb : pxb) * diag;  // kernel.rs:1071-1082
+        const float ind = into * imb;
+        float nbx = fmaf(below_x, iib, ind);
+        const float nby = fmaf(by[k], rw.gc[k], ind);
+        float nbm = fmaf(by[k], rw.md[k], fmaf(below_x, mib, into * mmb));
+        if (edge) {
+          nbm += sink;
+          nbx += sink;
+        }
+        diag = bm[k];  // row k at column j+1: the diagonal of row k-1
+        bm[k] = nbm;
+        bx[k] = nbx;
+        by[k] = nby;
+        below_x = nbx;
+      }
+      P = qm;
+      const uint32_t node = w >> kSlotShift;
+      if (node) {
+        float* r = nodes + (size_t(node - 1) * 64 + lane) * NS;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          r[k] = bm[k];
+          r[R + k] = by[k];
+        }
+        r[2 * R] = P;
+      }
+    }
+  }
 }
 
 // Waves per SIMD the occupancy-raised FP32 variants ask the compiler for: one more than the
@@ -510,6 +649,48 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     uint32_t keep = (two && lane == La) ? 0u : ~0u;
     asm volatile("" : "+v"(keep));
 
+    // Suffix sharing: the region's backward sweeps, once per item, before its forward sweeps.
+    constexpr bool kSuffix = std::is_same<T, float>::value && !PD;
+    const bool suffix = kSuffix && a.suffix && !a.pair_mode;
+    T* nodes = scratch ? scratch + a.node_off : nullptr;
+    if constexpr (kSuffix) {
+      if (suffix) {
+        const bool bottom = lane == La - 1 || (two && lane == L - 1);
+        const int delay = lane < La ? La - 1 - lane : (lane < L ? L - 1 - lane : 1 << 20);
+        const float sink = bottom ? float(Num<float>::kScale) : 0.f;
+        // All ones in the read's last lanes; opaque (see shift_up and bsweep).
+        uint32_t from_lds = bottom ? ~0u : 0u;
+        asm volatile("" : "+v"(from_lds));
+        EdgeRow eg;  // lane l+1's first row, zero below a read's last lane
+        eg.pm = and_bits(shl1z(rw.pm[0]), ~from_lds);
+        eg.px = and_bits(shl1z(rw.px[0]), ~from_lds);
+        eg.mm = and_bits(shl1z(rw.mm[0]), ~from_lds);
+        eg.im = and_bits(shl1z(rw.im[0]), ~from_lds);
+        eg.mi = and_bits(shl1z(rw.mi[0]), ~from_lds);
+        eg.gc = and_bits(shl1z(rw.gc[0]), ~from_lds);
+        eg.rb = shl1z(rw.rb[0]) & ~from_lds;
+        const int steps_extra = max(La, Lb) - 1;
+        const int w1 = __builtin_amdgcn_readfirstlane(a.region_bw[g + 1]);
+        for (int w = __builtin_amdgcn_readfirstlane(a.region_bw[g]); w < w1; ++w) {
+          BwdSweep b = a.bsweeps[w];
+          b.hap = __builtin_amdgcn_readfirstlane(b.hap);
+          b.right = __builtin_amdgcn_readfirstlane(b.right);
+          b.left = __builtin_amdgcn_readfirstlane(b.left);
+          b.start_node = __builtin_amdgcn_readfirstlane(b.start_node);
+          b.w0 = __builtin_amdgcn_readfirstlane(b.w0);
+          b.nw = __builtin_amdgcn_readfirstlane(b.nw);
+          HapPlan h = a.plan[b.hap];
+          h.len = __builtin_amdgcn_readfirstlane(h.len);
+          stage_bwd(a, b, h, L, lane, lw);
+          const int Wb = b.right - b.left;
+          bsweep<R>(rw, eg, lw, Wb, Wb + steps_extra, lane, L, delay, from_lds, sink,
+                    b.start_node >= 0 ? nodes + size_t(b.start_node) * 64 * node_stride<R>()
+                                      : nullptr,
+                    nodes);
+        }
+      }
+    }
+
     double prev_a = 0.0, prev_b = 0.0;
     for (int k = k_begin; k < k_end; ++k) {
       HapPlan p = a.plan[hap0 + k];
@@ -519,6 +700,8 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       p.nsnap = __builtin_amdgcn_readfirstlane(p.nsnap);
       p.snap0 = __builtin_amdgcn_readfirstlane(p.snap0);
       p.start_slot = __builtin_amdgcn_readfirstlane(p.start_slot);
+      p.cut = __builtin_amdgcn_readfirstlane(p.cut);
+      p.cut_node = __builtin_amdgcn_readfirstlane(p.cut_node);
       double res_a, res_b;
       if (!a.pair_mode && p.dup) {
         res_a = prev_a;  // kernel.rs:237-246
@@ -534,22 +717,77 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
           scale = T(double(p.start_src_len) / double(p.len));  // kernel.rs:641
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
-        stage_hap<PD>(a, p, start, L, lane, lw, nsnap);
+        // A cut haplotype sweeps up to the column before its cut (kernel.rs:251-252).
+        const bool cut = suffix && p.cut > 0;
+        const int W = (cut ? p.cut - 1 : p.len) - start;
+        stage_hap<PD>(a, p, start, W, L, lane, lw, nsnap);
         // Read A's last lane finishes column W at step W + La - 2.
-        const int W = p.len - start;
         T acc_a = T(0);
-        const T acc =
-            nsnap ? sweep<T, R, PD, true>(rw, lw, W, L, lane, init, snap_in, scale, scratch,
-                                          row0_above, keep, W + La - 2, La - 1, acc_a)
-                  : sweep<T, R, PD, false>(rw, lw, W, L, lane, init, snap_in, scale, scratch,
-                                           row0_above, keep, W + La - 2, La - 1, acc_a);
+        Lane<T, R, PD> st;
+        if (nsnap)
+          sweep<T, R, PD, true>(rw, lw, W, L, lane, init, snap_in, scale, scratch, row0_above,
+                                keep, W + La - 2, La - 1, acc_a, cut, st);
+        else
+          sweep<T, R, PD, false>(rw, lw, W, L, lane, init, snap_in, scale, scratch, row0_above,
+                                 keep, W + La - 2, La - 1, acc_a, cut, st);
         // A read's last row is its last lane's last register (kernel.rs:897-914).
         auto finish = [](double raw) {
           if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted)) return __builtin_nan("");
           return log10(raw) - Num<T>::kLog10Scale;
         };
-        res_a = finish(double(acc_a));
-        res_b = finish(double(readlane(acc, L - 1)));
+        double raw_a = double(cut ? readlane(st.acc, La - 1) : acc_a);
+        double raw_b = double(readlane(st.acc, L - 1));
+        if constexpr (kSuffix) {
+          if (cut) {
+            // The join (kernel.rs:744-791): the values entering the cut column from column W
+            // (match from the diagonal, deletion from the left), times the cut node's backward
+            // values, summed in FP64 per read; DP row 0's identity row contributes C/m times
+            // the start-weight sum S through its deletion value.
+            // Each DP row's term goes to LDS at its row index (read A's rows 0..na, then read
+            // B's), is summed in chunks of eight rows, and the chunks in order: the grouping
+            // depends on row numbers only, not on how the rows sit in lanes, so a read gets the
+            // same bits whatever it shares its wave with and at any rows per lane.
+            const float um = shr1z(st.m[R - 1]), ux = shr1z(st.x[R - 1]), uy = shr1z(st.y[R - 1]);
+            const uint32_t hc = uint32_t(a.hap_bytes[p.off + p.cut - 1]);
+            const float* rec = nodes + (size_t(p.cut_node) * 64 + lane) * node_stride<R>();
+            double* rows = reinterpret_cast<double*>(lw);   // [na + 1 + nb + 1]
+            double* chunks = rows + 2 * kMaxRowsPerLane * 64 / 2;  // after 512 rows
+            const int base = in_b ? na + 1 : 0;
+            const int first = R * (in_b ? lane - La : lane) - (in_b ? padb : pada);  // row - 1
+            wave_sync_lds();
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+              const float dm = k ? st.m[k - 1] : um;
+              const float dx = k ? st.x[k - 1] : ux;
+              const float dy = k ? st.y[k - 1] : uy;
+              const bool match = (hc == uint32_t('N')) | (hc == rw.rb[k]);
+              const float jm = (match ? rw.pm[k] : rw.px[k]) *
+                               fmaf(dy, rw.im[k], fmaf(dx, rw.im[k], dm * rw.mm[k]));
+              const float jy = fmaf(st.y[k], rw.gc[k], st.m[k] * rw.md[k]);
+              const double term = fma(double(jm), double(rec[k]), double(jy) * double(rec[R + k]));
+              const int row = first + k + 1;  // DP row; rows above DP row 0 hold nothing
+              if (lane < L && row >= 0) rows[base + row] = term;
+            }
+            wave_sync_lds();
+            const int ca = (na + 1 + 7) >> 3, cb = two ? (nb + 1 + 7) >> 3 : 0;
+            for (int c = lane; c < ca + cb; c += 64) {
+              const int r0 = c < ca ? 8 * c : na + 1 + 8 * (c - ca);
+              const int r1 = c < ca ? min(8 * c + 8, na + 1) : min(r0 + 8, na + 1 + nb + 1);
+              double sum = 0.0;
+              for (int r = r0; r < r1; ++r) sum += rows[r];
+              chunks[c] = sum;
+            }
+            wave_sync_lds();
+            double sa = 0.0, sb = 0.0;
+            for (int c = 0; c < ca; ++c) sa += chunks[c];
+            for (int c = ca; c < ca + cb; ++c) sb += chunks[c];
+            wave_sync_lds();
+            raw_a += sa * (1.0 / Num<float>::kScale);
+            raw_b += sb * (1.0 / Num<float>::kScale);
+          }
+        }
+        res_a = finish(raw_a);
+        res_b = finish(raw_b);
       }
       if (lane == 0) {
         a.out[out_a + p.src] = res_a;

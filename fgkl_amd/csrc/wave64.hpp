@@ -47,6 +47,14 @@ __device__ __forceinline__ double shr1z(double src) {
   return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
+// DPP wave_shl:1 with lane 63 receiving zero (bound_ctrl).
+__device__ __forceinline__ uint32_t shl1z(uint32_t src) {
+  return uint32_t(__builtin_amdgcn_mov_dpp(int(src), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float shl1z(float src) {
+  return __builtin_bit_cast(float, int(shl1z(__builtin_bit_cast(uint32_t, src))));
+}
+
 __device__ __forceinline__ float readlane(float v, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }

@@ -131,6 +131,13 @@ int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_grap
                    fgkl_run_stats* stats);
 int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoods_len);
 int64_t fgkl_batch_pairs(const fgkl_batch* batch);
+/*
+ * The work the batch's plan computes, in DP cells (read bases x haplotype columns): [0] nominal
+ * (every read base x every haplotype base, as GCUPS counts them), [1] with prefix sharing alone
+ * (FP64 mode), [2] forward and [3] backward with suffix sharing too (the FP32 pass with
+ * prefix sharing; plain batches only, PD batches report 0 backward and forward = [1]).
+ */
+int fgkl_batch_work(const fgkl_batch* batch, int64_t* cells4);
 
 /*
  * Launch autotuner (no reference counterpart: fgkl picks lanes per batch from the CPU's SIMD

@@ -630,6 +630,95 @@ def test_tuning_keeps_results_and_is_inherited_by_later_batches(gpu):
     b.close()
 
 
+# ---- suffix sharing (plan.rs, kernel.rs:579-602, 744-891, 1029-1126) ---------------------------
+@pytest.mark.parametrize("wl,regions", [(1, 6), (4, 8), (0, 1)])
+def test_suffix_sharing_matches_the_oracle(gpu, wl, regions):
+    data = workload(wl, seed=515 + wl, n_regions=regions)
+    expected = batch_reference(data)
+    b = F.Batch(data)
+    work = b.work()
+    assert work["backward"] > 0 and work["forward"] < work["prefix"] < work["nominal"]
+    st = b.run(F.MODE_FP32_RESCUE, share=True)
+    assert_close(b.results(), expected, FLOAT_TOL, f"suffix sharing, workload {wl}")
+    shared = b.results()
+    b.run(F.MODE_FP32_RESCUE, share=False)
+    assert_close(b.results(), shared, FLOAT_TOL, "prefix-only vs suffix sharing")
+    b.run(F.MODE_FP64, share=True)  # FP64 mode shares prefixes only
+    assert_rel(b.results(), expected, what="fp64 with a suffix plan")
+    # The scratch now holds FP64 records: the FP32 pass must read none it did not write.
+    b.run(F.MODE_FP32_RESCUE, share=True)
+    assert np.array_equal(b.results(), shared)
+    for _ in range(2):  # graph replays give the eager bits
+        b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+        assert np.array_equal(b.results(), shared)
+    b.close()
+
+
+def _edited_sets(seed, cases):
+    rng = Rng(seed)
+    regions = []
+    for case in range(cases):
+        alphabet = b"AC" if case % 2 == 0 else b"ACGT"
+        base = bytes(alphabet[rng.below(len(alphabet))] for _ in range(30 + rng.below(150)))
+        haps = [base]
+        for _ in range(1 + rng.below(12)):
+            src = haps[rng.below(len(haps))]
+            if rng.chance(0.15):
+                haps.append(src)
+                continue
+            h = bytearray(src)
+            at = rng.below(len(h))
+            h[at:at + rng.below(4)] = bytes(alphabet[rng.below(len(alphabet))]
+                                             for _ in range(rng.below(4)))
+            haps.append(bytes(h) if h else src)
+        reads = []
+        for _ in range(1 + rng.below(12)):
+            src = haps[rng.below(len(haps))]
+            n = 1 + rng.below(len(src))
+            s0 = rng.below(len(src) - n + 1)
+            reads.append(F.ReadRef(src[s0:s0 + n], [10 + rng.below(31) for _ in range(n)],
+                                   [20 + rng.below(26) for _ in range(n)],
+                                   [20 + rng.below(26) for _ in range(n)],
+                                   [1 + rng.below(10) for _ in range(n)]))
+        regions.append((reads, haps))
+    return regions
+
+
+def test_suffix_sharing_on_edited_haplotype_sets(gpu):
+    # Random edits of a base sequence over small alphabets, with copies (as the reference's plan
+    # tests build them, plan.rs:320-340): cuts at every depth, resumed backward sweeps, duplicate
+    # haplotypes, reads of every length from 1 base, alone and two to a wave.
+    data = batch_from_regions(_edited_sets(2718, 60))
+    expected = batch_reference(data)
+    b = F.Batch(data)
+    assert b.work()["backward"] > 0
+    b.run(F.MODE_FP32_RESCUE, share=True)
+    got = b.results()
+    assert_close(got, expected, FLOAT_TOL, "edited sets, suffix sharing")
+    b.run(F.MODE_FP32_NO_RESCUE, share=True)
+    nores = b.results()
+    ok = np.isfinite(nores)
+    assert np.array_equal(nores[ok], got[ok])
+    b.close()
+
+
+def test_suffix_shared_bits_do_not_depend_on_the_other_regions(gpu):
+    regions = _edited_sets(3141, 12)
+    whole = F.Batch(batch_from_regions(regions))
+    whole.run(F.MODE_FP32_RESCUE, share=True)
+    all_bits = whole.results()
+    whole.close()
+    base = 0
+    for g in (0, 5, 11):
+        alone = F.Batch(batch_from_regions([regions[g]]))
+        alone.run(F.MODE_FP32_RESCUE, share=True)
+        bits = alone.results()
+        alone.close()
+        start = sum(len(r) * len(h) for r, h in regions[:g])
+        assert np.array_equal(bits, all_bits[start:start + len(bits)])
+        base += len(bits)
+
+
 def test_batch_graph_replays_match_eager_runs(gpu):
     data = workload(1, seed=99, n_regions=5)
     b = F.Batch(data)
