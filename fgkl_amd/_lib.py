@@ -25,6 +25,7 @@ class RunStats(C.Structure):
         ("rescued_pairs", C.c_int64),
         ("bucket_ms", C.c_double * 8),
         ("bucket_cells", C.c_int64 * 8),
+        ("bucket_work", C.c_int64 * 8),
     ]
 
 

@@ -245,6 +245,7 @@ int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_grap
     for (int i = 0; i < 8; ++i) {
       stats->bucket_ms[i] = st.bucket_ms[i];
       stats->bucket_cells[i] = st.bucket_cells[i];
+      stats->bucket_work[i] = st.bucket_work[i];
     }
   }
   return rc;

@@ -169,6 +169,7 @@ struct DeviceBatch::Impl {
   int32_t bucket_n[kMaxRowsPerLane] = {};
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
   int64_t bucket_cells[kMaxRowsPerLane] = {};
+  int64_t bucket_work[kMaxRowsPerLane] = {};
   int32_t* rescue_pairs[kMaxRowsPerLane] = {};
   BwdSweep* bsweeps = nullptr;
   int32_t* bwrites = nullptr;
@@ -407,6 +408,7 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
     }
     d.bucket_pairs[R - 1] = pairs;
     d.bucket_cells[R - 1] = p.bucket_cells[R - 1];
+    d.bucket_work[R - 1] = p.bucket_work[R - 1];
     if ((rc = d.upload(&d.bucket_items[R - 1], items.data(), items.size()))) return rc;
     if ((rc = d.alloc(&d.rescue_pairs[R - 1], size_t(2 * pairs)))) return rc;
   }
@@ -497,7 +499,10 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
     if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue)
       for (int R = 0; R < kMaxRowsPerLane; ++R) rescued += counts[16 + R];
     stats->rescued_pairs = rescued;
-    for (int R = 0; R < kMaxRowsPerLane; ++R) stats->bucket_cells[R] = d.bucket_cells[R];
+    for (int R = 0; R < kMaxRowsPerLane; ++R) {
+      stats->bucket_cells[R] = d.bucket_cells[R];
+      stats->bucket_work[R] = d.bucket_work[R];
+    }
     if (!use_graph) {
       float first = 0.f;
       HIP_TRY(hipEventElapsedTime(&first, d.ev[0], d.ev[1]));

@@ -22,6 +22,7 @@ struct RunStats {
   int64_t rescued_pairs = 0;
   double bucket_ms[kMaxRowsPerLane] = {};  // primary sweep launch of each rows-per-lane bucket
   int64_t bucket_cells[kMaxRowsPerLane] = {};  // DP cells (read x haplotype bases) it covers
+  int64_t bucket_work[kMaxRowsPerLane] = {};   // cells it computes with prefix + suffix sharing
 };
 
 // How the FP32 sweep of one rows-per-lane bucket is launched.  Every configuration computes

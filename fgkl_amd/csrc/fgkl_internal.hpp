@@ -128,6 +128,7 @@ struct HostPlan {
   std::vector<int32_t> bucket_items[kMaxRowsPerLane];
   int64_t bucket_pairs[kMaxRowsPerLane] = {};  // read x haplotype pairs of each bucket
   int64_t bucket_cells[kMaxRowsPerLane] = {};  // their read bases x haplotype bases
+  int64_t bucket_work[kMaxRowsPerLane] = {};   // cells the FP32 pass computes (both sharings)
 };
 
 // Validates the batch and builds its plan.  Returns kOk or kInvalidArgument with a message

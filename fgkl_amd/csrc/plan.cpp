@@ -507,6 +507,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       bucket.push_back(int32_t(b));
       p.bucket_pairs[R - 1] += int64_t(H) * (b >= 0 ? 2 : 1);
       p.bucket_cells[R - 1] += (na + nb) * hap_bases;
+      p.bucket_work[R - 1] += (na + nb) * cols;
       item_costs[R - 1].push_back(item_cost(R, L));
     }
   }

@@ -119,6 +119,7 @@ typedef struct fgkl_run_stats {
   int64_t rescued_pairs;  /* FP32 pairs recomputed in FP64 */
   double bucket_ms[8];    /* primary sweep launch per rows-per-lane bucket R=1..8 (eager runs) */
   int64_t bucket_cells[8]; /* DP cells (read bases x haplotype bases) each of those launches covers */
+  int64_t bucket_work[8];  /* DP cells each computes with prefix and suffix sharing (FP32 pass) */
 } fgkl_run_stats;
 
 int fgkl_batch_create(fgkl_batch** out, int pd, int64_t n_regions,
