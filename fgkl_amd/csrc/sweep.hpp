@@ -503,78 +503,119 @@ __device__ __forceinline__ void stage_bwd(const SweepArgs& a, const BwdSweep& b,
 // `from_lds` is all ones in the bottom lanes, zero elsewhere, and opaque to the optimiser: words
 // are merged with bit operations rather than a select, which the compiler could turn into a DPP
 // move run only in the lanes that keep its result (a DPP move reads a lane outside EXEC as zero).
-// Lanes past the reads (lane >= L) start from zeros: their records were never written, and the
-// upward DPP moves lane L's values into lane L-1, where a stale NaN times the zero parameters
-// below a read's last row would still be NaN.
+// The state a lane carries through a backward sweep: its rows' B_M, B_X, B_Y at the column it
+// computed last, B_M of the row below its last row one column further right, and the word of
+// the column it is on.
+template <int R> struct BLane {
+  float bm[R], bx[R], by[R];
+  float P;
+  uint32_t w;
+};
+
+// One backward column of a lane's rows, bottom-up: row k reads row k+1 at this column (insertion,
+// new values) and one column to the right (match, the old value), and itself one column to the
+// right (deletion); the row below the last row is lane l+1's first row (`qm`, `qx` this column,
+// P the previous one), with parameters `eg`.  Records a node when the word carries one.
+template <int R>
+__device__ __forceinline__ void bcells(BLane<R>& s, const Rows<float, R, false>& rw,
+                                       const EdgeRow& eg, float qm, float qx, float sink,
+                                       float* nodes, int lane) {
+  const uint32_t hb = s.w & 0x1FFu;
+  const bool hN = hb == uint32_t('N');
+  float below_x = qx, diag = s.P;
+#pragma unroll
+  for (int k = R - 1; k >= 0; --k) {
+    const bool edge = k == R - 1;
+    const int kb = edge ? k : k + 1;
+    const float pmb = edge ? eg.pm : rw.pm[kb];
+    const float pxb = edge ? eg.px : rw.px[kb];
+    const uint32_t rbb = edge ? eg.rb : rw.rb[kb];
+    const float mmb = edge ? eg.mm : rw.mm[kb];
+    const float imb = edge ? eg.im : rw.im[kb];
+    const float mib = edge ? eg.mi : rw.mi[kb];
+    const float iib = edge ? eg.gc : rw.gc[kb];
+    const bool match = hN | (hb == rbb);
+    const float into = (match ? pmb : pxb) * diag;  // kernel.rs:1071-1082
+    const float ind = into * imb;
+    float nbx = fmaf(below_x, iib, ind);
+    const float nby = fmaf(s.by[k], rw.gc[k], ind);
+    float nbm = fmaf(s.by[k], rw.md[k], fmaf(below_x, mib, into * mmb));
+    if (edge) {
+      nbm += sink;
+      nbx += sink;
+    }
+    diag = s.bm[k];  // row k one column to the right: the diagonal of row k-1
+    s.bm[k] = nbm;
+    s.bx[k] = nbx;
+    s.by[k] = nby;
+    below_x = nbx;
+  }
+  s.P = qm;
+  const uint32_t node = s.w >> kSlotShift;
+  if (node) {
+    float* r = nodes + (size_t(node - 1) * 64 + lane) * node_stride<R>();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      r[k] = s.bm[k];
+      r[R + k] = s.by[k];
+    }
+    r[2 * R] = s.P;
+  }
+}
+
+// One backward sweep of Wb columns.  A lane starts `delay` steps after its read's last lane: the
+// read's last lanes take the words from LDS (`from_lds` all ones there, zero elsewhere, opaque to
+// the optimiser: words are merged with bit operations rather than a select, which the compiler
+// could turn into a DPP move run only in the lanes that keep its result, and a DPP move reads a
+// lane outside EXEC as zero); the others take them from the lane below.  `sink` is 2^120 in a
+// read's last lane (whose last register holds the read's last row), zero elsewhere.  Starts from
+// node record `rec_in` (this lane's values at column `right`) or from zeros.  Lanes past the
+// reads (lane >= L) start from zeros: their records were never written, and the upward DPP moves
+// lane L's values into lane L-1, where a stale NaN times the zero parameters below a read's last
+// row would still be NaN.  `full` is the step from which every lane of the item is inside the
+// sweep (the largest delay).
 template <int R>
 __device__ void bsweep(const Rows<float, R, false>& rw, const EdgeRow& eg, const uint32_t* lw,
-                       int Wb, int steps, int lane, int L, int delay, uint32_t from_lds,
+                       int Wb, int steps, int full, int lane, int L, int delay, uint32_t from_lds,
                        float sink, const float* rec_in, float* nodes) {
   constexpr int NS = node_stride<R>();
-  float bm[R], bx[R], by[R];
-  float P = 0.f;  // B_M of the row below the last row, one column to the right
+  BLane<R> s;
+  s.P = 0.f;
+  s.w = 0;
   if (rec_in) {
     const float* r = rec_in + lane * NS;
     const bool live = lane < L;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      bm[k] = live ? r[k] : 0.f;
-      by[k] = live ? r[R + k] : 0.f;
-      bx[k] = 0.f;
+      s.bm[k] = live ? r[k] : 0.f;
+      s.by[k] = live ? r[R + k] : 0.f;
+      s.bx[k] = 0.f;
     }
-    P = live ? r[2 * R] : 0.f;
+    s.P = live ? r[2 * R] : 0.f;
   } else {
 #pragma unroll
-    for (int k = 0; k < R; ++k) bm[k] = bx[k] = by[k] = 0.f;
+    for (int k = 0; k < R; ++k) s.bm[k] = s.bx[k] = s.by[k] = 0.f;
   }
-  uint32_t w = 0;
-  for (int t = 0; t < steps; ++t) {
-    w = (shl1z(w) & ~from_lds) | (lw[t] & from_lds);
-    const float qm = shl1z(bm[0]), qx = shl1z(bx[0]);  // the row below, this column
-    const int c = t - delay;
-    if (c >= 0 && c < Wb) {
-      const uint32_t hb = w & 0x1FFu;
-      const bool hN = hb == uint32_t('N');
-      float below_x = qx, diag = P;
-#pragma unroll
-      for (int k = R - 1; k >= 0; --k) {
-        const bool edge = k == R - 1;
-        const float pmb = edge ? eg.pm : rw.pm[k < R - 1 ? k + 1 : k];
-        const float pxb = edge ? eg.px : rw.px[k < R - 1 ? k + 1 : k];
-        const uint32_t rbb = edge ? eg.rb : rw.rb[k < R - 1 ? k + 1 : k];
-        const float mmb = edge ? eg.mm : rw.mm[k < R - 1 ? k + 1 : k];
-        const float imb = edge ? eg.im : rw.im[k < R - 1 ? k + 1 : k];
-        const float mib = edge ? eg.mi : rw.mi[k < R - 1 ? k + 1 : k];
-        const float iib = edge ? eg.gc : rw.gc[k < R - 1 ? k + 1 : k];
-        const bool match = hN | (hb == rbb);
-        const float into = (match ? pmb : pxb) * diag;  // kernel.rs:1071-1082
-        const float ind = into * imb;
-        float nbx = fmaf(below_x, iib, ind);
-        const float nby = fmaf(by[k], rw.gc[k], ind);
-        float nbm = fmaf(by[k], rw.md[k], fmaf(below_x, mib, into * mmb));
-        if (edge) {
-          nbm += sink;
-          nbx += sink;
-        }
-        diag = bm[k];  // row k at column j+1: the diagonal of row k-1
-        bm[k] = nbm;
-        bx[k] = nbx;
-        by[k] = nby;
-        below_x = nbx;
-      }
-      P = qm;
-      const uint32_t node = w >> kSlotShift;
-      if (node) {
-        float* r = nodes + (size_t(node - 1) * 64 + lane) * NS;
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          r[k] = bm[k];
-          r[R + k] = by[k];
-        }
-        r[2 * R] = P;
-      }
-    }
+  auto step = [&](uint32_t word, bool live) {
+    s.w = (shl1z(s.w) & ~from_lds) | (word & from_lds);
+    const float qm = shl1z(s.bm[0]), qx = shl1z(s.bx[0]);  // the row below, this column
+    if (live) bcells<R>(s, rw, eg, qm, qx, sink, nodes, lane);
+  };
+  // Ramp: lanes start one after the other; rounded up to four steps so the steady loop reads
+  // aligned groups of four words.
+  const int ramp = min(steps, (full + 3) & ~3);
+  int t = 0;
+  for (; t < ramp; ++t) step(lw[t], t >= delay && t - delay < Wb);
+  // Steady state: every lane of the item is inside the sweep until its last lanes pass column
+  // `left` (they have delay 0).
+  for (; t + 4 <= Wb; t += 4) {
+    const uint4 q = *reinterpret_cast<const uint4*>(lw + t);
+    step(q.x, true);
+    step(q.y, true);
+    step(q.z, true);
+    step(q.w, true);
   }
+  for (; t < steps; ++t) step(lw[t], t >= delay && t - delay < Wb);
 }
 
 // Waves per SIMD the occupancy-raised FP32 variants ask the compiler for: one more than the
@@ -683,7 +724,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
           h.len = __builtin_amdgcn_readfirstlane(h.len);
           stage_bwd(a, b, h, L, lane, lw);
           const int Wb = b.right - b.left;
-          bsweep<R>(rw, eg, lw, Wb, Wb + steps_extra, lane, L, delay, from_lds, sink,
+          bsweep<R>(rw, eg, lw, Wb, Wb + steps_extra, steps_extra, lane, L, delay, from_lds, sink,
                     b.start_node >= 0 ? nodes + size_t(b.start_node) * 64 * node_stride<R>()
                                       : nullptr,
                     nodes);
