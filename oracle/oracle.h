@@ -97,6 +97,17 @@ int fgklo_port_batch(int64_t n_regions, const int64_t* read_region, const int64_
                      const uint8_t* del_gop, const uint8_t* gcp, const int64_t* hap_region,
                      const int64_t* hap_off, const uint8_t* hap_bases, double* out, int nthreads,
                      int force_avx2, int share_suffixes);
+/*
+ * Smith-Waterman (sw.c): GATK's SmithWatermanJavaAligner (crates/smithwaterman/src/reference.rs).
+ * params = {match, mismatch, gap open, gap extend}; strategy 0 SOFTCLIP, 1 INDEL, 2 LEADING_INDEL,
+ * 3 IGNORE.  Writes the CIGAR in alternate order as (length << 2 | op), op 0 M, 1 I, 2 D, 3 S (at
+ * most nref + nalt + 2 elements) and the reference offset.  Returns 0, or -1 for an empty
+ * sequence.
+ */
+int fgklo_sw_align(const uint8_t* ref, int64_t nref, const uint8_t* alt, int64_t nalt,
+                   const int32_t params[4], int strategy, int32_t* offset, uint32_t* cigar,
+                   int32_t* ncigar);
+
 /* The vector width fgklo_port_batch runs with on this CPU. */
 const char* fgklo_port_backend(void);
 

@@ -41,6 +41,10 @@ def oracle() -> C.CDLL:
         o.fgklo_port_batch.argtypes = [C.c_int64, i64p, i64p, u8p, u8p, u8p, u8p, u8p, i64p,
                                        i64p, u8p, f64p, C.c_int, C.c_int, C.c_int]
         o.fgklo_port_backend.restype = C.c_char_p
+        o.fgklo_sw_align.restype = C.c_int
+        o.fgklo_sw_align.argtypes = [u8p, C.c_int64, u8p, C.c_int64, C.POINTER(C.c_int32), C.c_int,
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_int32)]
         o.fgklo_base_matches_pd.restype = C.c_int
         o.fgklo_base_matches_pd.argtypes = [C.c_uint8, C.c_uint8]
         o.fgklo_pd_log10_likelihood.restype = C.c_double
@@ -121,6 +125,29 @@ def port_batch(data, nthreads: int = 8, force_avx2: bool = False,
     if rc != 0:
         raise RuntimeError("the CPU baseline needs AVX-512 or AVX2")
     return out
+
+
+SW_STRATEGIES = {"SOFTCLIP": 0, "INDEL": 1, "LEADING_INDEL": 2, "IGNORE": 3}
+
+
+def cigar_text(elems) -> str:
+    return "".join(f"{int(e) >> 2}{'MIDS'[int(e) & 3]}" for e in elems)
+
+
+def sw_align(ref: bytes, alt: bytes, params, strategy) -> tuple[str, int]:
+    """GATK's aligner on one pair (oracle/sw.c): (CIGAR, reference offset)."""
+    if isinstance(strategy, str):
+        strategy = SW_STRATEGIES[strategy]
+    cig = (C.c_uint32 * (len(ref) + len(alt) + 2))()
+    n, off = C.c_int32(), C.c_int32()
+    p = (C.c_int32 * 4)(*params)
+    rb = np.frombuffer(bytes(ref), np.uint8)
+    ab = np.frombuffer(bytes(alt), np.uint8)
+    rc = oracle().fgklo_sw_align(_p(rb), len(ref), _p(ab), len(alt), p, strategy, C.byref(off),
+                                 cig, C.byref(n))
+    if rc != 0:
+        raise ValueError("empty sequence")
+    return cigar_text(cig[: n.value]), off.value
 
 
 def port_backend() -> str:

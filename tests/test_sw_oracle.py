@@ -1,0 +1,239 @@
+"""The Smith-Waterman oracle (oracle/sw.c, GATK's SmithWatermanJavaAligner restated) pinned by the
+reference's known answers (crates/smithwaterman/src/lib.rs:928-968) and by what a CIGAR must
+consume under each overhang strategy, on related and unrelated sequences."""
+from __future__ import annotations
+
+import re
+
+import numpy as np
+import pytest
+
+from fgkl_amd.synthetic import Rng
+from _oracle import sw_align
+
+HAP_TO_REF = (200, -150, -260, -11)   # lib.rs:741
+READ_TO_HAP = (10, -15, -30, -5)      # lib.rs:742
+ORIGINAL = (3, -1, -4, -3)            # lib.rs:743
+STRATEGIES = ("SOFTCLIP", "INDEL", "LEADING_INDEL", "IGNORE")
+
+
+def test_exact_substrings_short_circuit_for_softclip_and_ignore():  # lib.rs:928-938
+    assert sw_align(b"AAACGTACGTAAA", b"ACGTACGT", HAP_TO_REF, "SOFTCLIP") == ("8M", 2)
+    assert sw_align(b"ACGTACGTACGT", b"ACGT", HAP_TO_REF, "IGNORE")[1] == 8  # last occurrence
+
+
+def test_simple_indels_follow_gatk_conventions():  # lib.rs:941-953
+    ref = b"NNNNNNNNNNACGTTTGCAAGGCTTAGGCTNNNNNNNNNN"
+    assert sw_align(ref, b"NNNNNNNNNNACGTTTGCGGCTTAGGCTNNNNNNNNNN", HAP_TO_REF, "INDEL") == ("18M2D20M", 0)
+    assert sw_align(ref, b"NNNNNNNNNNACGTTTGCAAGGCTTTTAGGCTNNNNNNNNNN", HAP_TO_REF, "INDEL") == ("23M2I17M", 0)
+
+
+def test_read_bases_beyond_the_reference_are_soft_clipped():  # lib.rs:956-967
+    read = b"GGGGACGTACGTAGGCCTTAGCAGGGG"
+    assert sw_align(b"ACGTACGTAGGCCTTAGCA", read, READ_TO_HAP, "SOFTCLIP") == ("4S19M4S", 0)
+    assert sw_align(b"TTTTTTTTTTACGTACGTAGGCCTTAGCATTTTTTTTTT", read, READ_TO_HAP, "SOFTCLIP")[0] == "4I19M4I"
+
+
+def test_empty_sequences_are_rejected():  # lib.rs:971-981
+    with pytest.raises(ValueError):
+        sw_align(b"", b"A", HAP_TO_REF, "SOFTCLIP")
+    with pytest.raises(ValueError):
+        sw_align(b"A", b"", HAP_TO_REF, "INDEL")
+
+
+def _related(rng, ref_len, alt_len):  # lib.rs:773-810, the reference's test generator
+    ref = bytes(b"ACGT"[rng.below(4)] for _ in range(ref_len))
+    start = rng.below(max(ref_len - alt_len, 0) + 1)
+    alt = bytearray(ref[start:min(start + alt_len, ref_len)])
+    for _ in range(rng.below(4)):
+        if not alt:
+            break
+        pos = rng.below(len(alt))
+        kind = rng.below(3)
+        if kind == 0:
+            alt[pos] = b"ACGT"[rng.below(4)]
+        elif kind == 1:
+            alt[pos:pos] = bytes(b"ACGT"[rng.below(4)] for _ in range(1 + rng.below(6)))
+        else:
+            end = min(pos + 1 + rng.below(6), len(alt))
+            if len(alt) > end - pos:
+                del alt[pos:end]
+    if rng.chance(0.3):
+        extra = bytes(b"ACGT"[rng.below(4)] for _ in range(1 + rng.below(20)))
+        alt = bytearray(extra) + alt if rng.chance(0.5) else alt + extra
+    return ref, bytes(alt or b"A")
+
+
+def _consumed(cigar):
+    ops = [(int(n), op) for n, op in re.findall(r"(\d+)([MIDS])", cigar)]
+    alt = sum(n for n, op in ops if op in "MIS")
+    ref = sum(n for n, op in ops if op in "MD")
+    return ops, alt, ref
+
+
+@pytest.mark.parametrize("params", [HAP_TO_REF, READ_TO_HAP, ORIGINAL])
+def test_cigars_consume_what_each_strategy_requires(params):
+    rng = Rng(7)
+    for case in range(120):
+        ref, alt = _related(rng, 40 + rng.below(120), 20 + rng.below(80)) if case % 2 == 0 \
+            else _related(rng, 5 + rng.below(30), 3 + rng.below(30))
+        for strategy in STRATEGIES:
+            cigar, offset = sw_align(ref, alt, params, strategy)
+            ops, alt_used, ref_used = _consumed(cigar)
+            assert alt_used == len(alt), (cigar, strategy)
+            assert all(n > 0 for n, _ in ops)
+            assert all(a[1] != b[1] for a, b in zip(ops, ops[1:]))  # runs are merged
+            if strategy == "INDEL":
+                assert offset == 0 and ref_used == len(ref) and "S" not in cigar
+            elif strategy == "SOFTCLIP":
+                assert all(op != "S" for _, op in ops[1:-1])
+                assert 0 <= offset and offset + ref_used <= len(ref)
+            elif strategy == "LEADING_INDEL":
+                assert offset == 0 and ref_used <= len(ref) and "S" not in cigar
+            else:  # IGNORE folds an unaligned alt prefix into the first op and moves the offset back
+                assert "S" not in cigar and offset <= len(ref)
+
+
+def _scores(ref, alt, params, strategy):
+    """The score matrix by a separate affine-gap recursion: row-wise numpy, horizontal gaps by a
+    prefix maximum (exact because opening a gap never beats extending one for these weights)."""
+    match, mismatch, gap_open, ext = params
+    assert gap_open <= ext
+    n, m = len(ref), len(alt)
+    a = np.frombuffer(alt, np.uint8)
+    border = strategy in ("INDEL", "LEADING_INDEL")
+    run = lambda k: gap_open + (k - 1) * ext  # a gap of k bases
+    H = np.zeros((n + 1, m + 1), np.int64)
+    if border:
+        H[0, 1:] = [run(k) for k in range(1, m + 1)]
+        H[1:, 0] = [run(k) for k in range(1, n + 1)]
+    low = -(1 << 40)
+    down = np.full(m + 1, low, np.int64)
+    js = np.arange(m + 1, dtype=np.int64)
+    for i in range(1, n + 1):
+        diag = H[i - 1, :-1] + np.where(a == ref[i - 1], match, mismatch)
+        down[1:] = np.maximum(down[1:] + ext, H[i - 1, 1:] + gap_open)
+        row = np.empty(m + 1, np.int64)
+        row[0] = H[i, 0]
+        row[1:] = np.maximum(diag, down[1:])
+        best = np.maximum.accumulate(row - js * ext)          # max over j' <= j
+        right = np.full(m + 1, low, np.int64)
+        right[1:] = gap_open - ext + js[1:] * ext + best[:-1]
+        H[i] = np.maximum(row, right)
+        H[i, 0] = row[0]
+    return H
+
+
+def _optimum(ref, alt, params, strategy):
+    H = _scores(ref, alt, params, strategy)
+    n, m = len(ref), len(alt)
+    if strategy == "INDEL":
+        return int(H[n, m])
+    col = int(H[1:, m].max())
+    return col if strategy == "LEADING_INDEL" else max(col, int(H[n, 1:].max()))
+
+
+def _walk(ref, alt, params, cigar, offset):
+    """The score of the path a CIGAR states, with soft clips free."""
+    match, mismatch, gap_open, ext = params
+    i, j, score = offset, 0, 0
+    for n, op in _consumed(cigar)[0]:
+        if op == "M":
+            score += sum(match if ref[i + k] == alt[j + k] else mismatch for k in range(n))
+            i, j = i + n, j + n
+        elif op == "S":
+            j += n
+        else:
+            score += gap_open + (n - 1) * ext
+            if op == "I":
+                j += n
+            else:
+                i += n
+    return score
+
+
+def _ignore_from_softclip(cigar, offset):
+    """IGNORE states the SOFTCLIP alignment with the leading clip folded into the first op (and the
+    offset moved back by it) and the trailing clip as matches."""
+    ops = [[n, op] for n, op in _consumed(cigar)[0]]
+    if ops[0][1] == "S":
+        offset -= ops[0][0]
+        ops[1][0] += ops[0][0]
+        ops.pop(0)
+    if ops[-1][1] == "S":
+        if ops[-2][1] == "M":
+            ops[-2][0] += ops[-1][0]
+            ops.pop()
+        else:
+            ops[-1][1] = "M"
+    return "".join(f"{n}{op}" for n, op in ops), offset
+
+
+@pytest.mark.parametrize("params", [HAP_TO_REF, READ_TO_HAP, ORIGINAL])
+def test_alignments_score_the_optimum(params):
+    rng = Rng(11)
+    for case in range(60):
+        ref, alt = _related(rng, 30 + rng.below(90), 15 + rng.below(60)) if case % 3 \
+            else (bytes(b"ACGT"[rng.below(4)] for _ in range(5 + rng.below(40))),
+                  bytes(b"ACGT"[rng.below(4)] for _ in range(3 + rng.below(40))))
+        for strategy in ("SOFTCLIP", "INDEL", "LEADING_INDEL"):
+            cigar, offset = sw_align(ref, alt, params, strategy)
+            assert _walk(ref, alt, params, cigar, offset) == _optimum(ref, alt, params, strategy), \
+                (case, strategy, cigar)
+        assert sw_align(ref, alt, params, "IGNORE") == \
+            _ignore_from_softclip(*sw_align(ref, alt, params, "SOFTCLIP"))
+
+
+def test_ties_prefer_the_insertion_over_the_deletion():
+    """lib.rs:1-3 orders ties diagonal, insertion, deletion.  Worked by hand for ref "A", alt "C",
+    match 1, mismatch -100, open -3, extend -1, INDEL: the borders are -3, so cell (1,1) scores
+    diagonal -100, down -3-3 = -6, right -3-3 = -6.  The tie takes the insertion; the traceback then
+    leaves the leading reference base as a deletion: 1D1I (a deletion-first rule gives 1I1D)."""
+    assert sw_align(b"A", b"C", (1, -100, -3, -1), "INDEL") == ("1D1I", 0)
+    # and the diagonal beats both gaps when it ties them: -3 + -3 = -6 everywhere at (1,1)
+    assert sw_align(b"A", b"C", (1, -6, -3, -1), "INDEL") == ("1M", 0)
+
+
+def test_a_last_row_end_must_be_nearer_the_diagonal_to_win_a_tie():
+    """Worked by hand for ref "AA", alt "CC", weights 3/-1/-4/-3, SOFTCLIP (zero borders): cells
+    (1,1), (1,2) and (2,1) score -1 on a mismatch, (2,2) scores -2.  The last column's best is (1,2);
+    the last row's (2,1) ties it at the same distance from the diagonal, so (1,2) stays: one clipped
+    alt base, then a match at offset 0 (taking the tie would give 1M1S at offset 1)."""
+    assert sw_align(b"AA", b"CC", ORIGINAL, "SOFTCLIP") == ("1S1M", 0)
+
+
+def _end(cigar, offset, n_alt):
+    ops = _consumed(cigar)[0]
+    clip = ops[-1][0] if ops[-1][1] == "S" else 0
+    return offset + _consumed(cigar)[2], n_alt - clip
+
+
+@pytest.mark.parametrize("params", [HAP_TO_REF, READ_TO_HAP, ORIGINAL])
+def test_alignments_end_where_gatk_chooses(params):
+    """Where the alignment ends, from the separately computed score matrix: LEADING_INDEL takes the
+    last row of the best score in the last column; SOFTCLIP takes that, unless a cell of the last row
+    scores more, or as much and nearer the main diagonal (GATK's calculateCigar end selection)."""
+    rng = Rng(19)
+    seen_row_ends = 0
+    for case in range(150):
+        ref, alt = _related(rng, 8 + rng.below(40), 4 + rng.below(40))
+        if case % 4 == 0:
+            alt = alt[: max(1, len(alt) // 3)] + bytes(b"ACGT"[rng.below(4)] for _ in range(3))
+        n, m = len(ref), len(alt)
+        H = _scores(ref, alt, params, "LEADING_INDEL")
+        col = H[1:, m]
+        cigar, offset = sw_align(ref, alt, params, "LEADING_INDEL")
+        assert _consumed(cigar)[2] == 1 + int(np.flatnonzero(col == col.max())[-1]), (case, cigar)
+        if ref.rfind(alt) >= 0:
+            continue
+        H = _scores(ref, alt, params, "SOFTCLIP")
+        col = H[1:, m]
+        p1, p2, best = 1 + int(np.flatnonzero(col == col.max())[-1]), m, int(col.max())
+        for j in range(1, m + 1):
+            cur = int(H[n, j])
+            if cur > best or (cur == best and abs(n - j) < abs(p1 - p2)):
+                p1, p2, best = n, j, cur
+        seen_row_ends += p2 < m
+        cigar, offset = sw_align(ref, alt, params, "SOFTCLIP")
+        assert _end(cigar, offset, m) == (p1, p2), (case, cigar, offset)
+    assert seen_row_ends > 5
