@@ -41,6 +41,16 @@ class TuneReport(C.Structure):
     ]
 
 
+class SwStats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double),
+        ("bucket_ms", C.c_double * 8),
+        ("bucket_pairs", C.c_int64 * 8),
+        ("bucket_cells", C.c_int64 * 8),
+        ("substring_pairs", C.c_int64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/fgkl.h declares.
 SIGNATURES = {
     "fgkl_last_error": (C.c_char_p, []),
@@ -101,6 +111,18 @@ SIGNATURES = {
     "fgkl_grad_backward": (C.c_int, [C.c_void_p, f64p, f64p, f64p, C.c_double, f64p, f64p, f64p,
                                      f64p, f64p, f64p, f64p]),
     "fgkl_grad_free": (None, [C.c_void_p]),
+    "fgkl_sw_backend_name": (C.c_char_p, []),
+    "fgkl_sw_align_native": (C.c_int, [u8p, C.c_int64, u8p, C.c_int64, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int32, C.c_int32, i32p, C.c_char_p,
+                                       C.c_int64]),
+    "fgkl_sw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, i64p, u8p, i64p, u8p]),
+    "fgkl_sw_batch_run": (C.c_int, [C.c_void_p, i32p, C.c_int, C.POINTER(SwStats)]),
+    "fgkl_sw_batch_results": (C.c_int, [C.c_void_p, i32p, i64p, C.POINTER(C.c_uint32),
+                                        C.c_int64]),
+    "fgkl_sw_batch_pairs": (C.c_int64, [C.c_void_p]),
+    "fgkl_sw_batch_cells": (C.c_int64, [C.c_void_p]),
+    "fgkl_sw_batch_cigar_bound": (C.c_int64, [C.c_void_p]),
+    "fgkl_sw_batch_free": (None, [C.c_void_p]),
 }
 
 _lib = None

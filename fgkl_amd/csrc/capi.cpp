@@ -12,6 +12,7 @@
 #include "fgkl_grad.hpp"
 #include "fgkl_internal.hpp"
 #include "fgkl_replay.hpp"
+#include "fgkl_sw.hpp"
 #include "fgkl_synthetic.hpp"
 
 using namespace fgkl;
@@ -35,6 +36,10 @@ struct fgkl_synth {
 
 struct fgkl_grad {
   std::unique_ptr<GradBatch> grad;
+};
+
+struct fgkl_sw_batch {
+  std::unique_ptr<SwBatch> batch;
 };
 
 namespace fgkl {
@@ -492,3 +497,110 @@ int fgkl_grad_backward(fgkl_grad* g, const double* err, const double* ins, const
 void fgkl_grad_free(fgkl_grad* g) { delete g; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Smith-Waterman
+
+namespace {
+
+// One reusable batch per thread for the single-pair entry (crates/jni/src/lib.rs:23-26).
+thread_local std::unique_ptr<SwBatch> t_sw;
+
+// Alignment::cigar_string: "<len><op>" per element.
+std::string cigar_text(const uint32_t* ops, int64_t n) {
+  static const char kOps[] = "MIDNSHP=X";
+  std::string s;
+  for (int64_t k = 0; k < n; ++k) {
+    s += std::to_string(ops[k] >> 4);
+    s += kOps[ops[k] & 15u];
+  }
+  return s;
+}
+
+}  // namespace
+
+const char* fgkl_sw_backend_name(void) { return "hip-gfx950"; }
+
+int fgkl_sw_align_native(const uint8_t* reference, int64_t reference_len,
+                         const uint8_t* alternate, int64_t alternate_len, int32_t match_value,
+                         int32_t mismatch_penalty, int32_t gap_open, int32_t gap_extend,
+                         int32_t strategy, int32_t* offset, char* cigar, int64_t cigar_cap) {
+  try {
+    if (strategy < 0 || strategy > 3)  // crates/jni/src/lib.rs:283-291
+      return fail(kInvalidArgument, "unknown overhang strategy code " + std::to_string(strategy));
+    if (reference_len < 0 || alternate_len < 0 || (reference_len > 0 && !reference) ||
+        (alternate_len > 0 && !alternate))
+      return fail(kInvalidArgument, "null or negative-length sequence");
+    if (reference_len == 0 || alternate_len == 0)  // Error::EmptySequence
+      return fail(kInvalidArgument, "sequences must be non-empty");
+    if (!offset || !cigar || cigar_cap < 1) return fail(kInvalidArgument, "no room for the result");
+    if (!device_available())
+      return fail(kInternal, "no HIP device is available: the fgkl GPU engine needs an MI355X (gfx950)");
+    const int64_t ro[2] = {0, reference_len}, ao[2] = {0, alternate_len};
+    if (!t_sw) {
+      if (int rc = SwBatch::create(1, ro, reference, ao, alternate, t_sw)) return rc;
+    } else if (int rc = t_sw->assign(1, ro, reference, ao, alternate)) {
+      return rc;
+    }
+    const int32_t params[4] = {match_value, mismatch_penalty, gap_open, gap_extend};
+    if (int rc = t_sw->run(params, strategy, nullptr)) return rc;
+    std::vector<uint32_t> ops(static_cast<size_t>(t_sw->cigar_bound()));
+    int64_t starts[2] = {0, 0};
+    int32_t off = 0;
+    if (int rc = t_sw->results(&off, starts, ops.data(), int64_t(ops.size()))) return rc;
+    const std::string text = cigar_text(ops.data(), starts[1]);
+    if (int64_t(text.size()) + 1 > cigar_cap)
+      return fail(kInvalidArgument, "CIGAR text needs " + std::to_string(text.size() + 1) + " bytes");
+    std::memcpy(cigar, text.c_str(), text.size() + 1);
+    *offset = off;
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int fgkl_sw_batch_create(fgkl_sw_batch** out, int64_t n_pairs, const int64_t* ref_offsets,
+                         const uint8_t* ref, const int64_t* alt_offsets, const uint8_t* alt) {
+  try {
+    if (!out) return fail(kInvalidArgument, "null output handle");
+    *out = nullptr;
+    if (!device_available())
+      return fail(kInternal, "no HIP device is available: the fgkl GPU engine needs an MI355X (gfx950)");
+    std::unique_ptr<fgkl_sw_batch> h(new fgkl_sw_batch);
+    if (int rc = SwBatch::create(n_pairs, ref_offsets, ref, alt_offsets, alt, h->batch)) return rc;
+    *out = h.release();
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+  }
+}
+
+int fgkl_sw_batch_run(fgkl_sw_batch* batch, const int32_t params[4], int strategy,
+                      fgkl_sw_stats* stats) {
+  if (!batch || !params) return fail(kInvalidArgument, "null batch or weights");
+  SwRunStats st;
+  if (int rc = batch->batch->run(params, strategy, stats ? &st : nullptr)) return rc;
+  if (stats) {
+    stats->total_ms = st.total_ms;
+    for (int k = 0; k < 8; ++k) {
+      stats->bucket_ms[k] = st.bucket_ms[k];
+      stats->bucket_pairs[k] = st.bucket_pairs[k];
+      stats->bucket_cells[k] = st.bucket_cells[k];
+    }
+    stats->substring_pairs = st.substring_pairs;
+  }
+  return kOk;
+}
+
+int fgkl_sw_batch_results(fgkl_sw_batch* batch, int32_t* offsets, int64_t* cigar_starts,
+                          uint32_t* cigar, int64_t cigar_cap) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  return batch->batch->results(offsets, cigar_starts, cigar, cigar_cap);
+}
+
+int64_t fgkl_sw_batch_pairs(const fgkl_sw_batch* batch) { return batch ? batch->batch->n_pairs() : 0; }
+int64_t fgkl_sw_batch_cells(const fgkl_sw_batch* batch) { return batch ? batch->batch->cells() : 0; }
+int64_t fgkl_sw_batch_cigar_bound(const fgkl_sw_batch* batch) {
+  return batch ? batch->batch->cigar_bound() : 0;
+}
+void fgkl_sw_batch_free(fgkl_sw_batch* batch) { delete batch; }

@@ -1,5 +1,6 @@
-// JNI entry points of FgklPairHmm and FgklPdHmm (crates/jni/src/lib.rs:149-254), exported from
-// libfgkl.so so the reference's Java bindings load this engine unchanged (NativeLoader.java).
+// JNI entry points of FgklPairHmm, FgklPdHmm and FgklSmithWaterman (crates/jni/src/lib.rs:149-313),
+// exported from libfgkl.so so the reference's Java bindings load this engine unchanged
+// (NativeLoader.java).
 //
 // No jni.h is available here, so the few JNI types and the function-table slots used are
 // declared below; the slot numbers are the JNI specification's fixed interface-table indices.
@@ -42,6 +43,7 @@ constexpr int kNewStringUTF = 167;
 constexpr int kGetArrayLength = 171;
 constexpr int kGetByteArrayRegion = 200;
 constexpr int kGetIntArrayRegion = 203;
+constexpr int kSetIntArrayRegion = 211;
 constexpr int kSetDoubleArrayRegion = 214;
 constexpr int kExceptionCheck = 228;
 
@@ -63,6 +65,7 @@ void throw_new(JNIEnv* env, const char* cls, const std::string& msg) {
 
 const char* const kIllegalArgument = "java/lang/IllegalArgumentException";
 const char* const kRuntimeException = "java/lang/RuntimeException";
+const char* const kIndexOutOfBounds = "java/lang/ArrayIndexOutOfBoundsException";
 
 // ReadBatch::fetch's copies (lib.rs:105-133): a null array is a JNI error.
 bool copy_bytes(JNIEnv* env, jbyteArray a, std::vector<uint8_t>& out) {
@@ -138,6 +141,43 @@ jstring backend(JNIEnv* env) {
   return fn<jstring (*)(JNIEnv*, const char*)>(env, kNewStringUTF)(env, fgkl_backend_name());
 }
 
+// crates/jni/src/lib.rs:282-312: the strategy code is checked first, then the arrays are copied,
+// the pair aligned, the offset written into offset[0] and the CIGAR returned as a string.
+jstring sw_align(JNIEnv* env, jbyteArray reference, jbyteArray alternate, jint match_value,
+                 jint mismatch, jint gap_open, jint gap_extend, jint strategy, jintArray offset) {
+  if (strategy < 0 || strategy > 3) {
+    throw_new(env, kIllegalArgument, "unknown overhang strategy code " + std::to_string(strategy));
+    return nullptr;
+  }
+  std::vector<uint8_t> r, a;
+  if (!copy_bytes(env, reference, r) || !copy_bytes(env, alternate, a)) return nullptr;
+  std::vector<char> text(2 * (r.size() + a.size()) + 64);
+  int32_t off = 0;
+  const int rc = fgkl_sw_align_native(r.data(), int64_t(r.size()), a.data(), int64_t(a.size()),
+                                      match_value, mismatch, gap_open, gap_extend, strategy, &off,
+                                      text.data(), int64_t(text.size()));
+  if (rc == FGKL_INVALID_ARGUMENT) {
+    throw_new(env, kIllegalArgument, fgkl_last_error());
+    return nullptr;
+  }
+  if (rc != FGKL_OK) {
+    throw_new(env, kRuntimeException, std::string("fgkl native code failed: ") + fgkl_last_error());
+    return nullptr;
+  }
+  if (!offset) {
+    throw_new(env, kRuntimeException, "JNI error: null array");
+    return nullptr;
+  }
+  if (fn<jsize (*)(JNIEnv*, jarray)>(env, kGetArrayLength)(env, offset) < 1) {
+    throw_new(env, kIndexOutOfBounds, "offset array is empty");
+    return nullptr;
+  }
+  fn<void (*)(JNIEnv*, jintArray, jsize, jsize, const jint*)>(env, kSetIntArrayRegion)(env, offset, 0, 1,
+                                                                                       &off);
+  if (pending(env)) return nullptr;
+  return fn<jstring (*)(JNIEnv*, const char*)>(env, kNewStringUTF)(env, text.data());
+}
+
 }  // namespace
 
 extern "C" {
@@ -180,6 +220,26 @@ Java_com_fulcrumgenomics_fgkl_pdhmm_FgklPdHmm_computeNative(
             hap_offsets, double_precision, likelihoods);
   } catch (const std::exception& e) {
     throw_new(env, kRuntimeException, std::string("fgkl native code panicked: ") + e.what());
+  }
+}
+
+// crates/jni/src/lib.rs:256-268
+__attribute__((visibility("default"))) jstring
+Java_com_fulcrumgenomics_fgkl_smithwaterman_FgklSmithWaterman_backendNative(JNIEnv* env, jclass) {
+  return fn<jstring (*)(JNIEnv*, const char*)>(env, kNewStringUTF)(env, fgkl_sw_backend_name());
+}
+
+// crates/jni/src/lib.rs:271-313
+__attribute__((visibility("default"))) jstring
+Java_com_fulcrumgenomics_fgkl_smithwaterman_FgklSmithWaterman_alignNative(
+    JNIEnv* env, jclass, jbyteArray reference, jbyteArray alternate, jint match_value,
+    jint mismatch_penalty, jint gap_open, jint gap_extend, jint strategy, jintArray offset) {
+  try {
+    return sw_align(env, reference, alternate, match_value, mismatch_penalty, gap_open, gap_extend,
+                    strategy, offset);
+  } catch (const std::exception& e) {
+    throw_new(env, kRuntimeException, std::string("fgkl native code panicked: ") + e.what());
+    return nullptr;
   }
 }
 

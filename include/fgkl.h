@@ -246,6 +246,65 @@ int fgkl_grad_backward(fgkl_grad* g, const double* err, const double* ins, const
                        double* kernel_ms);
 void fgkl_grad_free(fgkl_grad* g);
 
+/*
+ * Smith-Waterman: GATK's SmithWatermanJavaAligner (fgkl's crates/smithwaterman, restated from
+ * reference.rs:10-203) on the device.  Weights in GATK's SWParameters order {match, mismatch,
+ * gap open, gap extend} (lib.rs:41-64); overhang strategies in FgklSmithWaterman's codes
+ * (FgklSmithWaterman.java:49-57).  CIGARs are returned as BAM encodes them,
+ * (length << 4) | op with op 0 M, 1 I, 2 D, 4 S, or as text ("18M2D20M").
+ */
+#define FGKL_SW_SOFTCLIP 0
+#define FGKL_SW_INDEL 1
+#define FGKL_SW_LEADING_INDEL 2
+#define FGKL_SW_IGNORE 3
+
+/* Replaces Java_com_fulcrumgenomics_fgkl_smithwaterman_FgklSmithWaterman_backendNative
+ * (crates/jni/src/lib.rs:256-268): "hip-gfx950". */
+const char* fgkl_sw_backend_name(void);
+
+/*
+ * Replaces Java_com_fulcrumgenomics_fgkl_smithwaterman_FgklSmithWaterman_alignNative
+ * (crates/jni/src/lib.rs:271-313; Java declaration FgklSmithWaterman.java:66) and
+ * Aligner::align (crates/smithwaterman/src/lib.rs:330-360): one pair.  Writes the reference
+ * offset and the CIGAR as NUL-terminated text into cigar[0 .. cigar_cap).  An unknown strategy
+ * code or an empty sequence is FGKL_INVALID_ARGUMENT with the reference's messages
+ * (lib.rs:283-291, smithwaterman lib.rs:153); so is a CIGAR buffer too small for the text.
+ * The pair runs as a batch of one on the calling thread's reusable batch (one per thread, as
+ * the reference keeps one aligner per thread, crates/jni/src/lib.rs:23-26).
+ */
+int fgkl_sw_align_native(const uint8_t* reference, int64_t reference_len,
+                         const uint8_t* alternate, int64_t alternate_len, int32_t match_value,
+                         int32_t mismatch_penalty, int32_t gap_open, int32_t gap_extend,
+                         int32_t strategy, int32_t* offset, char* cigar, int64_t cigar_cap);
+
+/*
+ * Batched aligner (no reference counterpart: the reference aligns one pair per call on the
+ * calling thread).  Pair p aligns alternate bytes alt[alt_offsets[p] .. alt_offsets[p+1]) to
+ * ref[ref_offsets[p] .. ref_offsets[p+1]); both tables have n_pairs + 1 entries starting at 0.
+ * The pairs stay resident on the device; fgkl_sw_batch_run aligns all of them with one set of
+ * weights and one strategy and may be called again with others.  fgkl_sw_batch_results writes
+ * offsets[p] and pair p's CIGAR elements to cigar[cigar_starts[p] .. cigar_starts[p+1])
+ * (cigar_starts has n_pairs + 1 entries; cigar_cap >= fgkl_sw_batch_cigar_bound suffices).
+ */
+typedef struct fgkl_sw_batch fgkl_sw_batch;
+typedef struct fgkl_sw_stats {
+  double total_ms;           /* device time of the run, HIP events on the batch's stream */
+  double bucket_ms[8];       /* launch per rows-per-lane bucket R = 1..8 */
+  int64_t bucket_pairs[8];
+  int64_t bucket_cells[8];   /* reference x alternate bases of each bucket */
+  int64_t substring_pairs;   /* pairs resolved by the exact-substring shortcut */
+} fgkl_sw_stats;
+int fgkl_sw_batch_create(fgkl_sw_batch** out, int64_t n_pairs, const int64_t* ref_offsets,
+                         const uint8_t* ref, const int64_t* alt_offsets, const uint8_t* alt);
+int fgkl_sw_batch_run(fgkl_sw_batch* batch, const int32_t params[4], int strategy,
+                      fgkl_sw_stats* stats);
+int fgkl_sw_batch_results(fgkl_sw_batch* batch, int32_t* offsets, int64_t* cigar_starts,
+                          uint32_t* cigar, int64_t cigar_cap);
+int64_t fgkl_sw_batch_pairs(const fgkl_sw_batch* batch);
+int64_t fgkl_sw_batch_cells(const fgkl_sw_batch* batch);
+int64_t fgkl_sw_batch_cigar_bound(const fgkl_sw_batch* batch);
+void fgkl_sw_batch_free(fgkl_sw_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

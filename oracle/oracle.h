@@ -100,8 +100,8 @@ int fgklo_port_batch(int64_t n_regions, const int64_t* read_region, const int64_
 /*
  * Smith-Waterman (sw.c): GATK's SmithWatermanJavaAligner (crates/smithwaterman/src/reference.rs).
  * params = {match, mismatch, gap open, gap extend}; strategy 0 SOFTCLIP, 1 INDEL, 2 LEADING_INDEL,
- * 3 IGNORE.  Writes the CIGAR in alternate order as (length << 2 | op), op 0 M, 1 I, 2 D, 3 S (at
- * most nref + nalt + 2 elements) and the reference offset.  Returns 0, or -1 for an empty
+ * 3 IGNORE.  Writes the CIGAR as BAM encodes it, (length << 4 | op) with op 0 M, 1 I, 2 D, 4 S (at
+ * most nref + nalt + 2 elements), and the reference offset.  Returns 0, or -1 for an empty
  * sequence.
  */
 int fgklo_sw_align(const uint8_t* ref, int64_t nref, const uint8_t* alt, int64_t nalt,

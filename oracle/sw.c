@@ -15,7 +15,7 @@
 #define MATRIX_MIN_CUTOFF (-100000000)  /* lib.rs:20 */
 #define LOW_INIT_VALUE (INT_MIN / 2)    /* lib.rs:22 */
 
-enum { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 3 };
+enum { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4 };  /* BAM's operation codes ("MIDNSHP=X") */
 enum { SOFTCLIP = 0, INDEL = 1, LEADING_INDEL = 2, IGNORE = 3 };
 
 /* The last start of `alt` inside `ref`, or -1 (lib.rs last_index_of). */
@@ -90,7 +90,7 @@ static void calculate_matrix(const uint8_t* ref, int64_t nref, const uint8_t* al
     free(best_gap_v); free(gap_size_v); free(best_gap_h); free(gap_size_h);
 }
 
-static void push(uint32_t* out, int* n, uint32_t len, int op) { out[(*n)++] = (len << 2) | (uint32_t)op; }
+static void push(uint32_t* out, int* n, uint32_t len, int op) { out[(*n)++] = (len << 4) | (uint32_t)op; }
 
 /* reference.rs:102-203 */
 static int32_t calculate_cigar(const int32_t* sw, const int32_t* bt, int64_t n, int64_t m,
@@ -189,7 +189,7 @@ int fgklo_sw_align(const uint8_t* ref, int64_t nref, const uint8_t* alt, int64_t
     if (strategy == SOFTCLIP || strategy == IGNORE) {
         const int64_t s = last_index_of(ref, nref, alt, nalt);
         if (s >= 0) {
-            cigar[0] = ((uint32_t)nalt << 2) | OP_M;
+            cigar[0] = ((uint32_t)nalt << 4) | OP_M;
             *ncigar = 1;
             *offset = (int32_t)s;
             return 0;

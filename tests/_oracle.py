@@ -131,7 +131,7 @@ SW_STRATEGIES = {"SOFTCLIP": 0, "INDEL": 1, "LEADING_INDEL": 2, "IGNORE": 3}
 
 
 def cigar_text(elems) -> str:
-    return "".join(f"{int(e) >> 2}{'MIDS'[int(e) & 3]}" for e in elems)
+    return "".join(f"{int(e) >> 4}{'MIDNSHP=X'[int(e) & 15]}" for e in elems)
 
 
 def sw_align(ref: bytes, alt: bytes, params, strategy) -> tuple[str, int]:

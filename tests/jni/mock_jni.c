@@ -2,7 +2,8 @@
  * A mock JNIEnv for testing libfgkl.so's JNI entry points without a JVM (test infrastructure).
  * Java arrays are {kind, length, data} records; the interface table fills the slots the shim
  * uses (FindClass 6, ThrowNew 14, NewStringUTF 167, GetArrayLength 171, GetByteArrayRegion 200,
- * GetIntArrayRegion 203, SetDoubleArrayRegion 214, ExceptionCheck 228) and records exceptions.
+ * GetIntArrayRegion 203, SetIntArrayRegion 211, SetDoubleArrayRegion 214, ExceptionCheck 228) and
+ * records exceptions.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -12,7 +13,7 @@ typedef struct { int kind; int32_t len; void* data; } jarr;
 typedef struct { void* slots[233]; } table_t;
 
 static int g_pending;
-static char g_class[256], g_msg[2048], g_str[256];
+static char g_class[256], g_msg[2048], g_str[4096];
 static int g_set_calls;
 
 static uint8_t ExceptionCheck(void* env) { (void)env; return (uint8_t)g_pending; }
@@ -35,6 +36,9 @@ static void GetByteArrayRegion(void* env, void* a, int32_t s, int32_t n, int8_t*
 static void GetIntArrayRegion(void* env, void* a, int32_t s, int32_t n, int32_t* buf) {
   (void)env; memcpy(buf, (int32_t*)((jarr*)a)->data + s, sizeof(int32_t) * (size_t)n);
 }
+static void SetIntArrayRegion(void* env, void* a, int32_t s, int32_t n, const int32_t* buf) {
+  (void)env; g_set_calls++; memcpy((int32_t*)((jarr*)a)->data + s, buf, sizeof(int32_t) * (size_t)n);
+}
 static void SetDoubleArrayRegion(void* env, void* a, int32_t s, int32_t n, const double* buf) {
   (void)env; g_set_calls++; memcpy((double*)((jarr*)a)->data + s, buf, sizeof(double) * (size_t)n);
 }
@@ -50,6 +54,7 @@ static void init(void) {
   g_table.slots[171] = (void*)GetArrayLength;
   g_table.slots[200] = (void*)GetByteArrayRegion;
   g_table.slots[203] = (void*)GetIntArrayRegion;
+  g_table.slots[211] = (void*)SetIntArrayRegion;
   g_table.slots[214] = (void*)SetDoubleArrayRegion;
   g_table.slots[228] = (void*)ExceptionCheck;
   g_pending = 0; g_class[0] = 0; g_msg[0] = 0; g_str[0] = 0; g_set_calls = 0;
@@ -80,6 +85,24 @@ int mock_compute(void* fnp, int pd, int8_t* bases, int32_t nb, int8_t* quals, in
     ((compute_fn)fnp)(&g_env, NULL, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[8], (uint8_t)dp,
                       p[9]);
   return g_pending;
+}
+
+typedef void* (*sw_align_fn)(const void*, void*, void*, void*, int32_t, int32_t, int32_t, int32_t,
+                             int32_t, void*);
+
+/* Calls FgklSmithWaterman.alignNative with Java-like arrays (`nulls` bit 0 reference, 1
+ * alternate, 2 offset; n_offset is the offset array's length).  Returns the CIGAR string or NULL
+ * (then an exception is pending). */
+const char* mock_sw_align(void* fnp, int8_t* ref, int32_t nref, int8_t* alt, int32_t nalt,
+                          const int32_t* params, int32_t strategy, int32_t* offset,
+                          int32_t n_offset, int nulls) {
+  init();
+  jarr a[3] = {{1, nref, ref}, {1, nalt, alt}, {2, n_offset, offset}};
+  void* p[3];
+  for (int i = 0; i < 3; ++i) p[i] = (nulls >> i) & 1 ? NULL : (void*)&a[i];
+  void* s = ((sw_align_fn)fnp)(&g_env, NULL, p[0], p[1], params[0], params[1], params[2],
+                               params[3], strategy, p[2]);
+  return s ? (const char*)s : NULL;
 }
 
 const char* mock_backend(void* fnp) {

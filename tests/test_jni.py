@@ -1,5 +1,6 @@
 """The JNI entry points of libfgkl.so through a mock JNIEnv (tests/jni/mock_jni.c): the
-contract of crates/jni/src/lib.rs:149-254 and FgklPairHmmTest.java:124-190 without a JVM."""
+contract of crates/jni/src/lib.rs:149-313, FgklPairHmmTest.java:124-190 and
+FgklSmithWatermanTest.java:73-122 without a JVM."""
 from __future__ import annotations
 
 import ctypes as C
@@ -17,6 +18,8 @@ from _oracle import pd_reference_all, reference_all
 HERE = Path(__file__).resolve().parent
 PAIR = "Java_com_fulcrumgenomics_fgkl_pairhmm_FgklPairHmm_computeNative"
 PD = "Java_com_fulcrumgenomics_fgkl_pdhmm_FgklPdHmm_computeNative"
+SW = "Java_com_fulcrumgenomics_fgkl_smithwaterman_FgklSmithWaterman_alignNative"
+SW_BACKEND = "Java_com_fulcrumgenomics_fgkl_smithwaterman_FgklSmithWaterman_backendNative"
 
 
 @pytest.fixture(scope="module")
@@ -26,6 +29,7 @@ def mock(tmp_path_factory):
                    check=True)
     m = C.CDLL(str(out))
     m.mock_compute.restype = C.c_int
+    m.mock_sw_align.restype = C.c_char_p
     m.mock_backend.restype = C.c_char_p
     m.mock_exception_class.restype = C.c_char_p
     m.mock_exception_message.restype = C.c_char_p
@@ -124,3 +128,66 @@ def test_jni_entries_match_the_oracle(gpu, mock):  # FgklPairHmmTest.java:129-14
     out, exc = call(mock, pdd, pd=True, dp=True)
     assert exc is None
     assert np.all(np.abs(out - expected) <= 1e-12 * np.maximum(1.0, np.abs(expected)))
+
+
+def sw_call(m, ref: bytes, alt: bytes, params=(200, -150, -260, -11), strategy=1, n_offset=1,
+            nulls=0):
+    r = np.frombuffer(ref, np.int8).copy() if ref else np.zeros(0, np.int8)
+    a = np.frombuffer(alt, np.int8).copy() if alt else np.zeros(0, np.int8)
+    w = (C.c_int32 * 4)(*params)
+    off = np.full(max(n_offset, 1), -9, np.int32)
+    res = m.mock_sw_align(entry(SW), p8(r), len(r), p8(a), len(a), w, strategy,
+                          off.ctypes.data_as(C.POINTER(C.c_int32)), n_offset, nulls)
+    exc = (m.mock_exception_class().decode(), m.mock_exception_message().decode())
+    return (res.decode() if res is not None else None), int(off[0]), (exc if exc[1] else None)
+
+
+def test_sw_backend_names_the_engine(mock):
+    assert mock.mock_backend(entry(SW_BACKEND)).decode() == "hip-gfx950"
+
+
+def test_sw_bad_strategy_and_empty_sequences_throw_illegal_argument(mock):
+    # crates/jni/src/lib.rs:283-291; FgklSmithWatermanTest.java:113-122
+    cig, _, exc = sw_call(mock, b"ACGT", b"ACGT", strategy=4)
+    assert cig is None and exc == ("java/lang/IllegalArgumentException",
+                                   "unknown overhang strategy code 4")
+    cig, _, exc = sw_call(mock, b"", b"ACGT")
+    assert cig is None and exc[0] == "java/lang/IllegalArgumentException"
+    assert "non-empty" in exc[1]
+    cig, _, exc = sw_call(mock, b"ACGT", b"")
+    assert cig is None and exc[0] == "java/lang/IllegalArgumentException"
+
+
+def test_sw_null_arrays_are_a_jni_error(mock):
+    for bit in (0, 1):
+        cig, _, exc = sw_call(mock, b"ACGT", b"ACGT", nulls=1 << bit)
+        assert cig is None and exc == ("java/lang/RuntimeException", "JNI error: null array")
+
+
+@pytest.mark.skipif(F.device_count() > 0, reason="checks the no-GPU host")
+def test_sw_without_a_gpu_throws_runtime_exception(mock):
+    cig, _, exc = sw_call(mock, b"ACGTTT", b"ACGT")
+    assert cig is None and exc[0] == "java/lang/RuntimeException" and "no HIP device" in exc[1]
+
+
+@pytest.mark.gpu
+def test_sw_jni_entry_matches_the_oracle(gpu, mock):  # FgklSmithWatermanTest.java:82-111
+    from _oracle import sw_align
+    from _sw_cases import HAP_TO_REF, READ_TO_HAP, STRATEGIES, related_pair
+    from fgkl_amd.synthetic import Rng
+
+    assert sw_call(mock, b"AAACGTACGTAAA", b"ACGTACGT", strategy=0)[:2] == ("8M", 2)
+    assert sw_call(mock, b"ACGTACGTAGGCCTTAGCA", b"GGGGACGTACGTAGGCCTTAGCAGGGG",
+                   params=READ_TO_HAP, strategy=0)[:2] == ("4S19M4S", 0)
+    rng = Rng(17)
+    for i in range(40):
+        ref, alt = (related_pair(rng, 40 + rng.below(300), 20 + rng.below(200)) if i % 2 == 0
+                    else related_pair(rng, 5 + rng.below(30), 3 + rng.below(30)))
+        for params in (HAP_TO_REF, READ_TO_HAP):
+            for code, strategy in enumerate(STRATEGIES):
+                cig, off, exc = sw_call(mock, ref, alt, params=params, strategy=code)
+                assert exc is None
+                assert (cig, off) == sw_align(ref, alt, params, strategy), (i, strategy)
+    # the offset array must hold the offset (the JVM throws on an empty one)
+    cig, _, exc = sw_call(mock, b"ACGTTT", b"ACGT", n_offset=0)
+    assert cig is None and exc[0] == "java/lang/ArrayIndexOutOfBoundsException"

@@ -10,11 +10,7 @@ import pytest
 
 from fgkl_amd.synthetic import Rng
 from _oracle import sw_align
-
-HAP_TO_REF = (200, -150, -260, -11)   # lib.rs:741
-READ_TO_HAP = (10, -15, -30, -5)      # lib.rs:742
-ORIGINAL = (3, -1, -4, -3)            # lib.rs:743
-STRATEGIES = ("SOFTCLIP", "INDEL", "LEADING_INDEL", "IGNORE")
+from _sw_cases import HAP_TO_REF, ORIGINAL, READ_TO_HAP, STRATEGIES, related_pair
 
 
 def test_exact_substrings_short_circuit_for_softclip_and_ignore():  # lib.rs:928-938
@@ -41,29 +37,6 @@ def test_empty_sequences_are_rejected():  # lib.rs:971-981
         sw_align(b"A", b"", HAP_TO_REF, "INDEL")
 
 
-def _related(rng, ref_len, alt_len):  # lib.rs:773-810, the reference's test generator
-    ref = bytes(b"ACGT"[rng.below(4)] for _ in range(ref_len))
-    start = rng.below(max(ref_len - alt_len, 0) + 1)
-    alt = bytearray(ref[start:min(start + alt_len, ref_len)])
-    for _ in range(rng.below(4)):
-        if not alt:
-            break
-        pos = rng.below(len(alt))
-        kind = rng.below(3)
-        if kind == 0:
-            alt[pos] = b"ACGT"[rng.below(4)]
-        elif kind == 1:
-            alt[pos:pos] = bytes(b"ACGT"[rng.below(4)] for _ in range(1 + rng.below(6)))
-        else:
-            end = min(pos + 1 + rng.below(6), len(alt))
-            if len(alt) > end - pos:
-                del alt[pos:end]
-    if rng.chance(0.3):
-        extra = bytes(b"ACGT"[rng.below(4)] for _ in range(1 + rng.below(20)))
-        alt = bytearray(extra) + alt if rng.chance(0.5) else alt + extra
-    return ref, bytes(alt or b"A")
-
-
 def _consumed(cigar):
     ops = [(int(n), op) for n, op in re.findall(r"(\d+)([MIDS])", cigar)]
     alt = sum(n for n, op in ops if op in "MIS")
@@ -75,8 +48,8 @@ def _consumed(cigar):
 def test_cigars_consume_what_each_strategy_requires(params):
     rng = Rng(7)
     for case in range(120):
-        ref, alt = _related(rng, 40 + rng.below(120), 20 + rng.below(80)) if case % 2 == 0 \
-            else _related(rng, 5 + rng.below(30), 3 + rng.below(30))
+        ref, alt = related_pair(rng, 40 + rng.below(120), 20 + rng.below(80)) if case % 2 == 0 \
+            else related_pair(rng, 5 + rng.below(30), 3 + rng.below(30))
         for strategy in STRATEGIES:
             cigar, offset = sw_align(ref, alt, params, strategy)
             ops, alt_used, ref_used = _consumed(cigar)
@@ -173,7 +146,7 @@ def _ignore_from_softclip(cigar, offset):
 def test_alignments_score_the_optimum(params):
     rng = Rng(11)
     for case in range(60):
-        ref, alt = _related(rng, 30 + rng.below(90), 15 + rng.below(60)) if case % 3 \
+        ref, alt = related_pair(rng, 30 + rng.below(90), 15 + rng.below(60)) if case % 3 \
             else (bytes(b"ACGT"[rng.below(4)] for _ in range(5 + rng.below(40))),
                   bytes(b"ACGT"[rng.below(4)] for _ in range(3 + rng.below(40))))
         for strategy in ("SOFTCLIP", "INDEL", "LEADING_INDEL"):
@@ -216,7 +189,7 @@ def test_alignments_end_where_gatk_chooses(params):
     rng = Rng(19)
     seen_row_ends = 0
     for case in range(150):
-        ref, alt = _related(rng, 8 + rng.below(40), 4 + rng.below(40))
+        ref, alt = related_pair(rng, 8 + rng.below(40), 4 + rng.below(40))
         if case % 4 == 0:
             alt = alt[: max(1, len(alt) // 3)] + bytes(b"ACGT"[rng.below(4)] for _ in range(3))
         n, m = len(ref), len(alt)
