@@ -209,6 +209,94 @@ def side_bench(args, config: int) -> dict:
     }
 
 
+SW_PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, 32-bit lane ops
+SW_VALU_PER_CELL = 24.4  # VALU instructions per cell of the 8-rows-per-lane steady-state step
+
+
+def sw_bench(args) -> dict:
+    """Smith-Waterman: the reference's aligner benchmark workload (sw-bench, bench.rs:41-53: pairs
+    of a 500-base reference and its first 500 bases with three substitutions and a deletion,
+    haplotype-to-reference weights, INDEL strategy), here as batches resident on the device,
+    timed with HIP events; a larger batch beside it; parity of every pair against the SIMD CPU
+    baseline and of a sample against the scalar oracle; the baseline timed on the host cores over
+    the larger batch."""
+    import fgkl_amd.sw as S
+
+    params, strategy = S.NEW_SW_PARAMETERS, S.OverhangStrategy.INDEL
+    out = {"workload": "sw-bench: 500-base references, alternates = first 500 bases with 3 "
+                       "substitutions and 1 deletion (499 bases), weights 200/-150/-260/-11, "
+                       "INDEL", "dtype": "int32"}
+    results = {}
+    for label, n_pairs in (("pairs_2000", 2000), ("pairs_large", args.sw_large_pairs)):
+        ro, ref, ao, alt = S.bench_pairs(n_pairs, 500, 500, seed=5)
+        batch = S.SwBatch(ref=ref, ref_offsets=ro, alt=alt, alt_offsets=ao)
+        cells = batch.cells
+        for _ in range(max(args.warmup, 1)):
+            batch.run(params, strategy)
+        dev, t0 = [], time.perf_counter()
+        for _ in range(args.steps):
+            dev.append(batch.run(params, strategy).total_ms)
+        wall = (time.perf_counter() - t0) / args.steps
+        offs, starts, cig = batch.results_arrays()
+        results[label] = (ro, ref, ao, alt, offs, starts, cig)
+        ms = min(dev)
+        out[label] = {"pairs": n_pairs, "cells": cells, "device_ms": round(ms, 3),
+                      "gcups": round(cells / (ms * 1e-3) / 1e9, 2),
+                      "gcups_wall": round(cells / wall / 1e9, 2)}
+        log(f"smith-waterman {n_pairs} pairs: {ms:.2f} ms device, "
+            f"{cells / (ms * 1e-3) / 1e9:.1f} GCUPS")
+        del batch
+    best = max(out["pairs_2000"]["gcups"], out["pairs_large"]["gcups"])
+    achieved = best * 1e9 * SW_VALU_PER_CELL / 1e12  # lane ops: one cell per lane per row
+    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": SW_PEAK_INT32_TOPS,
+                       "unit": "Tops/s (32-bit lane ops)", "frac": round(achieved / SW_PEAK_INT32_TOPS, 4),
+                       "valu_per_cell": SW_VALU_PER_CELL}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from _oracle import cigar_text, sw_align, sw_port_backend, sw_port_raw
+
+        try:
+            threads = min(len(os.sched_getaffinity(0)), 16)
+        except AttributeError:
+            threads = min(os.cpu_count() or 1, 16)
+        ro, ref, ao, alt, offs, starts, cig = results["pairs_large"]
+        times, got = [], None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            got = sw_port_raw(ro, ref, ao, alt, params.as_array().tolist(), int(strategy),
+                              nthreads=threads)
+            times.append(time.perf_counter() - t0)
+        dt = min(times)
+        p_offs, p_start, p_cig, p_n = got
+        mismatch = 0
+        for p in range(len(ro) - 1):
+            a = cig[starts[p]:starts[p + 1]]
+            b = p_cig[p_start[p]:p_start[p] + p_n[p]]
+            if offs[p] != p_offs[p] or len(a) != len(b) or not np.array_equal(a, b):
+                mismatch += 1
+        sample = 16
+        oracle_mismatch = 0
+        for p in range(sample):
+            want = sw_align(bytes(ref[ro[p]:ro[p + 1]]), bytes(alt[ao[p]:ao[p + 1]]),
+                            params.as_array().tolist(), "INDEL")
+            if (cigar_text(cig[starts[p]:starts[p + 1]]), int(offs[p])) != want:
+                oracle_mismatch += 1
+        cells = out["pairs_large"]["cells"]
+        out["parity"] = {"pairs_differing_from_cpu_baseline": mismatch,
+                         "oracle_sample_pairs": sample, "pairs_differing_from_oracle": oracle_mismatch}
+        out["cpu_baseline"] = {
+            "value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": f"the {len(ro) - 1} pairs of the larger batch (sw-bench shapes), best of 3",
+            "implementation": f"GATK's aligner vectorised across pairs in C ({sw_port_backend()}), "
+                              "one trace byte per cell, traceback per pair; fgkl's anti-diagonal "
+                              "16-bit SIMD backend is Rust and not built here",
+            "cpu_model": cpu_model(), "seconds": [round(t, 3) for t in times]}
+        log(f"smith-waterman cpu baseline ({sw_port_backend()}, {threads} threads): "
+            f"{cells / dt / 1e9:.2f} GCUPS; pairs differing from it: {mismatch}, "
+            f"from the oracle sample: {oracle_mismatch}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +317,10 @@ def main() -> int:
                     help="set these bucket launch configurations instead of tuning")
     ap.add_argument("--no-side", action="store_true",
                     help="skip the rescue-stress and PD workloads")
+    ap.add_argument("--no-sw", action="store_true", help="skip the Smith-Waterman workload")
+    ap.add_argument("--sw-only", action="store_true",
+                    help="run only the Smith-Waterman workload and print its JSON")
+    ap.add_argument("--sw-large-pairs", type=int, default=16384)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -247,6 +339,9 @@ def main() -> int:
               "path shards by region with no exchange (see DESIGN.md); run with --gpus 1",
               file=sys.stderr)
         return 2
+    if args.sw_only:
+        print(json.dumps({"smith_waterman": sw_bench(args)}), flush=True)
+        return 0
 
     import fgkl_amd as F
     from fgkl_amd.synthetic import workload
@@ -394,6 +489,9 @@ def main() -> int:
     if not args.no_side:
         for config in SIDE_WORKLOADS:
             result[SIDE_WORKLOADS[config][0]] = side_bench(args, config)
+
+    if not args.no_sw:
+        result["smith_waterman"] = sw_bench(args)
 
     if not args.no_cpu:
         cb = cpu_baseline(data, args, fp32_res)

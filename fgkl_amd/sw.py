@@ -171,3 +171,28 @@ class SwBatch:
     def align(self, params, strategy) -> list[Alignment]:
         self.run(params, strategy)
         return self.results()
+
+
+def bench_pairs(n_pairs: int = 2000, ref_len: int = 500, alt_len: int = 500, seed: int = 5):
+    """The aligner benchmark's synthetic pairs (crates/smithwaterman/src/bin/bench.rs:41-53, the same
+    SplitMix64 stream): a random reference of ref_len bases and its first alt_len bases with three
+    substitutions and one deletion.  Returns (ref_offsets, ref, alt_offsets, alt) arrays."""
+    from .synthetic import splitmix_array
+
+    k = min(alt_len, ref_len)
+    per = ref_len + 7
+    u = splitmix_array(seed, n_pairs * per).reshape(n_pairs, per)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    ref = acgt[(u[:, :ref_len] % np.uint64(4)).astype(np.int64)]
+    alt = ref[:, :k].copy()
+    rows = np.arange(n_pairs)
+    for e in range(3):
+        pos = (u[:, ref_len + 2 * e] % np.uint64(k)).astype(np.int64)
+        alt[rows, pos] = acgt[(u[:, ref_len + 2 * e + 1] % np.uint64(4)).astype(np.int64)]
+    drop = (u[:, ref_len + 6] % np.uint64(k)).astype(np.int64)
+    keep = np.ones_like(alt, dtype=bool)
+    keep[rows, drop] = False
+    alt = alt[keep].reshape(n_pairs, k - 1)
+    ro = np.arange(n_pairs + 1, dtype=np.int64) * ref_len
+    ao = np.arange(n_pairs + 1, dtype=np.int64) * (k - 1)
+    return ro, ref.reshape(-1), ao, alt.reshape(-1)

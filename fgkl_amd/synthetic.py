@@ -187,6 +187,13 @@ def workload(config: int, seed: int = 7, n_regions: int = 0) -> BatchData:
     return _from_handle(lib().fgkl_synth_workload(config, seed, n), config == 3)
 
 
+def splitmix_array(seed: int, n: int) -> np.ndarray:
+    """The first n values of SplitMix64 from `seed`, as uint64."""
+    out = np.zeros(n, dtype=np.uint64)
+    lib().fgkl_synth_splitmix(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
 def splitmix(seed: int, n: int) -> list[int]:
     out = np.zeros(n, dtype=np.uint64)
     lib().fgkl_synth_splitmix(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))

@@ -108,6 +108,18 @@ int fgklo_sw_align(const uint8_t* ref, int64_t nref, const uint8_t* alt, int64_t
                    const int32_t params[4], int strategy, int32_t* offset, uint32_t* cigar,
                    int32_t* ncigar);
 
+/*
+ * Lane-batched SIMD Smith-Waterman (sw_port.c, the CPU baseline): n_pairs pairs as offset tables
+ * into ref / alt, one weight set and strategy; pair p's CIGAR goes to cigar[cig_start[p] ..] (room
+ * for n + m + 4 elements), its length to ncig[p], its offset to offsets[p].  Returns 0, -1 without
+ * AVX2, -2 for an empty sequence.
+ */
+int fgklo_sw_port_batch(int64_t n_pairs, const int64_t* ref_off, const uint8_t* ref,
+                        const int64_t* alt_off, const uint8_t* alt, const int32_t params[4],
+                        int strategy, int32_t* offsets, const int64_t* cig_start, uint32_t* cigar,
+                        int32_t* ncig, int nthreads, int force_avx2);
+const char* fgklo_sw_port_backend(void);
+
 /* The vector width fgklo_port_batch runs with on this CPU. */
 const char* fgklo_port_backend(void);
 

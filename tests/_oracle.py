@@ -45,6 +45,12 @@ def oracle() -> C.CDLL:
         o.fgklo_sw_align.argtypes = [u8p, C.c_int64, u8p, C.c_int64, C.POINTER(C.c_int32), C.c_int,
                                      C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_int32)]
+        o.fgklo_sw_port_batch.restype = C.c_int
+        o.fgklo_sw_port_batch.argtypes = [C.c_int64, i64p, u8p, i64p, u8p, C.POINTER(C.c_int32),
+                                          C.c_int, C.POINTER(C.c_int32), i64p,
+                                          C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.c_int,
+                                          C.c_int]
+        o.fgklo_sw_port_backend.restype = C.c_char_p
         o.fgklo_base_matches_pd.restype = C.c_int
         o.fgklo_base_matches_pd.argtypes = [C.c_uint8, C.c_uint8]
         o.fgklo_pd_log10_likelihood.restype = C.c_double
@@ -148,6 +154,50 @@ def sw_align(ref: bytes, alt: bytes, params, strategy) -> tuple[str, int]:
     if rc != 0:
         raise ValueError("empty sequence")
     return cigar_text(cig[: n.value]), off.value
+
+
+def sw_arrays(pairs):
+    """Offset tables and concatenated bytes of (reference, alternate) pairs."""
+    ro = np.zeros(len(pairs) + 1, np.int64)
+    ao = np.zeros(len(pairs) + 1, np.int64)
+    ro[1:] = np.cumsum([len(r) for r, _ in pairs])
+    ao[1:] = np.cumsum([len(a) for _, a in pairs])
+    ref = np.frombuffer(b"".join(r for r, _ in pairs), np.uint8).copy()
+    alt = np.frombuffer(b"".join(a for _, a in pairs), np.uint8).copy()
+    return ro, ref, ao, alt
+
+
+def sw_port_raw(ro, ref, ao, alt, params, strategy, nthreads=1, force_avx2=False):
+    """The lane-batched SIMD baseline (oracle/sw_port.c) on arrays: (offsets, cig_start, cigar,
+    ncig)."""
+    if isinstance(strategy, str):
+        strategy = SW_STRATEGIES[strategy]
+    n = len(ro) - 1
+    caps = (ro[1:] - ro[:-1]) + (ao[1:] - ao[:-1]) + 4
+    start = np.zeros(n + 1, np.int64)
+    start[1:] = np.cumsum(caps)
+    cig = np.zeros(max(int(start[-1]), 1), np.uint32)
+    ncig = np.zeros(max(n, 1), np.int32)
+    offs = np.zeros(max(n, 1), np.int32)
+    w = (C.c_int32 * 4)(*params)
+    rc = oracle().fgklo_sw_port_batch(n, _p(ro, C.c_int64), _p(ref), _p(ao, C.c_int64), _p(alt),
+                                      w, strategy, _p(offs, C.c_int32), _p(start, C.c_int64),
+                                      _p(cig, C.c_uint32), _p(ncig, C.c_int32), nthreads,
+                                      int(force_avx2))
+    if rc == -1:
+        raise RuntimeError("the Smith-Waterman baseline needs AVX2")
+    if rc == -2:
+        raise ValueError("empty sequence")
+    return offs[:n], start, cig, ncig[:n]
+
+
+def sw_port(pairs, params, strategy, nthreads=1, force_avx2=False) -> list[tuple[str, int]]:
+    offs, start, cig, ncig = sw_port_raw(*sw_arrays(pairs), params, strategy, nthreads, force_avx2)
+    return [(cigar_text(cig[start[p]:start[p] + ncig[p]]), int(offs[p])) for p in range(len(pairs))]
+
+
+def sw_port_backend() -> str:
+    return oracle().fgklo_sw_port_backend().decode()
 
 
 def port_backend() -> str:

@@ -210,3 +210,25 @@ def test_alignments_end_where_gatk_chooses(params):
         cigar, offset = sw_align(ref, alt, params, "SOFTCLIP")
         assert _end(cigar, offset, m) == (p1, p2), (case, cigar, offset)
     assert seen_row_ends > 5
+
+
+@pytest.mark.parametrize("force_avx2", [False, True], ids=["native", "avx2"])
+def test_the_simd_baseline_equals_the_oracle(force_avx2):
+    """oracle/sw_port.c (the aligner's CPU baseline): lanes of different sizes in one group, every
+    strategy and weight set, several threads."""
+    from _oracle import sw_port
+    from _sw_cases import random_seq
+
+    rng = Rng(61)
+    pairs = []
+    for case in range(90):
+        if case % 3 == 0:
+            pairs.append((random_seq(rng, 5 + rng.below(60)), random_seq(rng, 5 + rng.below(60))))
+        else:
+            pairs.append(related_pair(rng, 10 + rng.below(150), 5 + rng.below(100)))
+    pairs.append((b"AAACGTACGTAAA", b"ACGTACGT"))
+    for params in (HAP_TO_REF, READ_TO_HAP, ORIGINAL):
+        for strategy in STRATEGIES:
+            got = sw_port(pairs, params, strategy, nthreads=3, force_avx2=force_avx2)
+            for k, (ref, alt) in enumerate(pairs):
+                assert got[k] == sw_align(ref, alt, params, strategy), (k, strategy, params)
