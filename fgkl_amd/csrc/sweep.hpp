@@ -310,17 +310,25 @@ __device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Ce
 constexpr uint32_t kSlotShift = 20;
 constexpr int kMaxSlots = 4095;
 
-// One sweep of a read over haplotype columns start+1 .. start+W.  `lw[t]` holds the word of
-// column start+1+t (zero beyond W, 16-byte aligned).  Leaves in `s` each lane's state; s.acc is
-// the raw scaled sum over the read's last row (meaningful in lane L-1).  `cap` receives lane
-// cap_lane's sum right after step cap_t: the first read of a two-read wave finishes column W
-// there, and its last lane would go on summing the columns past W while the second read's lanes
-// finish.  With `freeze`, a lane stops once it has computed column W instead, so that every
-// lane ends holding its rows of column W (a cut haplotype joins from there) and cap is unused.
+// The column word a lane works on next: lane l-1's word moved down with DPP, or, in the first
+// lane of each read, the word from LDS (lane 0 gets it as the move's old operand; `head` is all
+// ones in the first lane of a second read).  Merged by bitwise ops on an opaque mask rather than a
+// select, so the move always runs in every lane (see shift_up).
+__device__ __forceinline__ uint32_t next_word(uint32_t from_lds, uint32_t w, uint32_t head) {
+  return (shr1(from_lds, w) & ~head) | (from_lds & head);
+}
+
+// One sweep of a wave's reads over haplotype columns start+1 .. start+W.  `lw[t]` holds the word
+// of column start+1+t (zero beyond W, 16-byte aligned).  Each read runs its own anti-diagonal:
+// the lane `pos` lanes below the read's first lane computes column t - pos + 1 at step t, so a
+// second read in the wave starts with the first instead of behind its last lane.  Every lane stops
+// once it has computed column W, so it ends holding its rows of column W (a cut haplotype joins
+// from there) and s.acc of a read's last lane is the raw scaled sum over the read's last row.
+// `span` is the lane count of the wave's longest read.
 template <typename T, int R, bool PD, bool SNAP>
-__device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L, int lane, T init,
-                      const T* snap_in, T scale, T* scratch, bool row0_above, uint32_t keep,
-                      int cap_t, int cap_lane, T& cap, bool freeze, Lane<T, R, PD>& s) {
+__device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int span, int L,
+                      int pos, int lane, uint32_t head, T init, const T* snap_in, T scale,
+                      T* scratch, bool row0_above, uint32_t keep, Lane<T, R, PD>& s) {
   constexpr int NV = snap_values<R, PD>();
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
   if (snap_in) {
@@ -373,59 +381,52 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int L
         store_snapshot<T, R, PD>(s, u, scratch + (size_t(sl - 1) * 64 + lane) * snap_stride<R, PD>());
     }
   };
-  const int steps = W + L - 1;
-  // Ramp: lane l starts at step l and keeps its column-`start` state until then.  Rounded up to
-  // four steps so the steady loop reads aligned groups of four column words.
-  const int ramp = min(steps, (L - 1 + 3) & ~3);
+  const int steps = W + span - 1;
+  // Ramp: a lane starts at step `pos` and keeps its column-`start` state until then.  Rounded up
+  // to four steps so the steady loop reads aligned groups of four column words.
+  const int ramp = min(steps, (span - 1 + 3) & ~3);
   int t = 0;
   Cell<T> Q;
-  // Lane l computes column t - l + 1 at step t; once frozen, only columns up to W.
-  const int last = freeze ? W : steps;
   for (; t < ramp; ++t) {
-    s.w = shr1(lw[t], s.w);
+    s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
-    if (t >= lane && t - lane < last) {
+    if (t >= pos && t - pos < W) {
       cells(s, rw, P, Q);
       P = Q;
       record(P);
     }
-    if (t == cap_t) cap = readlane(s.acc, cap_lane);
   }
-  // Steady state: every lane is live; the diagonal and up neighbours alternate between P and Q.
-  // It stops by step cap_t (frozen: before lane 0 passes column W); the single steps after it
-  // test for the capture and, frozen, which lanes still compute.
-  const int steady_end = freeze ? W : min(steps, cap_t + 1);
+  // Steady state: every lane of the reads is inside columns 1..W until the lanes at `pos` 0 pass
+  // column W at step W; the diagonal and up neighbours alternate between P and Q.
   uint4 q = *reinterpret_cast<const uint4*>(lw + t);
-  for (; t + 4 <= steady_end; t += 4) {
+  for (; t + 4 <= W; t += 4) {
     const uint4 cq = q;
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
-    s.w = shr1(cq.x, s.w);
+    s.w = next_word(cq.x, s.w, head);
     shift_up(s, Q, keep);
     cells(s, rw, P, Q);
     record(Q);
-    s.w = shr1(cq.y, s.w);
+    s.w = next_word(cq.y, s.w, head);
     shift_up(s, P, keep);
     cells(s, rw, Q, P);
     record(P);
-    s.w = shr1(cq.z, s.w);
+    s.w = next_word(cq.z, s.w, head);
     shift_up(s, Q, keep);
     cells(s, rw, P, Q);
     record(Q);
-    s.w = shr1(cq.w, s.w);
+    s.w = next_word(cq.w, s.w, head);
     shift_up(s, P, keep);
     cells(s, rw, Q, P);
     record(P);
   }
-  if (t - 1 == cap_t) cap = readlane(s.acc, cap_lane);
   for (; t < steps; ++t) {
-    s.w = shr1(lw[t], s.w);
+    s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
-    if (t - lane < last) {
+    if (t >= pos && t - pos < W) {
       cells(s, rw, P, Q);
       P = Q;
       record(P);
     }
-    if (t == cap_t) cap = readlane(s.acc, cap_lane);
   }
 }
 
@@ -689,6 +690,11 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     // shift_up).
     uint32_t keep = (two && lane == La) ? 0u : ~0u;
     asm volatile("" : "+v"(keep));
+    // Each read runs its own anti-diagonal (see sweep): a lane's offset below its read's first
+    // lane, the first lane of read B taking the column word from LDS, and the longest read's lanes.
+    const int pos = in_b ? lane - La : lane;
+    const uint32_t head = ~keep;
+    const int span = two ? max(La, Lb) : La;
 
     // Suffix sharing: the region's backward sweeps, once per item, before its forward sweeps.
     constexpr bool kSuffix = std::is_same<T, float>::value && !PD;
@@ -762,21 +768,19 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         const bool cut = suffix && p.cut > 0;
         const int W = (cut ? p.cut - 1 : p.len) - start;
         stage_hap<PD>(a, p, start, W, L, lane, lw, nsnap);
-        // Read A's last lane finishes column W at step W + La - 2.
-        T acc_a = T(0);
         Lane<T, R, PD> st;
         if (nsnap)
-          sweep<T, R, PD, true>(rw, lw, W, L, lane, init, snap_in, scale, scratch, row0_above,
-                                keep, W + La - 2, La - 1, acc_a, cut, st);
+          sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                scratch, row0_above, keep, st);
         else
-          sweep<T, R, PD, false>(rw, lw, W, L, lane, init, snap_in, scale, scratch, row0_above,
-                                 keep, W + La - 2, La - 1, acc_a, cut, st);
+          sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                 scratch, row0_above, keep, st);
         // A read's last row is its last lane's last register (kernel.rs:897-914).
         auto finish = [](double raw) {
           if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted)) return __builtin_nan("");
           return log10(raw) - Num<T>::kLog10Scale;
         };
-        double raw_a = double(cut ? readlane(st.acc, La - 1) : acc_a);
+        double raw_a = double(readlane(st.acc, La - 1));
         double raw_b = double(readlane(st.acc, L - 1));
         if constexpr (kSuffix) {
           if (cut) {
