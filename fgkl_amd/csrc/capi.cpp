@@ -96,11 +96,18 @@ int compute_region(int pd, int double_precision, int double_fallback, int64_t n_
   in.hap_off = hap_off;
   in.hap_bases = hap_bases;
   in.hap_flags = hap_flags;
-  std::unique_ptr<DeviceBatch> b;
-  if (int rc = DeviceBatch::create(in, b)) return rc;
+  // One reusable batch per thread and kind: consecutive calls reuse its device memory, stream
+  // and events (the reference keeps its runners per thread too, kernel.rs:144-191).
+  thread_local std::unique_ptr<DeviceBatch> t_batch[2];
+  std::unique_ptr<DeviceBatch>& b = t_batch[pd ? 1 : 0];
+  if (!b) {
+    if (int rc = DeviceBatch::create(in, b)) return rc;
+  } else if (int rc = b->assign(in)) {
+    return rc;
+  }
   const int mode = double_precision ? kModeFp64 : (double_fallback ? kModeFp32Rescue : kModeFp32NoRescue);
   RunStats st;
-  if (int rc = b->run(mode, 1, 0, &st)) return rc;
+  if (int rc = b->run_latency(mode, &st)) return rc;
   if (fallbacks) *fallbacks = uint64_t(st.rescued_pairs);
   return b->results(out, out_len);
 }

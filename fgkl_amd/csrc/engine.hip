@@ -154,7 +154,20 @@ struct DeviceBatch::Impl {
   HostPlan plan;
   bool pd = false;
   hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
+  std::vector<void*> allocs;  // per-region mode lists, allocated on first use
+  // The batch's arrays live in one device arena filled from one pinned staging buffer; both only
+  // grow, so a batch re-assigned with inputs that fit reuses them (the per-call entry points do).
+  void* arena = nullptr;
+  size_t arena_cap = 0;
+  void* staging = nullptr;
+  size_t staging_cap = 0;
+  struct Piece {
+    void** dst;
+    const void* src;  // host data to upload, or null for device-only arrays
+    size_t bytes, off;
+  };
+  std::vector<Piece> pieces;
+  size_t layout_bytes = 0;
   // device copies
   uint8_t *bases = nullptr, *quals = nullptr, *ins = nullptr, *del = nullptr, *gcp = nullptr;
   int64_t* read_off = nullptr;
@@ -177,7 +190,13 @@ struct DeviceBatch::Impl {
   int32_t* counters = nullptr;  // [0,8) main, [8,16) rescue, [16,24) rescue counts
   double* out = nullptr;
   void* scratch = nullptr;
-  size_t scratch_bytes = 0;
+  size_t scratch_bytes = 0, scratch_cap = 0;
+  // One-region batches (the per-call entry points): every (read, sorted haplotype) pair, its
+  // FP32 rescue list, and the rows per lane of the region's longest read.
+  int32_t* all_pairs = nullptr;
+  int32_t* all_rescue = nullptr;
+  int64_t n_all_pairs = 0;
+  int all_rows = 0;
   // per-region pair lists, for the per-call mode
   int32_t* region_pairs = nullptr;
   std::vector<int64_t> region_pair_off;  // (G * 8 + 1) offsets in pairs, bucket-major per region
@@ -196,14 +215,67 @@ struct DeviceBatch::Impl {
     }
   }
 
+  void free_lazy() {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    region_pairs = nullptr;
+    region_counters = nullptr;
+    region_pair_off.clear();
+    n_region_launches = 0;
+  }
+
   ~Impl() {
+    if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : bev)
       if (e) (void)hipEventDestroy(e);
-    for (void* p : allocs) (void)hipFree(p);
+    free_lazy();
+    if (arena) (void)hipFree(arena);
+    if (staging) (void)hipHostFree(staging);
+    if (scratch) (void)hipFree(scratch);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // Reserves an arena array of `count` elements (at least one), 256-byte aligned; `src` (may be
+  // null) is uploaded into it by commit().
+  template <typename P>
+  void place(P** dst, const P* src, size_t count) {
+    layout_bytes = (layout_bytes + 255) & ~size_t(255);
+    const size_t bytes = sizeof(P) * (count ? count : 1);
+    pieces.push_back(Piece{reinterpret_cast<void**>(dst), count ? src : nullptr, bytes, layout_bytes});
+    layout_bytes += bytes;
+  }
+  // Grows the arena and the staging buffer to the layout, points every array into the arena and
+  // uploads the host data with one copy on the batch's stream.
+  int commit() {
+    if (layout_bytes > arena_cap) {
+      if (arena) HIP_TRY(hipFree(arena));
+      arena = nullptr;
+      arena_cap = 0;
+      HIP_TRY(hipMalloc(&arena, layout_bytes));
+      arena_cap = layout_bytes;
+    }
+    size_t upload_end = 0;
+    for (const Piece& pc : pieces)
+      if (pc.src) upload_end = pc.off + pc.bytes;
+    if (upload_end > staging_cap) {
+      if (staging) HIP_TRY(hipHostFree(staging));
+      staging = nullptr;
+      staging_cap = 0;
+      HIP_TRY(hipHostMalloc(&staging, upload_end, hipHostMallocDefault));
+      staging_cap = upload_end;
+    }
+    for (const Piece& pc : pieces) {
+      *pc.dst = static_cast<char*>(arena) + pc.off;
+      if (pc.src) std::memcpy(static_cast<char*>(staging) + pc.off, pc.src, pc.bytes);
+    }
+    if (upload_end)
+      HIP_TRY(hipMemcpyAsync(arena, staging, upload_end, hipMemcpyHostToDevice, stream));
+    pieces.clear();
+    layout_bytes = 0;
+    return kOk;
   }
 
   template <typename P>
@@ -227,7 +299,7 @@ struct DeviceBatch::Impl {
     a.read_off = read_off; a.read_region = read_region; a.read_local = read_local;
     a.region_out = region_out; a.region_hap0 = region_hap0; a.region_nhap = region_nhap;
     a.plan = hplan; a.snaps = snaps; a.hap_bytes = hap_bytes; a.hap_words = hap_words;
-    a.scratch = scratch; a.out = out; a.lds_words = lds_words;
+    a.scratch = scratch_bytes ? scratch : nullptr; a.out = out; a.lds_words = lds_words;
     a.bsweeps = bsweeps; a.bwrites = bwrites; a.region_bw = region_bw;
     return a;
   }
@@ -365,38 +437,52 @@ DeviceBatch::~DeviceBatch() { delete impl_; }
 
 int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out) {
   std::unique_ptr<DeviceBatch> b(new DeviceBatch);
-  Impl& d = *b->impl_;
+  if (int rc = b->assign(in)) return rc;
+  out = std::move(b);
+  return kOk;
+}
+
+int DeviceBatch::assign(const BatchInput& in) {
+  Impl& d = *impl_;
+  // Work queued with the previous inputs finishes before their memory is reused.
+  if (d.stream) HIP_TRY(hipStreamSynchronize(d.stream));
+  d.drop_graphs();
+  d.free_lazy();
+  d.plan = HostPlan{};
   if (int rc = build_plan(in, d.plan)) return rc;
   if (int rc = device_state(&d.st)) return rc;
   d.pd = in.pd;
-  HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
-  for (auto& e : d.ev) HIP_TRY(hipEventCreate(&e));
-  for (auto& e : d.bev) HIP_TRY(hipEventCreate(&e));
+  if (!d.stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    for (auto& e : d.ev) HIP_TRY(hipEventCreate(&e));
+    for (auto& e : d.bev) HIP_TRY(hipEventCreate(&e));
+  }
   const HostPlan& p = d.plan;
   const int64_t NR = p.n_reads, NH = p.n_haps, G = int64_t(p.region_hap0.size());
   const size_t rb = size_t(in.read_off[NR]);
   const size_t hb = size_t(in.hap_off[NH]);
-  int rc = 0;
-  if ((rc = d.upload(&d.bases, in.bases, rb)) || (rc = d.upload(&d.quals, in.quals, rb)) ||
-      (rc = d.upload(&d.ins, in.ins, rb)) || (rc = d.upload(&d.del, in.del, rb)) ||
-      (rc = d.upload(&d.gcp, in.gcp, rb)) ||
-      (rc = d.upload(&d.read_off, in.read_off, size_t(NR + 1))) ||
-      (rc = d.upload(&d.read_region, p.read_region.data(), size_t(NR))) ||
-      (rc = d.upload(&d.read_local, p.read_local.data(), size_t(NR))) ||
-      (rc = d.upload(&d.region_out, p.region_out.data(), size_t(G + 1))) ||
-      (rc = d.upload(&d.region_hap0, p.region_hap0.data(), size_t(G))) ||
-      (rc = d.upload(&d.region_nhap, p.region_nhap.data(), size_t(G))) ||
-      (rc = d.upload(&d.hplan, p.plan.data(), size_t(NH))) ||
-      (rc = d.upload(&d.snaps, p.snaps.data(), p.snaps.size())) ||
-      (rc = d.upload(&d.bsweeps, p.bsweeps.data(), p.bsweeps.size())) ||
-      (rc = d.upload(&d.bwrites, p.bwrites.data(), p.bwrites.size())) ||
-      (rc = d.upload(&d.region_bw, p.region_bw.data(), p.region_bw.size())))
-    return rc;
-  if (in.pd) {
-    if ((rc = d.upload(&d.hap_words, p.words.data(), p.words.size()))) return rc;
-  } else {
-    if ((rc = d.upload(&d.hap_bytes, in.hap_bases, hb))) return rc;
-  }
+  d.place(&d.bases, in.bases, rb);
+  d.place(&d.quals, in.quals, rb);
+  d.place(&d.ins, in.ins, rb);
+  d.place(&d.del, in.del, rb);
+  d.place(&d.gcp, in.gcp, rb);
+  d.place(&d.read_off, in.read_off, size_t(NR + 1));
+  d.place(&d.read_region, p.read_region.data(), size_t(NR));
+  d.place(&d.read_local, p.read_local.data(), size_t(NR));
+  d.place(&d.region_out, p.region_out.data(), size_t(G + 1));
+  d.place(&d.region_hap0, p.region_hap0.data(), size_t(G));
+  d.place(&d.region_nhap, p.region_nhap.data(), size_t(G));
+  d.place(&d.hplan, p.plan.data(), size_t(NH));
+  d.place(&d.snaps, p.snaps.data(), p.snaps.size());
+  d.place(&d.bsweeps, p.bsweeps.data(), p.bsweeps.size());
+  d.place(&d.bwrites, p.bwrites.data(), p.bwrites.size());
+  d.place(&d.region_bw, p.region_bw.data(), p.region_bw.size());
+  d.hap_words = nullptr;
+  d.hap_bytes = nullptr;
+  if (in.pd)
+    d.place(&d.hap_words, p.words.data(), p.words.size());
+  else
+    d.place(&d.hap_bytes, in.hap_bases, hb);
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
     const auto& items = p.bucket_items[R - 1];
     d.bucket_n[R - 1] = int32_t(items.size() / 2);
@@ -409,11 +495,30 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
     d.bucket_pairs[R - 1] = pairs;
     d.bucket_cells[R - 1] = p.bucket_cells[R - 1];
     d.bucket_work[R - 1] = p.bucket_work[R - 1];
-    if ((rc = d.upload(&d.bucket_items[R - 1], items.data(), items.size()))) return rc;
-    if ((rc = d.alloc(&d.rescue_pairs[R - 1], size_t(2 * pairs)))) return rc;
+    d.place(&d.bucket_items[R - 1], items.data(), items.size());
   }
-  if ((rc = d.alloc(&d.counters, 32))) return rc;
-  if ((rc = d.alloc(&d.out, size_t(p.n_pairs)))) return rc;
+  for (int R = 1; R <= kMaxRowsPerLane; ++R)
+    d.place(&d.rescue_pairs[R - 1], static_cast<const int32_t*>(nullptr),
+            size_t(2 * d.bucket_pairs[R - 1]));
+  d.place(&d.counters, static_cast<const int32_t*>(nullptr), 32);
+  d.place(&d.out, static_cast<const double*>(nullptr), size_t(p.n_pairs));
+  std::vector<int32_t> all;
+  d.n_all_pairs = 0;
+  d.all_rows = 0;
+  d.all_pairs = d.all_rescue = nullptr;
+  if (G == 1 && p.n_pairs <= kMaxQueueItems) {
+    all.reserve(size_t(2 * p.n_pairs));
+    for (int64_t r = 0; r < NR; ++r) {
+      d.all_rows = std::max(d.all_rows, int(p.read_rows[size_t(r)]));
+      for (int32_t k = 0; k < p.region_nhap[0]; ++k) {
+        all.push_back(int32_t(r));
+        all.push_back(k);
+      }
+    }
+    d.n_all_pairs = int64_t(all.size() / 2);
+    d.place(&d.all_pairs, all.data(), all.size());
+    d.place(&d.all_rescue, static_cast<const int32_t*>(nullptr), all.size());
+  }
   if (p.max_slots > kMaxSlots || p.max_nodes > kMaxSlots) {
     set_error(kInvalidArgument, "a region holds more than " + std::to_string(kMaxSlots) +
                                     " distinct haplotype prefix columns or shared suffixes");
@@ -429,6 +534,7 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
                                     " bases exceeds the engine's LDS staging capacity");
     return kInvalidArgument;
   }
+  if (int rc = d.commit()) return rc;
   {
     std::lock_guard<std::mutex> lock(g_dev_mutex);
     for (int R = 1; R <= kMaxRowsPerLane; ++R) d.cfg[R - 1] = d.st->tuned[d.pd][R - 1];
@@ -445,13 +551,14 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
     }
   }
   if (need < 0) return kInternal;
-  if (need > 0) {
+  if (size_t(need) > d.scratch_cap) {
+    if (d.scratch) HIP_TRY(hipFree(d.scratch));
+    d.scratch = nullptr;
+    d.scratch_cap = 0;
     HIP_TRY(hipMalloc(&d.scratch, size_t(need)));
-    d.allocs.push_back(d.scratch);
-    d.scratch_bytes = size_t(need);
+    d.scratch_cap = size_t(need);
   }
-  HIP_TRY(hipStreamSynchronize(d.stream));
-  out = std::move(b);
+  d.scratch_bytes = size_t(need);
   return kOk;
 }
 
@@ -583,6 +690,67 @@ int DeviceBatch::run_per_region(RunStats* stats) {
     HIP_TRY(hipEventElapsedTime(&total, d.ev[0], d.ev[2]));
     stats->total_ms = total;
     stats->primary_ms = total;
+  }
+  return kOk;
+}
+
+int DeviceBatch::run_latency(int mode, RunStats* stats) {
+  Impl& d = *impl_;
+  if (d.plan.n_pairs == 0) {
+    if (stats) *stats = RunStats{};
+    return kOk;
+  }
+  if (!d.all_pairs) {
+    set_error(kInternal, "the latency run needs a one-region batch");
+    return kInternal;
+  }
+  const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
+  hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(d.ev[0], d.stream));
+  const int R = d.all_rows;
+  SweepArgs a = d.base_args();
+  a.items = d.all_pairs;
+  a.n_items = int32_t(d.n_all_pairs);
+  a.counter = d.counters;
+  a.pair_mode = 1;
+  a.share = 0;
+  a.scratch = nullptr;
+  if (fp32) {
+    a.rescue = 1;
+    a.rescue_pairs = d.all_rescue;
+    a.rescue_count = d.counters + 16;
+    if (int rc = d.launch<float>(R, a, d.n_all_pairs, LaunchCfg{R, 256, false})) return rc;
+  } else {
+    if (int rc = d.launch<double>(R, a, d.n_all_pairs)) return rc;
+  }
+  HIP_TRY(hipEventRecord(d.ev[1], d.stream));
+  if (mode == kModeFp32Rescue) {
+    SweepArgs b = d.base_args();
+    b.items = d.all_rescue;
+    b.n_items = 0;
+    b.n_items_dev = d.counters + 16;
+    b.counter = d.counters + 8;
+    b.pair_mode = 1;
+    b.share = 0;
+    b.scratch = nullptr;
+    if (int rc = d.launch<double>(R, b, d.n_all_pairs)) return rc;
+  }
+  HIP_TRY(hipEventRecord(d.ev[2], d.stream));
+  int32_t rescued = 0;
+  if (fp32)
+    HIP_TRY(hipMemcpyAsync(&rescued, d.counters + 16, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           d.stream));
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  if (stats) {
+    *stats = RunStats{};
+    float total = 0.f, first = 0.f;
+    HIP_TRY(hipEventElapsedTime(&total, d.ev[0], d.ev[2]));
+    HIP_TRY(hipEventElapsedTime(&first, d.ev[0], d.ev[1]));
+    stats->total_ms = total;
+    stats->primary_ms = first;
+    stats->rescue_ms = total - first;
+    stats->rescued_pairs = rescued;
   }
   return kOk;
 }

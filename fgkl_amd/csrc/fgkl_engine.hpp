@@ -47,9 +47,16 @@ class DeviceBatch {
   ~DeviceBatch();
   // Validates and plans `in`, then copies it into device memory.
   static int create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out);
+  // Replaces the batch's regions with `in`, reusing its device memory, stream and events where
+  // they suffice (waits for work queued on the previous inputs first).
+  int assign(const BatchInput& in);
   // Computes every pair; `share` enables prefix sharing; `use_graph` replays one captured
   // hipGraph of the whole run.
   int run(int mode, int share, int use_graph, RunStats* stats);
+  // One-region batches: every (read, haplotype) pair in its own wave, one launch at the region's
+  // largest rows per lane, then the FP64 rescue: the shortest device time for the small work of
+  // one per-call region, which leaves most of the GPU idle either way.  No prefix sharing.
+  int run_latency(int mode, RunStats* stats);
   // Copies the log10 likelihoods out, region by region, out[region_base + r * H + h].
   int results(double* out, int64_t n);
   int64_t n_pairs() const;
