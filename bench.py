@@ -9,9 +9,10 @@ haplotype lengths), as the reference counts them (synthetic.rs:166-170).
 
 Alongside: the plain per-call FP64 GPU engine on the same inputs (one pair per wave, one
 synchronous launch set per region), parity of every pair against it, the roofline of the
-dominant sweep kernel, an end-to-end call from host arrays, the gradient engine, a rescue-stress
-and a PD-HMM workload, and fgkl's lane-batched CPU backend (restated in C) timed on the host
-cores over a bounded sample (cpu_baseline).
+dominant sweep kernel, an end-to-end call from host arrays, the drop-in path (one JNI-shaped call
+per region), the gradient engine, a rescue-stress and a PD-HMM workload, Smith-Waterman on the
+reference's aligner benchmark, and fgkl's lane-batched CPU backend (restated in C) timed on the
+host cores over a bounded sample (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -209,6 +210,45 @@ def side_bench(args, config: int) -> dict:
     }
 
 
+def per_call_bench(data, fp64_res, n_regions: int) -> dict:
+    """The drop-in path: one JNI-shaped call per region (fgkl_pairhmm_compute_native, the C-ABI
+    behind FgklPairHmm.computeNative) over the first regions of the workload, from host arrays to
+    host results, as HaplotypeCaller makes them; every pair against the batch's FP64 result."""
+    import fgkl_amd as F
+
+    g = min(n_regions, data.n_regions)
+    regs = []
+    for k in range(g):
+        r0, r1 = int(data.region_reads[k]), int(data.region_reads[k + 1])
+        h0, h1 = int(data.region_haps[k]), int(data.region_haps[k + 1])
+        o0, o1 = int(data.read_offsets[r0]), int(data.read_offsets[r1])
+        q0, q1 = int(data.hap_offsets[h0]), int(data.hap_offsets[h1])
+        regs.append(([np.ascontiguousarray(a[o0:o1]) for a in
+                      (data.bases, data.quals, data.ins, data.dels, data.gcp)],
+                     (data.read_offsets[r0:r1 + 1] - o0).astype(np.int32),
+                     np.ascontiguousarray(data.hap_bases[q0:q1]),
+                     (data.hap_offsets[h0:h1 + 1] - q0).astype(np.int32),
+                     np.zeros((r1 - r0) * (h1 - h0))))
+    cells = first_regions(data, g).cells()
+    out = {"regions": g, "cells": cells,
+           "entry": "fgkl_pairhmm_compute_native (FgklPairHmm.computeNative), one call per region"}
+    for name, dp in (("fp32", 0), ("fp64", 1)):
+        arrs, ro, hb, ho, res = regs[0]
+        F.compute_native(*arrs, ro, hb, ho, dp, res)
+        t0 = time.perf_counter()
+        for arrs, ro, hb, ho, res in regs:
+            F.compute_native(*arrs, ro, hb, ho, dp, res)
+        dt = time.perf_counter() - t0
+        got = np.concatenate([r[4] for r in regs])
+        ref = fp64_res[: len(got)]
+        d = np.abs(got - ref)
+        out[name] = {"ms_per_region": round(dt * 1e3 / g, 3), "gcups": round(cells / dt / 1e9, 2),
+                     "max_abs_dlog10L_vs_batch_fp64": float(d.max())}
+        log(f"per-call {name}: {dt * 1e3 / g:.3f} ms per region, {cells / dt / 1e9:.1f} GCUPS, "
+            f"max |d| vs batch fp64 {d.max():.3g}")
+    return out
+
+
 SW_PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, 32-bit lane ops
 SW_VALU_PER_CELL = 24.4  # VALU instructions per cell of the 8-rows-per-lane steady-state step
 
@@ -321,6 +361,8 @@ def main() -> int:
     ap.add_argument("--no-side", action="store_true",
                     help="skip the rescue-stress and PD workloads")
     ap.add_argument("--no-sw", action="store_true", help="skip the Smith-Waterman workload")
+    ap.add_argument("--no-per-call", action="store_true", help="skip the per-region call leg")
+    ap.add_argument("--per-call-regions", type=int, default=256)
     ap.add_argument("--sw-only", action="store_true",
                     help="run only the Smith-Waterman workload and print its JSON")
     ap.add_argument("--sw-large-pairs", type=int, default=16384)
@@ -488,6 +530,10 @@ def main() -> int:
     result["end_to_end"] = {"seconds": round(e2e_s, 4), "gcups": round(cells / e2e_s / 1e9, 2),
                             "includes": "plan on the host, upload, graph build + run, download"}
     log(f"end to end: {e2e_s * 1e3:.1f} ms, {cells / e2e_s / 1e9:.1f} GCUPS")
+
+    if not args.no_per_call:
+        result["per_call"] = per_call_bench(data, fp32_res if args.no_baseline else fp64,
+                                            args.per_call_regions)
 
     if not args.no_adjoint:
         result["adjoint"] = adjoint_bench(args)
