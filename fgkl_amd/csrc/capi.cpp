@@ -541,7 +541,7 @@ int fgkl_sw_align_native(const uint8_t* reference, int64_t reference_len,
     if (reference_len == 0 || alternate_len == 0)  // Error::EmptySequence
       return fail(kInvalidArgument, "sequences must be non-empty");
     if (!offset || !cigar || cigar_cap < 1) return fail(kInvalidArgument, "no room for the result");
-    if (!device_available())
+    if (!t_sw && !device_available())
       return fail(kInternal, "no HIP device is available: the fgkl GPU engine needs an MI355X (gfx950)");
     const int64_t ro[2] = {0, reference_len}, ao[2] = {0, alternate_len};
     if (!t_sw) {

@@ -29,6 +29,7 @@ class SwBatch {
              const uint8_t* alt);
   ~SwBatch();
   // Aligns every pair with weights {match, mismatch, gap open, gap extend} and one strategy.
+  // With `stats` it returns when the alignments are done; without, results() waits for them.
   int run(const int32_t params[4], int strategy, SwRunStats* stats);
   // Copies the last run's results: offsets[p], CIGAR elements of pair p in
   // cigar[starts[p] .. starts[p+1]) as BAM encodes them (length << 4 | op, op 0 M 1 I 2 D 4 S).

@@ -498,6 +498,7 @@ struct SwBatch::Impl {
   bool ran = false;
 
   ~Impl() {
+    if (stream) (void)hipStreamSynchronize(stream);
     in.release();
     out.release();
     trace.release();
@@ -549,6 +550,8 @@ int SwBatch::create(int64_t n_pairs, const int64_t* ref_off, const uint8_t* ref,
 int SwBatch::assign(int64_t n_pairs, const int64_t* ref_off, const uint8_t* ref,
                     const int64_t* alt_off, const uint8_t* alt) {
   Impl& d = *d_;
+  // The previous inputs' upload and run finish before their buffers are reused.
+  if (d.stream) SW_TRY(hipStreamSynchronize(d.stream));
   d.ran = false;
   if (n_pairs < 0) {
     set_error(kInvalidArgument, "negative pair count");
@@ -673,7 +676,6 @@ int SwBatch::assign(int64_t n_pairs, const int64_t* ref_off, const uint8_t* ref,
   if (ref_len) std::memcpy(h + d.o_ref, ref, size_t(ref_len));
   if (alt_len) std::memcpy(h + d.o_alt, alt, size_t(alt_len));
   SW_TRY(hipMemcpyAsync(d.in.p, h, d.in_bytes, hipMemcpyHostToDevice, d.stream));
-  SW_TRY(hipStreamSynchronize(d.stream));
   return kOk;
 }
 
@@ -687,7 +689,7 @@ int SwBatch::run(const int32_t params[4], int strategy, SwRunStats* stats) {
   int* counters = static_cast<int*>(d.counters.p);
   sw_fill_i32<<<1, 64, 0, d.stream>>>(counters, kSwMaxRows + 1, 0);
   SW_TRY(hipGetLastError());
-  SW_TRY(hipEventRecord(d.ev[0], d.stream));
+  if (stats) SW_TRY(hipEventRecord(d.ev[0], d.stream));
   for (int R = 1; R <= kSwMaxRows; ++R) {
     const int64_t b0 = d.bucket_begin[R - 1], pairs = d.bucket_begin[R] - b0;
     if (pairs > 0) {
@@ -726,11 +728,11 @@ int SwBatch::run(const int32_t params[4], int strategy, SwRunStats* stats) {
                          lds, d.stream, a);
       SW_TRY(hipGetLastError());
     }
-    SW_TRY(hipEventRecord(d.ev[R], d.stream));
+    if (stats) SW_TRY(hipEventRecord(d.ev[R], d.stream));
   }
-  SW_TRY(hipStreamSynchronize(d.stream));
   d.ran = true;
   if (stats) {
+    SW_TRY(hipStreamSynchronize(d.stream));
     *stats = SwRunStats{};
     float ms = 0;
     SW_TRY(hipEventElapsedTime(&ms, d.ev[0], d.ev[kSwMaxRows]));

@@ -282,7 +282,9 @@ int fgkl_sw_align_native(const uint8_t* reference, int64_t reference_len,
  * calling thread).  Pair p aligns alternate bytes alt[alt_offsets[p] .. alt_offsets[p+1]) to
  * ref[ref_offsets[p] .. ref_offsets[p+1]); both tables have n_pairs + 1 entries starting at 0.
  * The pairs stay resident on the device; fgkl_sw_batch_run aligns all of them with one set of
- * weights and one strategy and may be called again with others.  fgkl_sw_batch_results writes
+ * weights and one strategy and may be called again with others (with stats it returns when the
+ * alignments are done; with stats NULL it may return earlier, and fgkl_sw_batch_results waits for
+ * them).  fgkl_sw_batch_results writes
  * offsets[p] and pair p's CIGAR elements to cigar[cigar_starts[p] .. cigar_starts[p+1])
  * (cigar_starts has n_pairs + 1 entries; cigar_cap >= fgkl_sw_batch_cigar_bound suffices).
  */
