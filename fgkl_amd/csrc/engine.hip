@@ -700,10 +700,8 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     if (stats) *stats = RunStats{};
     return kOk;
   }
-  if (!d.all_pairs) {
-    set_error(kInternal, "the latency run needs a one-region batch");
-    return kInternal;
-  }
+  // Batches of several regions (or one too large for the pair list) run the batched way.
+  if (!d.all_pairs) return run(mode == kModeFp64PerRegion ? kModeFp64 : mode, 1, 0, stats);
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
   hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
   HIP_TRY(hipGetLastError());

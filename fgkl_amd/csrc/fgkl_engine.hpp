@@ -56,6 +56,7 @@ class DeviceBatch {
   // One-region batches: every (read, haplotype) pair in its own wave, one launch at the region's
   // largest rows per lane, then the FP64 rescue: the shortest device time for the small work of
   // one per-call region, which leaves most of the GPU idle either way.  No prefix sharing.
+  // Other batches run as run(mode, share, no graph).
   int run_latency(int mode, RunStats* stats);
   // Copies the log10 likelihoods out, region by region, out[region_base + r * H + h].
   int results(double* out, int64_t n);

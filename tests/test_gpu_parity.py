@@ -41,14 +41,39 @@ def assert_rel(actual, expected, rel=DOUBLE_REL, what=""):
     assert d[i] <= bound[i], f"{what}: pair {i}: got {actual[i]!r}, expected {expected[i]!r} (diff {d[i]})"
 
 
+def _shared_batch_agrees(pd, precision, reads, haps, per_call):
+    """The per-call entry runs each pair in its own wave; the same region as a one-region batch
+    runs the prefix-shared sweeps.  Both must give the same likelihoods within the precision's
+    bound, so every per-call case here also exercises prefix sharing."""
+    if not reads or not haps:
+        return
+    from fgkl_amd.synthetic import batch_from_regions
+
+    b = F.Batch(batch_from_regions([(reads, haps)], pd=pd))
+    double = precision is F.Precision.DOUBLE
+    b.run(F.MODE_FP64 if double else F.MODE_FP32_RESCUE, share=True)
+    shared = b.results()
+    b.close()
+    ok = np.isfinite(per_call)
+    assert np.array_equal(ok, np.isfinite(shared))
+    d = np.abs(shared[ok] - per_call[ok])
+    bound = (DOUBLE_REL * np.maximum(1.0, np.abs(per_call[ok]))) if double else FLOAT_TOL
+    assert np.all(d <= bound), float(d.max())
+
+
 def compute(precision, reads, haps, fallback=True):
     hmm = F.PairHmm(F.Config(precision=precision, double_fallback=fallback))
-    return hmm.compute_log10_likelihoods(reads, haps), hmm.fallback_pairs()
+    out = hmm.compute_log10_likelihoods(reads, haps)
+    if fallback:
+        _shared_batch_agrees(False, precision, reads, haps, out)
+    return out, hmm.fallback_pairs()
 
 
 def pd_compute(precision, reads, haps):
     hmm = F.PdPairHmm(F.Config(precision=precision))
-    return hmm.compute_log10_likelihoods(reads, haps), hmm.fallback_pairs()
+    out = hmm.compute_log10_likelihoods(reads, haps)
+    _shared_batch_agrees(True, precision, reads, haps, out)
+    return out, hmm.fallback_pairs()
 
 
 def read(bases: bytes, q=30, i=45, d=45, g=10):
