@@ -246,6 +246,54 @@ def per_call_bench(data, fp64_res, n_regions: int) -> dict:
                      "max_abs_dlog10L_vs_batch_fp64": float(d.max())}
         log(f"per-call {name}: {dt * 1e3 / g:.3f} ms per region, {cells / dt / 1e9:.1f} GCUPS, "
             f"max |d| vs batch fp64 {d.max():.3g}")
+    out["threads"] = threaded_calls(regs, cells, fp64_res)
+    return out
+
+
+def threaded_calls(regs, cells, fp64_res, thread_counts=(8, 32)) -> dict:
+    """The same per-region FP32 calls from several caller threads at once, each thread taking every
+    T-th region: on their own (each thread's region on its own stream) and gathered into shared
+    batches by the aggregator (fgkl_aggregate_config with up to T/2 regions a batch, so that one
+    batch gathers while the other runs)."""
+    import threading
+
+    import fgkl_amd as F
+
+    def run(T):
+        def worker(t):
+            for arrs, ro, hb, ho, res in regs[t::T]:
+                F.compute_native(*arrs, ro, hb, ho, 0, res)
+
+        threads = [threading.Thread(target=worker, args=(t,)) for t in range(T)]
+        t0 = time.perf_counter()
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        return time.perf_counter() - t0
+
+    out = {}
+    for T in thread_counts:
+        row = {}
+        for label, agg in (("alone", 0), ("aggregated", max(T // 2, 2))):
+            F.aggregate(agg, 2000)
+            try:
+                run(T)
+                b0, r0 = F.aggregate_stats()
+                dt = run(T)
+                b1, r1 = F.aggregate_stats()
+            finally:
+                F.aggregate(0)
+            got = np.concatenate([r[4] for r in regs])
+            d = np.abs(got - fp64_res[: len(got)])
+            row[label] = {"gcups": round(cells / dt / 1e9, 2),
+                          "ms_per_region": round(dt * 1e3 / len(regs), 3),
+                          "max_abs_dlog10L_vs_batch_fp64": float(d.max())}
+            if agg:
+                row[label]["regions_per_batch"] = round((r1 - r0) / max(b1 - b0, 1), 2)
+            log(f"per-call fp32 x{T} threads {label}: {cells / dt / 1e9:.1f} GCUPS, "
+                f"max |d| vs batch fp64 {d.max():.3g}")
+        out[f"threads_{T}"] = row
     return out
 
 

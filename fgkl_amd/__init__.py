@@ -23,7 +23,8 @@ __all__ = [
     "Error", "EmptyRead", "MismatchedReadArrays", "EmptyHaplotype",
     "MismatchedHaplotypeArrays", "OutputLength", "BackendUnavailable", "InvalidArgument",
     "NativeError", "SNP", "DEL_START", "DEL_END", "ALT_A", "ALT_C", "ALT_G", "ALT_T",
-    "compute_native", "pd_compute_native", "Batch", "backend_name", "device_count",
+    "compute_native", "pd_compute_native", "aggregate", "aggregate_stats", "Batch",
+    "backend_name", "device_count",
 ]
 
 # pd.rs:15-27
@@ -278,6 +279,20 @@ def pd_compute_native(bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap
             double_precision, likelihoods)
 
 
+def aggregate(max_regions: int, max_wait_us: int = 1000) -> None:
+    """Gathers concurrent compute_native / pd_compute_native calls (up to `max_regions` arriving
+    within `max_wait_us` of the first) into one batched run; 0 turns it off (include/fgkl.h,
+    fgkl_aggregate_config)."""
+    _check(lib().fgkl_aggregate_config(int(max_regions), int(max_wait_us)))
+
+
+def aggregate_stats() -> tuple[int, int]:
+    """(batches run, regions they held) since the library loaded."""
+    b, r = C.c_int64(), C.c_int64()
+    lib().fgkl_aggregate_stats(C.byref(b), C.byref(r))
+    return b.value, r.value
+
+
 # ---- batched engine --------------------------------------------------------------------------
 class Batch:
     """Many regions resident in device memory (include/fgkl.h, fgkl_batch_*)."""
@@ -298,6 +313,12 @@ class Batch:
     def run(self, mode: int = MODE_FP32_RESCUE, share: bool = True, graph: bool = False):
         st = _lib.RunStats()
         _check(lib().fgkl_batch_run(self._h, mode, int(share), int(graph), C.byref(st)))
+        return st
+
+    def run_pairs(self, mode: int = MODE_FP32_RESCUE):
+        """Every (read, haplotype) pair in its own wave (include/fgkl.h, fgkl_batch_run_pairs)."""
+        st = _lib.RunStats()
+        _check(lib().fgkl_batch_run_pairs(self._h, mode, C.byref(st)))
         return st
 
     def work(self) -> dict:

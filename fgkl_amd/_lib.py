@@ -80,6 +80,7 @@ SIGNATURES = {
          i64p, u8p, u8p],
     ),
     "fgkl_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)]),
+    "fgkl_batch_run_pairs": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(RunStats)]),
     "fgkl_batch_results": (C.c_int, [C.c_void_p, f64p, C.c_int64]),
     "fgkl_batch_pairs": (C.c_int64, [C.c_void_p]),
     "fgkl_batch_work": (C.c_int, [C.c_void_p, i64p]),
@@ -111,6 +112,8 @@ SIGNATURES = {
     "fgkl_grad_backward": (C.c_int, [C.c_void_p, f64p, f64p, f64p, C.c_double, f64p, f64p, f64p,
                                      f64p, f64p, f64p, f64p]),
     "fgkl_grad_free": (None, [C.c_void_p]),
+    "fgkl_aggregate_config": (C.c_int, [C.c_int, C.c_int]),
+    "fgkl_aggregate_stats": (None, [i64p, i64p]),
     "fgkl_sw_backend_name": (C.c_char_p, []),
     "fgkl_sw_align_native": (C.c_int, [u8p, C.c_int64, u8p, C.c_int64, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, C.c_int32, i32p, C.c_char_p,

@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libfgkl.so"
 SOURCES = ["sweep_f32.hip", "sweep_f64.hip", "engine.hip", "adjoint.hip", "sw.hip", "plan.cpp",
-           "synthetic.cpp", "replay.cpp", "capi.cpp", "jni_shim.cpp"]
+           "synthetic.cpp", "replay.cpp", "aggregate.cpp", "capi.cpp", "jni_shim.cpp"]
 ARCH = os.environ.get("FGKL_OFFLOAD_ARCH", "gfx950")
 
 

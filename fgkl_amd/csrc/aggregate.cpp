@@ -1,0 +1,218 @@
+// Aggregation of per-call regions: concurrent JNI-shaped calls (one region each, as HaplotypeCaller
+// threads make them) are gathered for up to a short wait and run as one batch through the batched
+// engine (prefix and suffix sharing, one launch set per rows-per-lane bucket), each caller blocking
+// until its own region's likelihoods are written.
+//
+// Leader / follower: a queued caller that finds no batch being gathered for its kind, and one of
+// the kind's two batch slots free, becomes the leader: it waits until `max_regions` calls are
+// queued or `max_wait_us` has passed, takes them, runs them in its slot and wakes every caller it
+// served; meanwhile the next queued caller may lead the next batch in the other slot.  The
+// inputs are read in place: their callers block until their results are written.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fgkl_aggregate.hpp"
+#include "fgkl_engine.hpp"
+
+namespace fgkl {
+namespace {
+
+struct Job {
+  const RegionCall* call;
+  int rc = kOk;
+  std::string err;
+  bool done = false;
+};
+
+// Two batches per kind: while one runs, the next leader gathers and uploads into the other.
+struct Queue {
+  std::vector<Job*> pending;
+  bool gathering = false;
+  std::unique_ptr<DeviceBatch> batch[2];  // device memory and streams reused from batch to batch
+  bool busy[2] = {false, false};
+};
+
+struct Aggregator {
+  std::mutex mu;
+  std::condition_variable cv;
+  Queue queues[4];  // [pd][double_precision]
+  int max_regions = 0;
+  int max_wait_us = 0;
+  int64_t batches = 0, regions = 0;
+};
+
+Aggregator& agg() {
+  static Aggregator a;
+  return a;
+}
+
+std::once_flag g_env_once;
+
+// FGKL_AGGREGATE=<regions>[:<wait microseconds>] turns aggregation on for programs that cannot
+// call fgkl_aggregate_config (a JVM running the unchanged Java bindings).
+void read_env() {
+  const char* v = std::getenv("FGKL_AGGREGATE");
+  if (!v || !*v) return;
+  char* end = nullptr;
+  const long n = std::strtol(v, &end, 10);
+  long w = 1000;
+  if (end && *end == ':') w = std::strtol(end + 1, nullptr, 10);
+  if (n > 1 && w >= 0) {
+    std::lock_guard<std::mutex> lk(agg().mu);
+    agg().max_regions = int(n);
+    agg().max_wait_us = int(w);
+  }
+}
+
+constexpr int64_t kPairRunMaxCells = int64_t(15) * 1000 * 1000 * 1000;
+
+template <typename T>
+void append(std::vector<T>& dst, const T* src, int64_t n) {
+  if (n > 0) dst.insert(dst.end(), src, src + n);
+}
+
+// Runs the jobs' regions as one batch; on any failure, each region alone, so an invalid region
+// fails only its own call.
+void run_jobs(std::unique_ptr<DeviceBatch>& batch, const std::vector<Job*>& jobs) {
+  const RegionCall& first = *jobs[0]->call;
+  std::vector<int64_t> region_read{0}, region_hap{0}, read_off{0}, hap_off{0};
+  std::vector<uint8_t> bases, quals, ins, del, gcp, hap_bases, hap_flags;
+  int64_t pairs = 0, cells = 0;
+  for (const Job* j : jobs) {
+    const RegionCall& c = *j->call;
+    const int64_t rb = c.read_off[c.n_reads], hb = c.hap_off[c.n_haps];
+    pairs += c.n_reads * c.n_haps;
+    cells += rb * hb;
+    for (int64_t r = 1; r <= c.n_reads; ++r) read_off.push_back(read_off.back() - c.read_off[r - 1] + c.read_off[r]);
+    for (int64_t h = 1; h <= c.n_haps; ++h) hap_off.push_back(hap_off.back() - c.hap_off[h - 1] + c.hap_off[h]);
+    region_read.push_back(region_read.back() + c.n_reads);
+    region_hap.push_back(region_hap.back() + c.n_haps);
+    append(bases, c.bases, rb);
+    append(quals, c.quals, rb);
+    append(ins, c.ins, rb);
+    append(del, c.del, rb);
+    append(gcp, c.gcp, rb);
+    append(hap_bases, c.hap_bases, hb);
+    if (c.pd) append(hap_flags, c.hap_flags, hb);
+  }
+  BatchInput in;
+  in.pd = first.pd;
+  in.n_regions = int64_t(jobs.size());
+  in.region_read = region_read.data();
+  in.region_hap = region_hap.data();
+  in.read_off = read_off.data();
+  in.bases = bases.data();
+  in.quals = quals.data();
+  in.ins = ins.data();
+  in.del = del.data();
+  in.gcp = gcp.data();
+  in.hap_off = hap_off.data();
+  in.hap_bases = hap_bases.data();
+  in.hap_flags = first.pd ? hap_flags.data() : nullptr;
+  // A batch of a few regions finishes sooner with every pair in a wave of its own than with
+  // prefix-shared sweeps, whose waves each walk a read through all of its region's haplotypes
+  // (a floor of a few milliseconds); measured on the batched uneven workload the two meet near
+  // 128 regions, 1.5e10 cells.
+  in.pairs_only = pairs <= kMaxPairList && cells <= kPairRunMaxCells;
+  const int mode = first.double_precision ? kModeFp64 : kModeFp32Rescue;
+  int rc = batch ? batch->assign(in) : DeviceBatch::create(in, batch);
+  std::vector<double> out;
+  if (rc == kOk) rc = in.pairs_only ? batch->run_latency(mode, nullptr) : batch->run(mode, 1, 0, nullptr);
+  if (rc == kOk) {
+    out.resize(size_t(batch->n_pairs()));
+    rc = batch->results(out.data(), int64_t(out.size()));
+  }
+  if (rc == kOk) {
+    size_t o = 0;
+    for (Job* j : jobs) {
+      std::memcpy(j->call->out, out.data() + o, sizeof(double) * size_t(j->call->out_len));
+      o += size_t(j->call->out_len);
+      j->rc = kOk;
+    }
+    return;
+  }
+  for (Job* j : jobs) {
+    j->rc = compute_region_alone(*j->call);
+    if (j->rc != kOk) j->err = last_error();
+  }
+}
+
+}  // namespace
+
+void aggregate_config(int max_regions, int max_wait_us) {
+  std::call_once(g_env_once, read_env);
+  std::lock_guard<std::mutex> lk(agg().mu);
+  agg().max_regions = max_regions > 1 ? max_regions : 0;
+  agg().max_wait_us = max_wait_us > 0 ? max_wait_us : 0;
+}
+
+bool aggregate_enabled() {
+  std::call_once(g_env_once, read_env);
+  std::lock_guard<std::mutex> lk(agg().mu);
+  return agg().max_regions > 1;
+}
+
+void aggregate_stats(int64_t* batches, int64_t* regions) {
+  std::lock_guard<std::mutex> lk(agg().mu);
+  if (batches) *batches = agg().batches;
+  if (regions) *regions = agg().regions;
+}
+
+int aggregate_run(const RegionCall& call) {
+  Aggregator& a = agg();
+  Queue& q = a.queues[(call.pd ? 2 : 0) + (call.double_precision ? 1 : 0)];
+  Job job;
+  job.call = &call;
+  std::unique_lock<std::mutex> lk(a.mu);
+  q.pending.push_back(&job);
+  a.cv.notify_all();
+  while (!job.done) {
+    if (!q.gathering && !(q.busy[0] && q.busy[1]) &&
+        std::find(q.pending.begin(), q.pending.end(), &job) != q.pending.end()) {
+      // Lead: gather until the batch is full or the wait is over, take a free batch slot, and
+      // let the next leader gather while this batch runs.
+      q.gathering = true;
+      const auto deadline =
+          std::chrono::steady_clock::now() + std::chrono::microseconds(a.max_wait_us);
+      while (int(q.pending.size()) < a.max_regions &&
+             a.cv.wait_until(lk, deadline) != std::cv_status::timeout) {
+      }
+      const size_t take = std::min(q.pending.size(), size_t(std::max(a.max_regions, 1)));
+      std::vector<Job*> jobs(q.pending.begin(), q.pending.begin() + long(take));
+      q.pending.erase(q.pending.begin(), q.pending.begin() + long(take));
+      const int slot = q.busy[0] ? 1 : 0;
+      q.busy[slot] = true;
+      q.gathering = false;
+      ++a.batches;
+      a.regions += int64_t(jobs.size());
+      a.cv.notify_all();
+      lk.unlock();
+      try {
+        run_jobs(q.batch[slot], jobs);
+      } catch (const std::exception& e) {
+        for (Job* j : jobs) {
+          j->rc = kInternal;
+          j->err = std::string("fgkl native code failed: ") + e.what();
+        }
+      }
+      lk.lock();
+      for (Job* j : jobs) j->done = true;
+      q.busy[slot] = false;
+      a.cv.notify_all();
+    } else {
+      a.cv.wait(lk);
+    }
+  }
+  lk.unlock();
+  if (job.rc != kOk) set_error(job.rc, job.err);
+  return job.rc;
+}
+
+}  // namespace fgkl

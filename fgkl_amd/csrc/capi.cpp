@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/fgkl.h"
+#include "fgkl_aggregate.hpp"
 #include "fgkl_engine.hpp"
 #include "fgkl_grad.hpp"
 #include "fgkl_internal.hpp"
@@ -66,12 +67,9 @@ int split(const int32_t* off, int64_t n, int64_t total, std::vector<int64_t>& ou
   return kOk;
 }
 
-// PairHmm::compute_log10_likelihoods (lib.rs:272-293) / PdPairHmm (pd.rs:69-95) on one region.
-int compute_region(int pd, int double_precision, int double_fallback, int64_t n_reads,
-                   const int64_t* read_off, const uint8_t* bases, const uint8_t* quals,
-                   const uint8_t* ins, const uint8_t* del, const uint8_t* gcp, int64_t n_haps,
-                   const int64_t* hap_off, const uint8_t* hap_bases, const uint8_t* hap_flags,
-                   double* out, int64_t out_len, uint64_t* fallbacks) {
+// The checks of PairHmm::compute_log10_likelihoods (lib.rs:54-67, :279-281, jni lib.rs:325-335).
+int check_region(int64_t n_reads, const int64_t* read_off, int64_t n_haps, const int64_t* hap_off,
+                 int64_t out_len) {
   if (n_reads < 0 || n_haps < 0) return fail(kInvalidArgument, "negative read or haplotype count");
   for (int64_t i = 0; i < n_reads; ++i)
     if (read_off[i + 1] == read_off[i])
@@ -84,7 +82,17 @@ int compute_region(int pd, int double_precision, int double_fallback, int64_t n_
     return fail(kInvalidArgument, "output has " + std::to_string(out_len) +
                                       " elements but reads x haplotypes is " +
                                       std::to_string(expected));
-  if (expected == 0) return kOk;
+  return kOk;
+}
+
+// PairHmm::compute_log10_likelihoods (lib.rs:272-293) / PdPairHmm (pd.rs:69-95) on one region.
+int compute_region(int pd, int double_precision, int double_fallback, int64_t n_reads,
+                   const int64_t* read_off, const uint8_t* bases, const uint8_t* quals,
+                   const uint8_t* ins, const uint8_t* del, const uint8_t* gcp, int64_t n_haps,
+                   const int64_t* hap_off, const uint8_t* hap_bases, const uint8_t* hap_flags,
+                   double* out, int64_t out_len, uint64_t* fallbacks) {
+  if (int rc = check_region(n_reads, read_off, n_haps, hap_off, out_len)) return rc;
+  if (n_reads * n_haps == 0) return kOk;
   BatchInput in;
   in.pd = pd != 0;
   in.n_regions = 1;
@@ -96,6 +104,7 @@ int compute_region(int pd, int double_precision, int double_fallback, int64_t n_
   in.hap_off = hap_off;
   in.hap_bases = hap_bases;
   in.hap_flags = hap_flags;
+  in.pairs_only = n_reads * n_haps <= kMaxPairList;
   // One reusable batch per thread and kind: consecutive calls reuse its device memory, stream
   // and events (the reference keeps its runners per thread too, kernel.rs:144-191).
   thread_local std::unique_ptr<DeviceBatch> t_batch[2];
@@ -129,6 +138,23 @@ int compute_native(int pd, const uint8_t* bases, int64_t bases_len, const uint8_
       return fail(kInvalidArgument, "haplotype bases and PD flags differ in total length");
     if (int rc = split(hap_offsets, hap_offsets_len, hap_bases_len, hoff)) return rc;
     const int64_t n_reads = read_offsets_len - 1, n_haps = hap_offsets_len - 1;
+    if (aggregate_enabled()) {
+      if (int rc = check_region(n_reads, roff.data(), n_haps, hoff.data(), likelihoods_len)) return rc;
+      if (n_reads * n_haps == 0) return kOk;
+      RegionCall call;
+      call.pd = pd != 0;
+      call.double_precision = double_precision != 0;
+      call.n_reads = n_reads;
+      call.read_off = roff.data();
+      call.bases = bases; call.quals = quals; call.ins = ins; call.del = del; call.gcp = gcp;
+      call.n_haps = n_haps;
+      call.hap_off = hoff.data();
+      call.hap_bases = hap_bases;
+      call.hap_flags = pd ? hap_flags : nullptr;
+      call.out = likelihoods;
+      call.out_len = likelihoods_len;
+      return aggregate_run(call);
+    }
     return compute_region(pd, double_precision, 1, n_reads, roff.data(), bases, quals, ins, del,
                           gcp, n_haps, hoff.data(), hap_bases, pd ? hap_flags : nullptr,
                           likelihoods, likelihoods_len, nullptr);
@@ -138,6 +164,14 @@ int compute_native(int pd, const uint8_t* bases, int64_t bases_len, const uint8_
 }
 
 }  // namespace
+
+namespace fgkl {
+int compute_region_alone(const RegionCall& c) {
+  return compute_region(c.pd, c.double_precision, 1, c.n_reads, c.read_off, c.bases, c.quals,
+                        c.ins, c.del, c.gcp, c.n_haps, c.hap_off, c.hap_bases, c.hap_flags, c.out,
+                        c.out_len, nullptr);
+}
+}  // namespace fgkl
 
 extern "C" {
 
@@ -242,6 +276,20 @@ int fgkl_batch_create(fgkl_batch** out, int pd, int64_t n_regions, const int64_t
   } catch (const std::exception& e) {
     return fail(kInternal, std::string("fgkl native code failed: ") + e.what());
   }
+}
+
+int fgkl_batch_run_pairs(fgkl_batch* batch, int mode, fgkl_run_stats* stats) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  RunStats st;
+  const int rc = batch->batch->run_latency(mode, stats ? &st : nullptr);
+  if (rc == kOk && stats) {
+    *stats = fgkl_run_stats{};
+    stats->total_ms = st.total_ms;
+    stats->primary_ms = st.primary_ms;
+    stats->rescue_ms = st.rescue_ms;
+    stats->rescued_pairs = st.rescued_pairs;
+  }
+  return rc;
 }
 
 int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_graph,
@@ -611,3 +659,15 @@ int64_t fgkl_sw_batch_cigar_bound(const fgkl_sw_batch* batch) {
   return batch ? batch->batch->cigar_bound() : 0;
 }
 void fgkl_sw_batch_free(fgkl_sw_batch* batch) { delete batch; }
+
+// ---------------------------------------------------------------------------------------------
+// Aggregation of per-call regions
+
+int fgkl_aggregate_config(int max_regions, int max_wait_us) {
+  if (max_regions < 0 || max_wait_us < 0)
+    return fail(kInvalidArgument, "aggregation needs a non-negative region count and wait");
+  aggregate_config(max_regions, max_wait_us);
+  return kOk;
+}
+
+void fgkl_aggregate_stats(int64_t* batches, int64_t* regions) { aggregate_stats(batches, regions); }

@@ -55,6 +55,8 @@ constexpr int kBlocks[3] = {64, 128, 256};
 int block_index(int block) { return block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1; }
 
 constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
+// Pairs a one-pair-per-wave run takes in a single launch: 256 CUs hold 8192 waves at 32 per CU.
+constexpr int64_t kOneLaunchPairs = 8192;
 
 struct DeviceState {
   int device = -1;
@@ -191,12 +193,15 @@ struct DeviceBatch::Impl {
   double* out = nullptr;
   void* scratch = nullptr;
   size_t scratch_bytes = 0, scratch_cap = 0;
-  // One-region batches (the per-call entry points): every (read, sorted haplotype) pair, its
-  // FP32 rescue list, and the rows per lane of the region's longest read.
+  // Batches of up to kMaxPairList pairs (the per-call entry points, aggregated calls): every
+  // (read, sorted haplotype) pair grouped by the read's rows per lane (bucket R from
+  // all_off[R - 1]), their FP32 rescue list, and the largest rows per lane.
   int32_t* all_pairs = nullptr;
   int32_t* all_rescue = nullptr;
   int64_t n_all_pairs = 0;
+  int64_t all_off[kMaxRowsPerLane + 1] = {};
   int all_rows = 0;
+  bool pairs_only = false;  // planned for run_latency only
   // per-region pair lists, for the per-call mode
   int32_t* region_pairs = nullptr;
   std::vector<int64_t> region_pair_off;  // (G * 8 + 1) offsets in pairs, bucket-major per region
@@ -452,6 +457,7 @@ int DeviceBatch::assign(const BatchInput& in) {
   if (int rc = build_plan(in, d.plan)) return rc;
   if (int rc = device_state(&d.st)) return rc;
   d.pd = in.pd;
+  d.pairs_only = in.pairs_only;
   if (!d.stream) {
     HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
     for (auto& e : d.ev) HIP_TRY(hipEventCreate(&e));
@@ -506,16 +512,23 @@ int DeviceBatch::assign(const BatchInput& in) {
   d.n_all_pairs = 0;
   d.all_rows = 0;
   d.all_pairs = d.all_rescue = nullptr;
-  if (G == 1 && p.n_pairs <= kMaxQueueItems) {
+  std::fill(std::begin(d.all_off), std::end(d.all_off), 0);
+  if (p.n_pairs <= kMaxPairList) {
     all.reserve(size_t(2 * p.n_pairs));
-    for (int64_t r = 0; r < NR; ++r) {
-      d.all_rows = std::max(d.all_rows, int(p.read_rows[size_t(r)]));
-      for (int32_t k = 0; k < p.region_nhap[0]; ++k) {
-        all.push_back(int32_t(r));
-        all.push_back(k);
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      d.all_off[R - 1] = int64_t(all.size() / 2);
+      for (int64_t r = 0; r < NR; ++r) {
+        if (p.read_rows[size_t(r)] != R) continue;
+        d.all_rows = R;
+        const int32_t H = p.region_nhap[size_t(p.read_region[size_t(r)])];
+        for (int32_t k = 0; k < H; ++k) {
+          all.push_back(int32_t(r));
+          all.push_back(k);
+        }
       }
     }
     d.n_all_pairs = int64_t(all.size() / 2);
+    d.all_off[kMaxRowsPerLane] = d.n_all_pairs;
     d.place(&d.all_pairs, all.data(), all.size());
     d.place(&d.all_rescue, static_cast<const int32_t*>(nullptr), all.size());
   }
@@ -572,6 +585,10 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
   if (d.plan.n_pairs == 0) {
     if (stats) *stats = RunStats{};
     return kOk;
+  }
+  if (d.pairs_only) {
+    set_error(kInvalidArgument, "this batch was planned for one-pair-per-wave runs only");
+    return kInvalidArgument;
   }
   if (mode == kModeFp64PerRegion) return run_per_region(stats);
   HIP_TRY(hipEventRecord(d.ev[0], d.stream));
@@ -700,28 +717,43 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     if (stats) *stats = RunStats{};
     return kOk;
   }
-  // Batches of several regions (or one too large for the pair list) run the batched way.
+  // Batches too large for the pair list run the batched way.
   if (!d.all_pairs) return run(mode == kModeFp64PerRegion ? kModeFp64 : mode, 1, 0, stats);
+  if (mode != kModeFp32Rescue && mode != kModeFp64 && mode != kModeFp64PerRegion &&
+      mode != kModeFp32NoRescue) {
+    set_error(kInvalidArgument, "unknown run mode");
+    return kInvalidArgument;
+  }
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
   hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(d.ev[0], d.stream));
-  const int R = d.all_rows;
-  SweepArgs a = d.base_args();
-  a.items = d.all_pairs;
-  a.n_items = int32_t(d.n_all_pairs);
-  a.counter = d.counters;
-  a.pair_mode = 1;
-  a.share = 0;
-  a.scratch = nullptr;
-  if (fp32) {
-    a.rescue = 1;
-    a.rescue_pairs = d.all_rescue;
-    a.rescue_count = d.counters + 16;
-    if (int rc = d.launch<float>(R, a, d.n_all_pairs, LaunchCfg{R, 256, false})) return rc;
-  } else {
-    if (int rc = d.launch<double>(R, a, d.n_all_pairs)) return rc;
+  // A batch whose pairs leave the GPU partly idle runs them in one launch at the largest rows per
+  // lane (launches on a stream run one after another, and each would add a pair's sweep to the
+  // time to results); a larger one launches each rows-per-lane bucket at its own rows, each with
+  // its own work-queue head, the FP32 launches sharing one rescue list.
+  const bool one_launch = d.n_all_pairs <= kOneLaunchPairs;
+  for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+    const int64_t first = one_launch ? 0 : d.all_off[R - 1];
+    const int64_t n = one_launch ? (R == d.all_rows ? d.n_all_pairs : 0) : d.all_off[R] - first;
+    if (!n) continue;
+    SweepArgs a = d.base_args();
+    a.items = d.all_pairs + 2 * first;
+    a.n_items = int32_t(n);
+    a.counter = d.counters + (R - 1);
+    a.pair_mode = 1;
+    a.share = 0;
+    a.scratch = nullptr;
+    if (fp32) {
+      a.rescue = 1;
+      a.rescue_pairs = d.all_rescue;
+      a.rescue_count = d.counters + 16;
+      if (int rc = d.launch<float>(R, a, n, LaunchCfg{R, 256, false})) return rc;
+    } else {
+      if (int rc = d.launch<double>(R, a, n)) return rc;
+    }
   }
+  const int R = d.all_rows;
   HIP_TRY(hipEventRecord(d.ev[1], d.stream));
   if (mode == kModeFp32Rescue) {
     SweepArgs b = d.base_args();

@@ -96,7 +96,13 @@ struct BatchInput {
   const int64_t* hap_off = nullptr;  // n_haps + 1, byte ranges into hap_bases / hap_flags
   const uint8_t* hap_bases = nullptr;
   const uint8_t* hap_flags = nullptr;  // PD only
+  // Plan only what runs with every pair in a wave of its own need (no read pairing, no suffix
+  // sharing); such a batch holds at most kMaxPairList pairs and runs no other way.
+  bool pairs_only = false;
 };
+
+// Pairs of a batch's one-pair-per-wave list.
+constexpr int64_t kMaxPairList = int64_t(1) << 20;
 
 // A wave holds the n read rows plus at least one identity row that reproduces DP row 0, at
 // R rows per lane over 64 lanes: reads of up to 64 * 8 - 1 = 511 bases.

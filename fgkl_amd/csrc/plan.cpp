@@ -380,7 +380,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         }
       }
     }
-    if (!in.pd && H > 1) {
+    if (!in.pd && H > 1 && !in.pairs_only) {
       // Suffix sharing for plain haplotypes (the reference shares suffixes in its plain kernel
       // only, kernel.rs:53-81); the kernel applies it in the FP32 pass with prefix sharing.
       std::vector<const uint8_t*> hb(static_cast<size_t>(H));
@@ -422,6 +422,14 @@ int build_plan(const BatchInput& in, HostPlan& p) {
   p.region_bw.push_back(int32_t(p.bsweeps.size()));
   p.region_out[size_t(G)] = out_base;
   p.n_pairs = out_base;
+  if (in.pairs_only) {
+    if (p.n_pairs > kMaxPairList) {
+      set_error(kInvalidArgument, "a pairs-only batch holds more than " +
+                                      std::to_string(kMaxPairList) + " pairs");
+      return kInvalidArgument;
+    }
+    return kOk;
+  }
 
   // Work items.  Two reads of a region can share a wave: read A's rows fill lanes [0, L_A), read
   // B's the lanes after, both swept against the same haplotype columns (B lags L_A steps).  A

@@ -100,6 +100,19 @@ int fgkl_pairhmm_compute(fgkl_pairhmm* hmm,
                          double* out, int64_t out_len);
 
 /*
+ * Aggregation of per-call regions (SURVEY's per-call -> batch-aggregator mode; no reference
+ * counterpart).  With max_regions > 1, fgkl_pairhmm_compute_native / fgkl_pdhmm_compute_native
+ * calls that arrive from several threads within max_wait_us of the first (up to max_regions)
+ * run as one batch with prefix and suffix sharing; each call still returns when its own region's
+ * likelihoods are written, and a region the batch cannot take (invalid input) fails only its own
+ * call.  0 turns it off (the default).  The environment variable FGKL_AGGREGATE=<regions>[:<wait
+ * us>] sets it for programs that cannot call this (a JVM with the unchanged Java bindings).
+ * fgkl_aggregate_stats reports the batches run and the regions they held.
+ */
+int fgkl_aggregate_config(int max_regions, int max_wait_us);
+void fgkl_aggregate_stats(int64_t* batches, int64_t* regions);
+
+/*
  * Batched engine (no reference counterpart: the reference computes one region per call on the
  * calling thread, crates/pairhmm/src/lib.rs:377-397).  A batch holds many regions resident in
  * device memory; region g owns reads region_reads[g]..region_reads[g+1] and haplotypes
@@ -130,6 +143,14 @@ int fgkl_batch_create(fgkl_batch** out, int pd, int64_t n_regions,
                       const uint8_t* hap_flags);
 int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_graph,
                    fgkl_run_stats* stats);
+/*
+ * Every (read, haplotype) pair in a wave of its own, without prefix or suffix sharing: more cells,
+ * but no wave walks a read through all of its region's haplotypes, so a batch of a few regions
+ * that cannot fill the GPU finishes sooner than fgkl_batch_run makes it.  The per-call entry
+ * points run this way.  Batches of more than 2^20 pairs run as fgkl_batch_run(mode, 1, 0) does.
+ * Stats carry total_ms, primary_ms, rescue_ms and rescued_pairs.
+ */
+int fgkl_batch_run_pairs(fgkl_batch* batch, int mode, fgkl_run_stats* stats);
 int fgkl_batch_results(fgkl_batch* batch, double* likelihoods, int64_t likelihoods_len);
 int64_t fgkl_batch_pairs(const fgkl_batch* batch);
 /*

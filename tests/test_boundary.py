@@ -9,6 +9,8 @@ import re
 import subprocess
 from pathlib import Path
 
+import threading
+
 import numpy as np
 import pytest
 
@@ -138,6 +140,40 @@ def test_without_a_gpu_the_engine_fails_loudly():
         _call(d, ro, ho)
     with pytest.raises(F.BackendUnavailable):
         F.PairHmm()
+
+
+def test_aggregated_calls_validate_and_fail_like_single_calls():
+    """With aggregation on, each call is still checked on its own thread before it is queued, and a
+    batch that cannot run hands every caller the error of its own region."""
+    d, ro, ho = _good()
+    with pytest.raises(F.InvalidArgument, match="non-negative"):
+        F.aggregate(-1)
+    F.aggregate(4, 2000)
+    try:
+        with pytest.raises(F.InvalidArgument, match="output has 19 elements"):
+            _call(d, ro, ho, out=np.zeros(19))
+        F.compute_native(d.bases, d.quals, d.ins, d.dels, d.gcp, ro, np.zeros(0, np.uint8),
+                         np.array([0], np.int32), False, np.zeros(0))
+        if F.device_count() == 0:
+            b0, r0 = F.aggregate_stats()
+            errors = []
+
+            def worker():
+                try:
+                    _call(d, ro, ho)
+                except F.NativeError as e:
+                    errors.append(str(e))
+
+            threads = [threading.Thread(target=worker) for _ in range(3)]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+            assert len(errors) == 3 and all("no HIP device" in e for e in errors), errors
+            b1, r1 = F.aggregate_stats()
+            assert r1 - r0 == 3 and 1 <= b1 - b0 <= 3
+    finally:
+        F.aggregate(0)
 
 
 # ---- Python mirror validation (lib.rs:1070-1096, pd.rs:438-464) ----------------------------------

@@ -516,6 +516,72 @@ def test_pd_native_entry_matches_reference(gpu):
     assert_rel(out, expected, what="pd native")
 
 
+def test_aggregated_native_calls_match_the_oracle(gpu):
+    """Calls from several threads gathered into shared batches give each caller its own region's
+    likelihoods, in both precisions and for PD, within the precision's bound of the oracle."""
+    plain = [Region.generate(40 + i, 30 + 7 * i, 80 + 11 * i, 6 + i, 200 + 9 * i) for i in range(6)]
+    pdr = [PdRegion.generate(60 + i, 20 + 5 * i, 70 + 9 * i, 5 + i, 180) for i in range(4)]
+    cases = [(False, dp, r) for r in plain for dp in (True, False)]
+    cases += [(True, dp, r) for r in pdr for dp in (True, False)]
+    expected = {}
+    for i, (pd, dp, r) in enumerate(cases):
+        if pd:
+            expected[i] = pd_reference_all(r.region_reads_list(), r.region_pd_haps_list())
+        else:
+            expected[i] = reference_all(r.region_reads_list(), r.region_haps_list())
+    results, errors = {}, []
+
+    def call(i):
+        pd, dp, r = cases[i]
+        ro, ho = _native_arrays(r)
+        out = np.zeros(r.n_pairs())
+        if pd:
+            F.pd_compute_native(r.bases, r.quals, r.ins, r.dels, r.gcp, ro, r.hap_bases,
+                                r.hap_flags, ho, dp, out)
+        else:
+            F.compute_native(r.bases, r.quals, r.ins, r.dels, r.gcp, ro, r.hap_bases, ho, dp, out)
+        results[i] = out
+
+    def worker(idx):
+        try:
+            for i in idx:
+                call(i)
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    b0, r0 = F.aggregate_stats()
+    F.aggregate(8, 20000)
+    try:
+        threads = [threading.Thread(target=worker, args=(list(range(t, len(cases), 6)),))
+                   for t in range(6)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        # an invalid region fails only its own call, with the entry point's own message
+        bad = Region.generate(3, 4, 30, 2, 60)
+        ro, ho = _native_arrays(bad)
+        with pytest.raises(F.InvalidArgument, match="elements but reads x haplotypes"):
+            F.compute_native(bad.bases, bad.quals, bad.ins, bad.dels, bad.gcp, ro,
+                             bad.hap_bases, ho, True, np.zeros(bad.n_pairs() + 1))
+    finally:
+        F.aggregate(0)
+    assert not errors, errors
+    b1, r1 = F.aggregate_stats()
+    assert r1 - r0 == len(cases)
+    assert b1 - b0 < len(cases), "no two calls shared a batch"
+    for i, (pd, dp, r) in enumerate(cases):
+        what = f"case {i} pd={pd} double={dp}"
+        if dp:
+            assert_rel(results[i], expected[i], what=what)
+        else:
+            assert_close(results[i], expected[i], FLOAT_TOL, what)
+    # off again: calls run on their own and the counters stay put
+    pd, dp, r = cases[0]
+    call(0)
+    assert F.aggregate_stats() == (b1, r1)
+
+
 # ---- the batched engine ------------------------------------------------------------------------
 @pytest.mark.parametrize("wl,regions", [(0, 1), (1, 6), (2, 3), (3, 4), (4, 6)])
 def test_batch_workloads_match_the_oracle(gpu, wl, regions):
