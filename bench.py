@@ -250,7 +250,7 @@ def per_call_bench(data, fp64_res, n_regions: int) -> dict:
     return out
 
 
-def threaded_calls(regs, cells, fp64_res, thread_counts=(8, 32)) -> dict:
+def threaded_calls(regs, cells, fp64_res, thread_counts=(1, 8, 32)) -> dict:
     """The same per-region FP32 calls from several caller threads at once, each thread taking every
     T-th region: on their own (each thread's region on its own stream) and gathered into shared
     batches by the aggregator (fgkl_aggregate_config with up to T/2 regions a batch, so that one

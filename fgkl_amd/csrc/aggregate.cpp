@@ -4,9 +4,10 @@
 // until its own region's likelihoods are written.
 //
 // Leader / follower: a queued caller that finds no batch being gathered for its kind, and one of
-// the kind's two batch slots free, becomes the leader: it waits until `max_regions` calls are
-// queued or `max_wait_us` has passed, takes them, runs them in its slot and wakes every caller it
-// served; meanwhile the next queued caller may lead the next batch in the other slot.  The
+// the kind's two batch slots free, becomes the leader.  While the other slot's batch runs it waits
+// until `max_regions` calls are queued, that batch finishes or `max_wait_us` has passed (with both
+// slots free it does not wait); then it takes the queued calls, runs them in its slot and wakes
+// every caller it served, while the next queued caller may lead the next batch in the other slot.  The
 // inputs are read in place: their callers block until their results are written.
 #include <algorithm>
 #include <chrono>
@@ -176,12 +177,13 @@ int aggregate_run(const RegionCall& call) {
   while (!job.done) {
     if (!q.gathering && !(q.busy[0] && q.busy[1]) &&
         std::find(q.pending.begin(), q.pending.end(), &job) != q.pending.end()) {
-      // Lead: gather until the batch is full or the wait is over, take a free batch slot, and
-      // let the next leader gather while this batch runs.
+      // Lead: while the kind's other batch runs, gather until this one is full, the other
+      // finishes or the wait is over (an idle GPU takes what is queued at once); then take a free
+      // slot and let the next leader gather while this batch runs.
       q.gathering = true;
       const auto deadline =
           std::chrono::steady_clock::now() + std::chrono::microseconds(a.max_wait_us);
-      while (int(q.pending.size()) < a.max_regions &&
+      while (int(q.pending.size()) < a.max_regions && (q.busy[0] || q.busy[1]) &&
              a.cv.wait_until(lk, deadline) != std::cv_status::timeout) {
       }
       const size_t take = std::min(q.pending.size(), size_t(std::max(a.max_regions, 1)));
