@@ -176,6 +176,64 @@ SuffixPlan plan_suffixes(const std::vector<const uint8_t*>& bases, const std::ve
   return plan;
 }
 
+// lanes_for(n, R) for every read length and rows per lane, without a division.
+struct LaneTable {
+  int16_t t[kMaxRowsPerLane + 1][kMaxReadLen + 1];
+  LaneTable() {
+    for (int R = 1; R <= kMaxRowsPerLane; ++R)
+      for (int n = 0; n <= kMaxReadLen; ++n) t[R][n] = int16_t(lanes_for(n, R));
+  }
+  int64_t of(int64_t n, int R) const { return t[R][n]; }
+};
+const LaneTable lanes;
+
+// The reads of a region not yet placed in a work item, by length: the longest free read of at
+// most n bases (the latest such read when several have that length) in a few word operations.
+class FreeReads {
+ public:
+  void reset(const int64_t* read_off, int64_t r0, int64_t r1) {
+    std::fill(std::begin(count_), std::end(count_), 0);
+    std::fill(std::begin(bits_), std::end(bits_), 0);
+    for (int64_t r = r0; r < r1; ++r) ++count_[read_off[r + 1] - read_off[r]];
+    int32_t at = 0;
+    for (int n = 0; n <= kMaxReadLen; ++n) {
+      start_[n] = at;
+      at += count_[n];
+      top_[n] = start_[n];
+      if (count_[n]) bits_[n >> 6] |= uint64_t(1) << (n & 63);
+    }
+    reads_.resize(size_t(at));
+    for (int64_t r = r0; r < r1; ++r) reads_[size_t(top_[read_off[r + 1] - read_off[r]]++)] = r;
+    left_ = r1 - r0;
+  }
+  bool empty() const { return left_ == 0; }
+  // The longest free length of at most n (0 if none).
+  int64_t longest_at_most(int64_t n) const {
+    if (n <= 0) return 0;
+    if (n > kMaxReadLen) n = kMaxReadLen;
+    int w = int(n >> 6);
+    uint64_t m = bits_[w] & (~uint64_t(0) >> (63 - (n & 63)));
+    while (!m) {
+      if (--w < 0) return 0;
+      m = bits_[w];
+    }
+    return int64_t(w) * 64 + 63 - __builtin_clzll(m);
+  }
+  // Removes and returns the latest free read of length n (one must be free).
+  int64_t take(int64_t n) {
+    const int64_t r = reads_[size_t(--top_[n])];
+    if (top_[n] == start_[n]) bits_[n >> 6] &= ~(uint64_t(1) << (n & 63));
+    --left_;
+    return r;
+  }
+
+ private:
+  int32_t count_[kMaxReadLen + 1], start_[kMaxReadLen + 1], top_[kMaxReadLen + 1];
+  uint64_t bits_[(kMaxReadLen + 64) / 64];
+  std::vector<int64_t> reads_;
+  int64_t left_ = 0;
+};
+
 struct RegionHaps {
   const BatchInput* in;
   const std::vector<uint32_t>* end_state;
@@ -439,6 +497,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
   // each with the free read that lowers the modelled cost most; the candidates are, for every R,
   // the longest free read that fits beside it.
   constexpr int64_t kStepFixed = 20, kStepPerRow = 19;
+  FreeReads free_reads;
   for (int64_t g = 0; g < G; ++g) {
     int64_t cols = 0, swept = 0, hap_bases = 0;
     const int32_t H = p.region_nhap[size_t(g)];
@@ -474,40 +533,38 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     };
     auto single_cost = [&](int64_t n) {
       const int R = rows_per_lane(n);
-      return item_cost(R, lanes_for(n, R));
+      return item_cost(R, lanes.of(n, R));
     };
     auto pair_cost = [&](int64_t na, int64_t nb) {
-      const int R = rows_per_lane2(na, nb);
-      return R ? item_cost(R, lanes_for(na, R) + lanes_for(nb, R)) : INT64_MAX;
+      for (int R = 1; R <= kMaxRowsPerLane; ++R) {  // rows_per_lane2 through the table
+        const int64_t L = lanes.of(na, R) + lanes.of(nb, R);
+        if (L <= 64) return item_cost(R, L);
+      }
+      return INT64_MAX;
     };
     const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
     auto len = [&](int64_t r) { return in.read_off[r + 1] - in.read_off[r]; };
-    std::multiset<std::pair<int64_t, int64_t>> free;  // (length, read)
-    for (int64_t r = r0; r < r1; ++r) free.emplace(len(r), r);
-    while (!free.empty()) {
-      auto ia = std::prev(free.end());
-      const int64_t a = ia->second, na = ia->first;
-      free.erase(ia);
+    free_reads.reset(in.read_off, r0, r1);
+    while (!free_reads.empty()) {
+      const int64_t na = free_reads.longest_at_most(kMaxReadLen);
+      const int64_t a = free_reads.take(na);
       const int64_t alone = single_cost(na);
-      int64_t best_gain = 0, b = -1;
+      int64_t best_gain = 0, nb = 0, last = 0;
       for (int R = rows_per_lane(na); R <= kMaxRowsPerLane; ++R) {
-        const int64_t spare = 64 - lanes_for(na, R);
+        const int64_t spare = 64 - lanes.of(na, R);
         if (spare <= 0) continue;
         const int64_t max_n = spare * R - 1;  // its rows plus one identity row
-        auto it = free.upper_bound(std::make_pair(max_n, INT64_MAX));
-        if (it == free.begin()) continue;
-        --it;
-        const int64_t gain = alone + single_cost(it->first) - pair_cost(na, it->first);
+        const int64_t n = free_reads.longest_at_most(max_n);
+        // The gain depends on the partner's length only, so a repeated candidate changes nothing.
+        if (n == 0 || n == last) continue;
+        last = n;
+        const int64_t gain = alone + single_cost(n) - pair_cost(na, n);
         if (gain > best_gain) {
           best_gain = gain;
-          b = it->second;
+          nb = n;
         }
       }
-      int64_t nb = 0;
-      if (b >= 0) {
-        nb = len(b);
-        free.erase(free.find(std::make_pair(nb, b)));
-      }
+      const int64_t b = nb > 0 ? free_reads.take(nb) : -1;
       const int R = b >= 0 ? rows_per_lane2(na, nb) : rows_per_lane(na);
       const int64_t L = lanes_for(na, R) + (b >= 0 ? lanes_for(nb, R) : 0);
       auto& bucket = p.bucket_items[R - 1];
