@@ -37,12 +37,16 @@ constexpr uint8_t kSnp = 1, kDelStart = 2, kDelEnd = 4;
 constexpr uint8_t kAltA = 8, kAltC = 16, kAltG = 32, kAltT = 64;
 constexpr uint32_t kNormal = 0, kInsideDel = 1, kAfterDel = 2;
 
-// The per-column word the PD kernel passes between lanes: haplotype byte, accepted alternates,
-// the deletion state of the column in row 1 and in rows >= 2, and its DEL_END flag.
+// The per-column word the PD kernel passes between lanes: haplotype byte (bits 0-7), accepted
+// alternates (8-11), the deletion state of the column in row 1 (12: inside, 13: after) and in
+// rows >= 2 (14: inside, 15: after), its DEL_END flag (16), and bit 18 always set (bit 19 always
+// clear), which identity rows read as "inside a deletion, not after one".  A row finds its two
+// state bits at its own shift: kRow1Shift, kRowShift or kIdentShift.
+constexpr uint32_t kRow1Shift = 12, kRowShift = 14, kIdentShift = 18, kDelEndBit = 16;
 inline uint32_t pd_word(uint8_t base, uint8_t flag, uint32_t st1, uint32_t st2) {
   uint32_t alts = (flag & kSnp) ? ((flag >> 3) & 0xFu) : 0u;  // MatchSet::of (pd_kernel.rs:196-214)
-  return uint32_t(base) | (alts << 8) | (st1 << 12) | (st2 << 14) |
-         (uint32_t((flag & kDelEnd) != 0) << 16);
+  return uint32_t(base) | (alts << 8) | (st1 << kRow1Shift) | (st2 << kRowShift) |
+         (uint32_t((flag & kDelEnd) != 0) << kDelEndBit) | (1u << kIdentShift);
 }
 
 // ---------------------------------------------------------------------------------------------

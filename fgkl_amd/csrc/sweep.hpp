@@ -108,7 +108,9 @@ template <typename T, int R, bool PD> struct Rows {
   T pm[R], px[R], mm[R], im[R], mi[R], md[R], gc[R];
   uint32_t rb[R];
   uint32_t ralt[R];
-  uint32_t kind[R];  // 0: row >= 2, 1: row 1, 2: identity row (or a lane past the read)
+  // Where the row's deletion-state bits sit in a PD column word: kRowShift (rows >= 2),
+  // kRow1Shift (row 1) or kIdentShift (identity rows and lanes past the read).
+  uint32_t sh[R];
 };
 
 template <typename T, int R, bool PD>
@@ -131,7 +133,7 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       rw.im[k] = tab.im[gq];
       rw.rb[k] = b;
       rw.ralt[k] = alt_bit(b);
-      rw.kind[k] = arow == 0 ? 1u : 0u;
+      rw.sh[k] = arow == 0 ? kRow1Shift : kRowShift;
     } else {
       // Identity rows: D->D 1 keeps row 0's C/m; I->I 1 is harmless because the row above an
       // identity row is zero.  Lanes past the read: all zero.
@@ -140,7 +142,7 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       rw.gc[k] = lane < L ? T(1) : T(0);
       rw.rb[k] = 0x100u;
       rw.ralt[k] = 0u;
-      rw.kind[k] = 2u;
+      rw.sh[k] = kIdentShift;
     }
   }
 }
@@ -189,11 +191,37 @@ __device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u, ui
   u.x = and_bits(shr1z(s.x[R - 1]), keep);
   u.y = shr1z(s.y[R - 1]);
   if constexpr (PD) {
-    u.bm = shr1z(s.bm[R - 1]);
-    u.bx = shr1z(s.bx[R - 1]);
+    // Masked like the insertion value: a DEL_END column lets the branch values above reach the
+    // insertion value of the row below, identity rows included.
+    u.bm = and_bits(shr1z(s.bm[R - 1]), keep);
+    u.bx = and_bits(shr1z(s.bx[R - 1]), keep);
     u.by = shr1z(s.by[R - 1]);
   }
 }
+
+// Per-lane bitwise select: a where the mask is all ones, b where it is zero (v_bfi_b32).
+__device__ __forceinline__ float sel_bits(uint32_t k, float a, float b) {
+  const uint32_t x = __builtin_bit_cast(uint32_t, a), y = __builtin_bit_cast(uint32_t, b);
+  return __builtin_bit_cast(float, (x & k) | (y & ~k));
+}
+__device__ __forceinline__ double sel_bits(uint32_t k, double a, double b) {
+  const uint64_t m = (uint64_t(k) << 32) | k;
+  const uint64_t x = __builtin_bit_cast(uint64_t, a), y = __builtin_bit_cast(uint64_t, b);
+  return __builtin_bit_cast(double, (x & m) | (y & ~m));
+}
+
+// All ones if bit `b` of `w` is set, else zero.
+__device__ __forceinline__ uint32_t bit_mask(uint32_t w, uint32_t b) {
+  return uint32_t(__builtin_amdgcn_sbfe(int(w), b, 1));
+}
+
+// max of two non-negative, finite values.  For floats the bit patterns order as unsigned
+// integers, and the integer maximum needs none of the canonicalising moves IEEE mode puts in
+// front of v_max_f32 for operands the compiler cannot prove canonical.
+__device__ __forceinline__ float max_nonneg(float a, float b) {
+  return __builtin_bit_cast(float, max(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b)));
+}
+__device__ __forceinline__ double max_nonneg(double a, double b) { return fmax(a, b); }
 
 // The R cells of one column.  Rows are updated in place: match, deletion and branch values
 // bottom-up (each row still reads the old column of the row above), then insertion values
@@ -204,21 +232,12 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
   const uint32_t w = s.w;
   const uint32_t hb = w & 0xFFu;
   const bool hN = hb == uint32_t('N');
-  // Per row: the column's deletion state (row 1 or later rows) and DEL_END flag.  Identity rows
-  // hold their values as inside a deletion (row 0, branch values included).  Written as selects
-  // of values: a branch here splits the loop body, and the optimiser then folds `match ? pm :
-  // px` below into one load at a selected offset of the row table, which keeps the whole table
-  // in private memory.
-  uint32_t st[PD ? R : 1], de[PD ? R : 1];
-  if constexpr (PD) {
-    const uint32_t st1 = (w >> 12) & 3u, st2 = (w >> 14) & 3u, del_end = (w >> 16) & 1u;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const bool ident = rw.kind[k] == 2u;
-      st[k] = ident ? uint32_t(kInsideDel) : (rw.kind[k] == 1u ? st1 : st2);
-      de[k] = ident ? 0u : del_end;
-    }
-  }
+  // PD: the column's DEL_END flag as a mask, and per row its deletion state as two masks
+  // (inside, after; both zero in NORMAL columns), read at the row's shift; identity rows read
+  // the word's constant bits and hold their values as inside a deletion (row 0, branch values
+  // included).  All values are non-negative, so max(v, b & after) is v outside AFTER_DEL and
+  // max(v, b) inside it, and the states need no compares or selects.
+  const uint32_t de = PD ? bit_mask(w, kDelEndBit) : 0u;
 #pragma unroll
   for (int k = R - 1; k >= 0; --k) {
     const T pdm = k ? s.m[k - 1] : d.m;
@@ -237,20 +256,22 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       s.y[k] = Y;
     } else {
       // pd_reference.rs:62-94 for the column's deletion state.
-      const bool A = st[k] == kAfterDel, I = st[k] == kInsideDel, N = st[k] == kNormal;
+      const uint32_t I = bit_mask(w, rw.sh[k]), A = bit_mask(w, rw.sh[k] + 1);
       const T pdbm = k ? s.bm[k - 1] : d.bm;
       const T pdbx = k ? s.bx[k - 1] : d.bx;
       const T pdby = k ? s.by[k - 1] : d.by;
-      const T dme = A ? fmax(pdbm, pdm) : pdm;
-      const T dxe = A ? fmax(pdbx, pdx) : pdx;
-      const T dye = A ? fmax(pdby, pdy) : pdy;
-      const T lme = A ? fmax(s.bm[k], s.m[k]) : s.m[k];
-      const T lxe = A ? fmax(s.bx[k], s.x[k]) : s.x[k];
-      const T lye = A ? fmax(s.by[k], s.y[k]) : s.y[k];
+      const T dme = max_nonneg(pdm, and_bits(pdbm, A));
+      const T dxe = max_nonneg(pdx, and_bits(pdbx, A));
+      const T dye = max_nonneg(pdy, and_bits(pdby, A));
+      const T lme = max_nonneg(s.m[k], and_bits(s.bm[k], A));
+      const T lxe = max_nonneg(s.x[k], and_bits(s.bx[k], A));
+      const T lye = max_nonneg(s.y[k], and_bits(s.by[k], A));
       const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
-      const T BM = N ? s.m[k] : (I ? s.bm[k] : lme);
-      const T BX = N ? s.x[k] : (I ? s.bx[k] : lxe);
-      const T BY = N ? s.y[k] : (I ? s.by[k] : lye);
+      // NORMAL copies the row's values (lme is m there), AFTER_DEL takes the maxima, INSIDE_DEL
+      // holds the branch.
+      const T BM = sel_bits(I, s.bm[k], lme);
+      const T BX = sel_bits(I, s.bx[k], lxe);
+      const T BY = sel_bits(I, s.by[k], lye);
       s.m[k] = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
       s.y[k] = Y;
       s.bm[k] = BM;
@@ -268,8 +289,8 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       // pd_reference.rs:95-100
       const T pubm = k ? s.bm[k - 1] : u.bm;
       const T pubx = k ? s.bx[k - 1] : u.bx;
-      const T ume = de[k] ? fmax(pubm, pum) : pum;
-      const T uxe = de[k] ? fmax(pubx, pux) : pux;
+      const T ume = max_nonneg(pum, and_bits(pubm, de));
+      const T uxe = max_nonneg(pux, and_bits(pubx, de));
       s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[k]);
     }
   }
@@ -360,7 +381,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     for (int k = 0; k < R; ++k) {
       s.m[k] = T(0);
       s.x[k] = T(0);
-      s.y[k] = (rw.kind[k] == 2u && lane < L) ? init : T(0);
+      s.y[k] = (rw.sh[k] == kIdentShift && lane < L) ? init : T(0);
       if constexpr (PD) {
         s.bm[k] = T(0);
         s.bx[k] = T(0);
