@@ -107,6 +107,19 @@ def cpu_baseline(data, args, fp32_res) -> dict:
     small = first_regions(data, args.cpu_regions)
     ref = batch_reference(small, nthreads=threads)
     d_oracle = np.abs(fp32_res[: small.n_pairs()] - ref)
+    # The reference's pairhmm-bench region on one thread, as that benchmark runs one call.
+    from fgkl_amd.synthetic import Region
+    region = Region.generate(*REFERENCE_BENCH_REGION)
+    rb_times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        port_batch(region, nthreads=1)
+        rb_times.append(time.perf_counter() - t0)
+    rb = {"cells": region.cells(), "threads": 1, "seconds": [round(t, 3) for t in rb_times],
+          "mcells_per_s": round(region.cells() / min(rb_times) / 1e6, 1),
+          "note": "pairhmm-bench's default region (REFERENCE_BENCH_REGION) through the port, "
+                  "float with FP64 rescue, best of 2"}
+    log(f"cpu baseline on the pairhmm-bench region, 1 thread: {rb['mcells_per_s']:.0f} Mcells/s")
     log(f"cpu baseline ({port_backend()}): {sub.n_regions} regions, best of 3 {dt:.2f}s on "
         f"{threads} threads = {sub.cells() / dt / 1e9:.2f} GCUPS; gpu vs port max |d| "
         f"{d_port.max():.3g}, gpu vs oracle max |d| {d_oracle.max():.3g}")
@@ -122,6 +135,7 @@ def cpu_baseline(data, args, fp32_res) -> dict:
         "max_abs_dlog10L_gpu_vs_port": float(d_port.max()),
         "max_abs_dlog10L_gpu_vs_oracle": float(d_oracle.max()),
         "oracle_sample_pairs": small.n_pairs(),
+        "reference_bench_region": rb,
     }
 
 
@@ -294,6 +308,56 @@ def threaded_calls(regs, cells, fp64_res, thread_counts=(1, 8, 32)) -> dict:
             log(f"per-call fp32 x{T} threads {label}: {cells / dt / 1e9:.1f} GCUPS, "
                 f"max |d| vs batch fp64 {d.max():.3g}")
         out[f"threads_{T}"] = row
+    return out
+
+
+REFERENCE_BENCH_REGION = (7, 2000, 150, 64, 500)  # bench.rs:27-33 / pdbench.rs:27-33 defaults
+
+
+def reference_bench() -> dict:
+    """The reference's own throughput benchmarks (pairhmm-bench, bench.rs:54-107; pdhmm-bench,
+    pdbench.rs:56-134) at their default region: one region of 2000 reads of 150 bp against 64
+    haplotypes of 500 bp, seed 7, one compute call per iteration, best of 3 wall times, cells /
+    second, in float and double.  Here the call is the per-region C-ABI entry
+    (fgkl_pairhmm_compute_native / fgkl_pdhmm_compute_native) on host arrays; float results are
+    compared with the engine's double results (the double path is pinned to the oracle by the
+    tests)."""
+    import fgkl_amd as F
+    from fgkl_amd.synthetic import PdRegion, Region
+
+    out = {"region": "seed 7, 2000 reads x 150 bp, 64 haplotypes x 500 bp (the benchmarks' "
+                     "defaults)", "entry": "one per-region call, best of 3"}
+    for name, gen, pd in (("pairhmm", Region.generate, False), ("pdhmm", PdRegion.generate, True)):
+        d = gen(*REFERENCE_BENCH_REGION)
+        ro, ho = d.read_offsets.astype(np.int32), d.hap_offsets.astype(np.int32)
+        cells = d.cells()
+        row, res = {"cells": cells}, {}
+        for prec, dp in (("double", True), ("float", False)):
+            o = np.zeros(d.n_pairs())
+
+            def call():
+                if pd:
+                    F.pd_compute_native(d.bases, d.quals, d.ins, d.dels, d.gcp, ro, d.hap_bases,
+                                        d.hap_flags, ho, dp, o)
+                else:
+                    F.compute_native(d.bases, d.quals, d.ins, d.dels, d.gcp, ro, d.hap_bases, ho,
+                                     dp, o)
+
+            call()
+            best = float("inf")
+            for _ in range(3):
+                t0 = time.perf_counter()
+                call()
+                best = min(best, time.perf_counter() - t0)
+            res[prec] = o.copy()
+            row[prec] = {"ms": round(best * 1e3, 3), "mcells_per_s": round(cells / best / 1e6, 1),
+                         "gcups": round(cells / best / 1e9, 2)}
+        row["max_abs_dlog10L_float_vs_double"] = float(np.abs(res["float"] - res["double"]).max())
+        out[name] = row
+        log(f"reference bench {name}: float {row['float']['ms']} ms "
+            f"({row['float']['gcups']:.0f} GCUPS), double {row['double']['ms']} ms "
+            f"({row['double']['gcups']:.0f} GCUPS), max |float - double| "
+            f"{row['max_abs_dlog10L_float_vs_double']:.3g}")
     return out
 
 
@@ -582,6 +646,8 @@ def main() -> int:
     if not args.no_per_call:
         result["per_call"] = per_call_bench(data, fp32_res if args.no_baseline else fp64,
                                             args.per_call_regions)
+
+    result["reference_bench"] = reference_bench()
 
     if not args.no_adjoint:
         result["adjoint"] = adjoint_bench(args)

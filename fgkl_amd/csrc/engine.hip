@@ -334,6 +334,14 @@ struct DeviceBatch::Impl {
     if (a.scratch) {
       a.scratch_wave_stride = scratch_stride<T>(rows);
       a.node_off = snap_elems<T>(rows);
+      // Every wave of the grid owns a stride of scratch: refuse a grid the batch's scratch
+      // does not cover.
+      const int64_t need = int64_t(grid) * (c.block / 64) * a.scratch_wave_stride * int64_t(sizeof(T));
+      if (need > int64_t(scratch_bytes)) {
+        set_error(kInternal, "a launch needs " + std::to_string(need) +
+                                 " bytes of scratch, the batch holds " + std::to_string(scratch_bytes));
+        return kInternal;
+      }
     }
     hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows), dim3(grid), dim3(c.block),
                        lds_words * 4 * (c.block / 64), stream, a, dev_tab<T>(st));
