@@ -312,6 +312,7 @@ def threaded_calls(regs, cells, fp64_res, thread_counts=(1, 8, 32)) -> dict:
 
 
 REFERENCE_BENCH_REGION = (7, 2000, 150, 64, 500)  # bench.rs:27-33 / pdbench.rs:27-33 defaults
+HC_REGION = (7, 46, 150, 4, 300)  # a typical HaplotypeCaller region (docs/design.md:25)
 
 
 def reference_bench() -> dict:
@@ -358,6 +359,23 @@ def reference_bench() -> dict:
             f"({row['float']['gcups']:.0f} GCUPS), double {row['double']['ms']} ms "
             f"({row['double']['gcups']:.0f} GCUPS), max |float - double| "
             f"{row['max_abs_dlog10L_float_vs_double']:.3g}")
+    # A region of HaplotypeCaller's typical shape (46 reads x 3-4 haplotypes on real data): the
+    # latency of one per-region call, mean over 1000 calls.
+    d = Region.generate(*HC_REGION)
+    ro, ho = d.read_offsets.astype(np.int32), d.hap_offsets.astype(np.int32)
+    o = np.zeros(d.n_pairs())
+    small = {"region": "seed 7, 46 reads x 150 bp, 4 haplotypes x 300 bp", "cells": d.cells()}
+    for prec, dp in (("double", True), ("float", False)):
+        for _ in range(20):
+            F.compute_native(d.bases, d.quals, d.ins, d.dels, d.gcp, ro, d.hap_bases, ho, dp, o)
+        t0 = time.perf_counter()
+        for _ in range(1000):
+            F.compute_native(d.bases, d.quals, d.ins, d.dels, d.gcp, ro, d.hap_bases, ho, dp, o)
+        us = (time.perf_counter() - t0) * 1e3
+        small[prec] = {"us_per_call": round(us, 1), "gcups": round(d.cells() / us / 1e3, 2)}
+    out["hc_region_per_call"] = small
+    log(f"per-call latency, 46 x 4 region: float {small['float']['us_per_call']} us, double "
+        f"{small['double']['us_per_call']} us")
     return out
 
 

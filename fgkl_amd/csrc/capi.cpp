@@ -115,9 +115,8 @@ int compute_region(int pd, int double_precision, int double_fallback, int64_t n_
     return rc;
   }
   const int mode = double_precision ? kModeFp64 : (double_fallback ? kModeFp32Rescue : kModeFp32NoRescue);
-  RunStats st;
-  if (int rc = b->run_latency(mode, &st)) return rc;
-  if (fallbacks) *fallbacks = uint64_t(st.rescued_pairs);
+  if (int rc = b->run_latency(mode, nullptr)) return rc;
+  if (fallbacks) *fallbacks = uint64_t(b->last_rescued());
   return b->results(out, out_len);
 }
 

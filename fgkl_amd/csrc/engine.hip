@@ -163,6 +163,21 @@ struct DeviceBatch::Impl {
   size_t arena_cap = 0;
   void* staging = nullptr;
   size_t staging_cap = 0;
+  // Pinned, grow-only: the rescued count (first 8 bytes) and the results of the last run, which
+  // run_latency copies back in its own stream (out_on_host).
+  void* readback = nullptr;
+  size_t readback_cap = 0;
+  bool out_on_host = false;
+  int64_t last_rescued = 0;
+  int reserve_readback(size_t bytes) {
+    if (bytes <= readback_cap) return kOk;
+    if (readback) HIP_TRY(hipHostFree(readback));
+    readback = nullptr;
+    readback_cap = 0;
+    HIP_TRY(hipHostMalloc(&readback, bytes, hipHostMallocDefault));
+    readback_cap = bytes;
+    return kOk;
+  }
   struct Piece {
     void** dst;
     const void* src;  // host data to upload, or null for device-only arrays
@@ -239,6 +254,7 @@ struct DeviceBatch::Impl {
     free_lazy();
     if (arena) (void)hipFree(arena);
     if (staging) (void)hipHostFree(staging);
+    if (readback) (void)hipHostFree(readback);
     if (scratch) (void)hipFree(scratch);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -459,6 +475,7 @@ int DeviceBatch::assign(const BatchInput& in) {
   Impl& d = *impl_;
   // Work queued with the previous inputs finishes before their memory is reused.
   if (d.stream) HIP_TRY(hipStreamSynchronize(d.stream));
+  d.out_on_host = false;
   d.drop_graphs();
   d.free_lazy();
   d.plan = HostPlan{};
@@ -585,6 +602,7 @@ int DeviceBatch::assign(const BatchInput& in) {
 
 int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
   Impl& d = *impl_;
+  d.out_on_host = false;
   if (mode != kModeFp32Rescue && mode != kModeFp64 && mode != kModeFp64PerRegion &&
       mode != kModeFp32NoRescue) {
     set_error(kInvalidArgument, "unknown run mode");
@@ -721,21 +739,33 @@ int DeviceBatch::run_per_region(RunStats* stats) {
 
 int DeviceBatch::run_latency(int mode, RunStats* stats) {
   Impl& d = *impl_;
+  d.out_on_host = false;
+  d.last_rescued = 0;
   if (d.plan.n_pairs == 0) {
     if (stats) *stats = RunStats{};
     return kOk;
   }
   // Batches too large for the pair list run the batched way.
-  if (!d.all_pairs) return run(mode == kModeFp64PerRegion ? kModeFp64 : mode, 1, 0, stats);
+  if (!d.all_pairs) {
+    RunStats st;
+    const int rc = run(mode == kModeFp64PerRegion ? kModeFp64 : mode, 1, 0, &st);
+    if (rc == kOk) {
+      d.last_rescued = st.rescued_pairs;
+      if (stats) *stats = st;
+    }
+    return rc;
+  }
   if (mode != kModeFp32Rescue && mode != kModeFp64 && mode != kModeFp64PerRegion &&
       mode != kModeFp32NoRescue) {
     set_error(kInvalidArgument, "unknown run mode");
     return kInvalidArgument;
   }
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
+  const size_t out_bytes = sizeof(double) * size_t(d.plan.n_pairs);
+  if (int rc = d.reserve_readback(8 + out_bytes)) return rc;
   hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(d.ev[0], d.stream));
+  if (stats) HIP_TRY(hipEventRecord(d.ev[0], d.stream));
   // A batch whose pairs leave the GPU partly idle runs them in one launch at the largest rows per
   // lane (launches on a stream run one after another, and each would add a pair's sweep to the
   // time to results); a larger one launches each rows-per-lane bucket at its own rows, each with
@@ -762,7 +792,7 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     }
   }
   const int R = d.all_rows;
-  HIP_TRY(hipEventRecord(d.ev[1], d.stream));
+  if (stats) HIP_TRY(hipEventRecord(d.ev[1], d.stream));
   if (mode == kModeFp32Rescue) {
     SweepArgs b = d.base_args();
     b.items = d.all_rescue;
@@ -774,12 +804,16 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     b.scratch = nullptr;
     if (int rc = d.launch<double>(R, b, d.n_all_pairs)) return rc;
   }
-  HIP_TRY(hipEventRecord(d.ev[2], d.stream));
-  int32_t rescued = 0;
+  if (stats) HIP_TRY(hipEventRecord(d.ev[2], d.stream));
+  auto* rb = static_cast<char*>(d.readback);
   if (fp32)
-    HIP_TRY(hipMemcpyAsync(&rescued, d.counters + 16, sizeof(int32_t), hipMemcpyDeviceToHost,
-                           d.stream));
+    HIP_TRY(hipMemcpyAsync(rb, d.counters + 16, sizeof(int32_t), hipMemcpyDeviceToHost, d.stream));
+  HIP_TRY(hipMemcpyAsync(rb + 8, d.out, out_bytes, hipMemcpyDeviceToHost, d.stream));
   HIP_TRY(hipStreamSynchronize(d.stream));
+  int32_t rescued = 0;
+  if (fp32) std::memcpy(&rescued, rb, sizeof(rescued));
+  d.last_rescued = rescued;
+  d.out_on_host = true;
   if (stats) {
     *stats = RunStats{};
     float total = 0.f, first = 0.f;
@@ -795,6 +829,7 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
 
 int DeviceBatch::tune(int share, TuneReport* report) {
   Impl& d = *impl_;
+  d.out_on_host = false;
   const auto t0 = std::chrono::steady_clock::now();
   TuneReport rep;
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
@@ -868,9 +903,20 @@ int DeviceBatch::results(double* out, int64_t n) {
                                     std::to_string(d.plan.n_pairs) + " pairs");
     return kInvalidArgument;
   }
-  if (n) HIP_TRY(hipMemcpy(out, d.out, sizeof(double) * size_t(n), hipMemcpyDeviceToHost));
+  if (!n) return kOk;
+  const size_t bytes = sizeof(double) * size_t(n);
+  if (!d.out_on_host) {
+    // Through pinned memory in the batch's stream, after the work queued on it.
+    if (int rc = d.reserve_readback(8 + bytes)) return rc;
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(d.readback) + 8, d.out, bytes,
+                           hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+  }
+  std::memcpy(out, static_cast<const char*>(d.readback) + 8, bytes);
   return kOk;
 }
+
+int64_t DeviceBatch::last_rescued() const { return impl_->last_rescued; }
 
 int64_t DeviceBatch::n_pairs() const { return impl_->plan.n_pairs; }
 int64_t DeviceBatch::scratch_bytes() const { return int64_t(impl_->scratch_bytes); }

@@ -53,11 +53,14 @@ class DeviceBatch {
   // Computes every pair; `share` enables prefix sharing; `use_graph` replays one captured
   // hipGraph of the whole run.
   int run(int mode, int share, int use_graph, RunStats* stats);
-  // One-region batches: every (read, haplotype) pair in its own wave, one launch at the region's
-  // largest rows per lane, then the FP64 rescue: the shortest device time for the small work of
-  // one per-call region, which leaves most of the GPU idle either way.  No prefix sharing.
-  // Other batches run as run(mode, share, no graph).
+  // Every (read, haplotype) pair in its own wave (one launch for a small batch, else one per
+  // rows-per-lane bucket), then the FP64 rescue: the shortest time to results for batches too
+  // small to fill the GPU.  No prefix sharing.  The results come back to pinned host memory in
+  // the same stream, so results() then only copies them.  Stats are timed only when asked for.
+  // Batches beyond the pair list run as run(mode, share, no graph).
   int run_latency(int mode, RunStats* stats);
+  // FP32 pairs the last run_latency recomputed in FP64.
+  int64_t last_rescued() const;
   // Copies the log10 likelihoods out, region by region, out[region_base + r * H + h].
   int results(double* out, int64_t n);
   int64_t n_pairs() const;
