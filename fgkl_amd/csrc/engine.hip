@@ -55,6 +55,8 @@ constexpr int kBlocks[3] = {64, 128, 256};
 int block_index(int block) { return block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1; }
 
 constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
+// Uploads above this size go straight from the caller's arrays when no staging buffer fits them.
+constexpr size_t kDirectUploadBytes = size_t(64) << 20;
 // Pairs a one-pair-per-wave run takes in a single launch: 256 CUs hold 8192 waves at 32 per CU.
 constexpr int64_t kOneLaunchPairs = 8192;
 
@@ -279,21 +281,43 @@ struct DeviceBatch::Impl {
       arena_cap = layout_bytes;
     }
     size_t upload_end = 0;
-    for (const Piece& pc : pieces)
-      if (pc.src) upload_end = pc.off + pc.bytes;
-    if (upload_end > staging_cap) {
-      if (staging) HIP_TRY(hipHostFree(staging));
-      staging = nullptr;
-      staging_cap = 0;
-      HIP_TRY(hipHostMalloc(&staging, upload_end, hipHostMallocDefault));
-      staging_cap = upload_end;
-    }
     for (const Piece& pc : pieces) {
       *pc.dst = static_cast<char*>(arena) + pc.off;
-      if (pc.src) std::memcpy(static_cast<char*>(staging) + pc.off, pc.src, pc.bytes);
+      if (pc.src) upload_end = pc.off + pc.bytes;
     }
-    if (upload_end)
+    if (upload_end > kDirectUploadBytes && upload_end > staging_cap) {
+      // A large upload with no staging buffer that holds it: the first writes into freshly
+      // pinned pages cost more than the runtime's own staged copies from the caller's memory.
+      for (const Piece& pc : pieces)
+        if (pc.src)
+          HIP_TRY(hipMemcpyAsync(static_cast<char*>(arena) + pc.off, pc.src, pc.bytes,
+                                 hipMemcpyHostToDevice, stream));
+      HIP_TRY(hipStreamSynchronize(stream));  // the caller's arrays may go once assign returns
+    } else if (upload_end) {
+      if (upload_end > staging_cap) {
+        if (staging) HIP_TRY(hipHostFree(staging));
+        staging = nullptr;
+        staging_cap = 0;
+        HIP_TRY(hipHostMalloc(&staging, upload_end, hipHostMallocDefault));
+        staging_cap = upload_end;
+      }
+      // Into staging in chunks of at most 4 MiB, spread over host threads for large uploads.
+      struct Chunk { char* dst; const char* src; size_t bytes; };
+      std::vector<Chunk> chunks;
+      constexpr size_t kChunk = size_t(4) << 20;
+      for (const Piece& pc : pieces) {
+        if (!pc.src) continue;
+        for (size_t o = 0; o < pc.bytes; o += kChunk)
+          chunks.push_back(Chunk{static_cast<char*>(staging) + pc.off + o,
+                                 static_cast<const char*>(pc.src) + o, std::min(kChunk, pc.bytes - o)});
+      }
+      const int nt = host_threads(int64_t(upload_end), int64_t(kChunk) * 4);
+      run_parallel(nt, [&](int t) {
+        for (size_t i = size_t(t); i < chunks.size(); i += size_t(nt))
+          std::memcpy(chunks[i].dst, chunks[i].src, chunks[i].bytes);
+      });
       HIP_TRY(hipMemcpyAsync(arena, staging, upload_end, hipMemcpyHostToDevice, stream));
+    }
     pieces.clear();
     layout_bytes = 0;
     return kOk;
@@ -906,6 +930,12 @@ int DeviceBatch::results(double* out, int64_t n) {
   if (!n) return kOk;
   const size_t bytes = sizeof(double) * size_t(n);
   if (!d.out_on_host) {
+    if (bytes > kDirectUploadBytes / 4 && 8 + bytes > d.readback_cap) {
+      // Large results and no pinned buffer that holds them: straight into the caller's array.
+      HIP_TRY(hipMemcpyAsync(out, d.out, bytes, hipMemcpyDeviceToHost, d.stream));
+      HIP_TRY(hipStreamSynchronize(d.stream));
+      return kOk;
+    }
     // Through pinned memory in the batch's stream, after the work queued on it.
     if (int rc = d.reserve_readback(8 + bytes)) return rc;
     HIP_TRY(hipMemcpyAsync(static_cast<char*>(d.readback) + 8, d.out, bytes,

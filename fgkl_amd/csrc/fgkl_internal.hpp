@@ -1,7 +1,9 @@
 // Internal declarations shared by the host planner, the HIP engine and the C-ABI.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -140,6 +142,25 @@ struct HostPlan {
   int64_t bucket_cells[kMaxRowsPerLane] = {};  // their read bases x haplotype bases
   int64_t bucket_work[kMaxRowsPerLane] = {};   // cells the FP32 pass computes (both sharings)
 };
+
+// Runs fn(0) .. fn(n - 1) on up to n threads (n <= 1: on this one).
+template <typename F>
+void run_parallel(int n, F&& fn) {
+  if (n <= 1) {
+    if (n == 1) fn(0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(size_t(n));
+  for (int t = 0; t < n; ++t) pool.emplace_back([&fn, t] { fn(t); });
+  for (auto& th : pool) th.join();
+}
+
+// Host threads for `work` units, at least `per_thread` units each, at most 16.
+inline int host_threads(int64_t work, int64_t per_thread) {
+  const int64_t hw = std::max<int64_t>(1, int64_t(std::thread::hardware_concurrency()));
+  return int(std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(work / per_thread, hw), 16)));
+}
 
 // Validates the batch and builds its plan.  Returns kOk or kInvalidArgument with a message
 // set through set_error.

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 
 #include "fgkl_internal.hpp"
 
@@ -349,235 +350,293 @@ int build_plan(const BatchInput& in, HostPlan& p) {
   std::vector<uint32_t> end_state(in.pd ? size_t(NH) : 0);
   if (in.pd) {
     p.words.resize(size_t(in.hap_off[NH]));
-    std::vector<uint32_t> st1, st2;
-    for (int64_t h = 0; h < NH; ++h) {
-      const int64_t o = in.hap_off[h], m = in.hap_off[h + 1] - o;
-      st1.resize(size_t(m));
-      st2.resize(size_t(m));
-      const uint32_t e = column_states(in.hap_flags + o, m, kNormal, st1.data());
-      column_states(in.hap_flags + o, m, e, st2.data());
-      end_state[size_t(h)] = e;
-      for (int64_t j = 0; j < m; ++j)
-        p.words[size_t(o + j)] =
-            pd_word(in.hap_bases[o + j], in.hap_flags[o + j], st1[size_t(j)], st2[size_t(j)]);
-    }
+    const int nt = host_threads(NH, 2048);
+    run_parallel(nt, [&](int t) {
+      std::vector<uint32_t> st1, st2;
+      for (int64_t h = NH * t / nt; h < NH * (t + 1) / nt; ++h) {
+        const int64_t o = in.hap_off[h], m = in.hap_off[h + 1] - o;
+        st1.resize(size_t(m));
+        st2.resize(size_t(m));
+        const uint32_t e = column_states(in.hap_flags + o, m, kNormal, st1.data());
+        column_states(in.hap_flags + o, m, e, st2.data());
+        end_state[size_t(h)] = e;
+        for (int64_t j = 0; j < m; ++j)
+          p.words[size_t(o + j)] =
+              pd_word(in.hap_bases[o + j], in.hap_flags[o + j], st1[size_t(j)], st2[size_t(j)]);
+      }
+    });
   }
   RegionHaps rh{&in, &end_state};
 
+  // Output bases, region by region.
   int64_t out_base = 0;
-  std::vector<int64_t> item_costs[kMaxRowsPerLane];
-  std::vector<int64_t> order;
-  std::vector<int64_t> lcp;
-  std::map<int64_t, int32_t> counts;
   for (int64_t g = 0; g < G; ++g) {
-    const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
-    const int64_t h0 = in.region_hap[g], h1 = in.region_hap[g + 1], H = h1 - h0;
     p.region_out[size_t(g)] = out_base;
-    p.region_bw.push_back(int32_t(p.bsweeps.size()));
-    p.region_hap0[size_t(g)] = int32_t(h0);
-    p.region_nhap[size_t(g)] = int32_t(H);
-    out_base += (r1 - r0) * H;
-    for (int64_t r = r0; r < r1; ++r) {
-      p.read_region[size_t(r)] = int32_t(g);
-      p.read_local[size_t(r)] = int32_t(r - r0);
-      p.read_rows[size_t(r)] = rows_per_lane(in.read_off[r + 1] - in.read_off[r]);
-    }
-    if (H == 0) continue;
+    out_base += (in.region_read[g + 1] - in.region_read[g]) * (in.region_hap[g + 1] - in.region_hap[g]);
+  }
+  p.region_out[size_t(G)] = out_base;
+  p.n_pairs = out_base;
+  if (in.pairs_only && p.n_pairs > kMaxPairList) {
+    set_error(kInvalidArgument, "a pairs-only batch holds more than " +
+                                    std::to_string(kMaxPairList) + " pairs");
+    return kInvalidArgument;
+  }
 
-    // Stable sort, as Rust's sort_by (kernel.rs:55, pd_kernel.rs:72-76).
-    order.resize(size_t(H));
-    std::iota(order.begin(), order.end(), h0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t c) { return rh.less(a, c); });
-    lcp.assign(size_t(H), 0);
-    for (int64_t k = 1; k < H; ++k) lcp[size_t(k)] = rh.lcp(order[size_t(k - 1)], order[size_t(k)]);
-
-    // lcp_counts (kernel.rs:556-562) and one scratch slot per distinct start column.
-    counts.clear();
-    std::vector<char> dup(size_t(H), 0);
-    for (int64_t k = 1; k < H; ++k) {
-      const int64_t a = order[size_t(k - 1)], c = order[size_t(k)];
-      dup[size_t(k)] = rh.len(a) == rh.len(c) && lcp[size_t(k)] == rh.len(c);
-      if (!dup[size_t(k)]) counts[lcp[size_t(k)]] += 1;
-    }
-    std::map<int64_t, int32_t> slot_of;
-    for (auto& kv : counts)
-      if (kv.first > 0) slot_of.emplace(kv.first, int32_t(slot_of.size()));
-    std::vector<int32_t> last_len(slot_of.size(), 0);
-    p.max_slots = std::max<int32_t>(p.max_slots, int32_t(slot_of.size()));
-
-    for (int64_t k = 0; k < H; ++k) {
-      const int64_t h = order[size_t(k)];
-      HapPlan& hp = p.plan[size_t(h0 + k)];
-      hp.off = in.hap_off[h];
-      hp.len = int32_t(rh.len(h));
-      hp.lcp = int32_t(lcp[size_t(k)]);
-      hp.src = int32_t(h - h0);
-      hp.dup = dup[size_t(k)];
-      hp.start_slot = -1;
-      hp.start_src_len = 0;
-      hp.snap0 = int32_t(p.snaps.size());
-      hp.nsnap = 0;
-      hp.cut = 0;
-      hp.cut_node = 0;
-      p.max_hap_len = std::max(p.max_hap_len, hp.len);
-      if (hp.dup) continue;
-      const int64_t start = k == 0 ? 0 : lcp[size_t(k)];
-      hp.lcp = int32_t(start);
-      if (k > 0) counts[start] -= 1;  // kernel.rs:248-250
-      if (start > 0) {
-        hp.start_slot = slot_of.at(start);
-        hp.start_src_len = last_len[size_t(hp.start_slot)];
+  // Regions plan independently: each range of regions builds its part (snapshot lists, backward
+  // sweeps and work items with indices local to the part) and writes its regions' and
+  // haplotypes' own entries in place; the parts are then joined in region order, so the plan
+  // does not depend on how the regions were split.
+  struct Part {
+    std::vector<SnapEntry> snaps;
+    std::vector<BwdSweep> bsweeps;
+    std::vector<int32_t> bwrites, region_bw;
+    int32_t max_slots = 0, max_nodes = 0, max_hap_len = 0;
+    int64_t work_nominal = 0, work_prefix = 0, work_forward = 0, work_backward = 0;
+    std::vector<int32_t> bucket_items[kMaxRowsPerLane];
+    int64_t bucket_pairs[kMaxRowsPerLane] = {}, bucket_cells[kMaxRowsPerLane] = {},
+            bucket_work[kMaxRowsPerLane] = {};
+    std::vector<int64_t> item_costs[kMaxRowsPerLane];
+  };
+  auto plan_range = [&](int64_t g0, int64_t g1, Part& q) {
+    std::vector<int64_t> order;
+      std::vector<int64_t> lcp;
+      std::map<int64_t, int32_t> counts;
+      for (int64_t g = g0; g < g1; ++g) {
+      const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
+      const int64_t h0 = in.region_hap[g], h1 = in.region_hap[g + 1], H = h1 - h0;
+      q.region_bw.push_back(int32_t(q.bsweeps.size()));
+      p.region_hap0[size_t(g)] = int32_t(h0);
+      p.region_nhap[size_t(g)] = int32_t(H);
+      for (int64_t r = r0; r < r1; ++r) {
+        p.read_region[size_t(r)] = int32_t(g);
+        p.read_local[size_t(r)] = int32_t(r - r0);
+        p.read_rows[size_t(r)] = rows_per_lane(in.read_off[r + 1] - in.read_off[r]);
       }
-      // Columns in (start, len] some later haplotype starts from (kernel.rs:253-258).
-      for (auto it = slot_of.upper_bound(start); it != slot_of.end() && it->first <= hp.len; ++it) {
-        if (counts[it->first] > 0) {
-          p.snaps.push_back(SnapEntry{int32_t(it->first), it->second});
-          last_len[size_t(it->second)] = hp.len;
-          hp.nsnap += 1;
-        }
+      if (H == 0) continue;
+
+      // Stable sort, as Rust's sort_by (kernel.rs:55, pd_kernel.rs:72-76).
+      order.resize(size_t(H));
+      std::iota(order.begin(), order.end(), h0);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int64_t a, int64_t c) { return rh.less(a, c); });
+      lcp.assign(size_t(H), 0);
+      for (int64_t k = 1; k < H; ++k) lcp[size_t(k)] = rh.lcp(order[size_t(k - 1)], order[size_t(k)]);
+
+      // lcp_counts (kernel.rs:556-562) and one scratch slot per distinct start column.
+      counts.clear();
+      std::vector<char> dup(size_t(H), 0);
+      for (int64_t k = 1; k < H; ++k) {
+        const int64_t a = order[size_t(k - 1)], c = order[size_t(k)];
+        dup[size_t(k)] = rh.len(a) == rh.len(c) && lcp[size_t(k)] == rh.len(c);
+        if (!dup[size_t(k)]) counts[lcp[size_t(k)]] += 1;
       }
-    }
-    if (!in.pd && H > 1 && !in.pairs_only) {
-      // Suffix sharing for plain haplotypes (the reference shares suffixes in its plain kernel
-      // only, kernel.rs:53-81); the kernel applies it in the FP32 pass with prefix sharing.
-      std::vector<const uint8_t*> hb(static_cast<size_t>(H));
-      std::vector<int32_t> hl(static_cast<size_t>(H));
-      std::vector<char> dp(static_cast<size_t>(H));
+      std::map<int64_t, int32_t> slot_of;
+      for (auto& kv : counts)
+        if (kv.first > 0) slot_of.emplace(kv.first, int32_t(slot_of.size()));
+      std::vector<int32_t> last_len(slot_of.size(), 0);
+      q.max_slots = std::max<int32_t>(q.max_slots, int32_t(slot_of.size()));
+
       for (int64_t k = 0; k < H; ++k) {
-        const HapPlan& hp = p.plan[size_t(h0 + k)];
-        hb[size_t(k)] = in.hap_bases + hp.off;
-        hl[size_t(k)] = hp.len;
-        dp[size_t(k)] = char(hp.dup);
-      }
-      const SuffixPlan sp = plan_suffixes(hb, hl, lcp, dp);
-      for (int64_t k = 0; k < H; ++k) {
-        const int32_t d = sp.depth[size_t(k)];
-        if (d > 0) {
-          HapPlan& hp = p.plan[size_t(h0 + k)];
-          hp.cut = hp.len - d;
-          hp.cut_node = sp.node[size_t(k)] - 1;  // node records exclude node 0
+        const int64_t h = order[size_t(k)];
+        HapPlan& hp = p.plan[size_t(h0 + k)];
+        hp.off = in.hap_off[h];
+        hp.len = int32_t(rh.len(h));
+        hp.lcp = int32_t(lcp[size_t(k)]);
+        hp.src = int32_t(h - h0);
+        hp.dup = dup[size_t(k)];
+        hp.start_slot = -1;
+        hp.start_src_len = 0;
+        hp.snap0 = int32_t(q.snaps.size());
+        hp.nsnap = 0;
+        hp.cut = 0;
+        hp.cut_node = 0;
+        q.max_hap_len = std::max(q.max_hap_len, hp.len);
+        if (hp.dup) continue;
+        const int64_t start = k == 0 ? 0 : lcp[size_t(k)];
+        hp.lcp = int32_t(start);
+        if (k > 0) counts[start] -= 1;  // kernel.rs:248-250
+        if (start > 0) {
+          hp.start_slot = slot_of.at(start);
+          hp.start_src_len = last_len[size_t(hp.start_slot)];
+        }
+        // Columns in (start, len] some later haplotype starts from (kernel.rs:253-258).
+        for (auto it = slot_of.upper_bound(start); it != slot_of.end() && it->first <= hp.len; ++it) {
+          if (counts[it->first] > 0) {
+            q.snaps.push_back(SnapEntry{int32_t(it->first), it->second});
+            last_len[size_t(it->second)] = hp.len;
+            hp.nsnap += 1;
+          }
         }
       }
-      for (const auto& sw : sp.sweeps) {
-        const int32_t L = hl[size_t(sw.hap)];
-        BwdSweep b;
-        b.hap = int32_t(h0 + sw.hap);
-        b.right = sw.resume > 0 ? L - sw.resume : L + 1;
-        b.left = L - sw.need;
-        b.start_node = sw.resume > 0 ? sw.start - 1 : -1;
-        b.w0 = int32_t(p.bwrites.size() / 2);
-        b.nw = int32_t(sw.node_of.size());
-        for (const auto& dn : sw.node_of) {  // ascending depth: descending column
-          p.bwrites.push_back(L - dn.first);
-          p.bwrites.push_back(dn.second - 1);
+      if (!in.pd && H > 1 && !in.pairs_only) {
+        // Suffix sharing for plain haplotypes (the reference shares suffixes in its plain kernel
+        // only, kernel.rs:53-81); the kernel applies it in the FP32 pass with prefix sharing.
+        std::vector<const uint8_t*> hb(static_cast<size_t>(H));
+        std::vector<int32_t> hl(static_cast<size_t>(H));
+        std::vector<char> dp(static_cast<size_t>(H));
+        for (int64_t k = 0; k < H; ++k) {
+          const HapPlan& hp = p.plan[size_t(h0 + k)];
+          hb[size_t(k)] = in.hap_bases + hp.off;
+          hl[size_t(k)] = hp.len;
+          dp[size_t(k)] = char(hp.dup);
         }
-        p.bsweeps.push_back(b);
+        const SuffixPlan sp = plan_suffixes(hb, hl, lcp, dp);
+        for (int64_t k = 0; k < H; ++k) {
+          const int32_t d = sp.depth[size_t(k)];
+          if (d > 0) {
+            HapPlan& hp = p.plan[size_t(h0 + k)];
+            hp.cut = hp.len - d;
+            hp.cut_node = sp.node[size_t(k)] - 1;  // node records exclude node 0
+          }
+        }
+        for (const auto& sw : sp.sweeps) {
+          const int32_t L = hl[size_t(sw.hap)];
+          BwdSweep b;
+          b.hap = int32_t(h0 + sw.hap);
+          b.right = sw.resume > 0 ? L - sw.resume : L + 1;
+          b.left = L - sw.need;
+          b.start_node = sw.resume > 0 ? sw.start - 1 : -1;
+          b.w0 = int32_t(q.bwrites.size() / 2);
+          b.nw = int32_t(sw.node_of.size());
+          for (const auto& dn : sw.node_of) {  // ascending depth: descending column
+            q.bwrites.push_back(L - dn.first);
+            q.bwrites.push_back(dn.second - 1);
+          }
+          q.bsweeps.push_back(b);
+        }
+        q.max_nodes = std::max(q.max_nodes, sp.nodes - 1);
       }
-      p.max_nodes = std::max(p.max_nodes, sp.nodes - 1);
+    }
+    q.region_bw.push_back(int32_t(q.bsweeps.size()));
+    if (in.pairs_only) return;
+
+    // Work items.  Two reads of a region can share a wave: read A's rows fill lanes [0, L_A),
+    // read B's the lanes after, both swept against the same haplotype columns (B lags L_A
+    // steps).  A step costs about kStepFixed + kStepPerRow * R VALU instructions (wave-wide
+    // moves, the column word, then R cells), and a region's sweeps take sum(W) + H' (L - 1) steps
+    // (H' sweeps, forward over the columns between the shared prefix and the cut, and backward
+    // over shared suffixes).  Reads are paired greedily, longest first, each with the free read
+    // that lowers the modelled cost most; the candidates are, for every R, the longest free read
+    // that fits beside it.
+    constexpr int64_t kStepFixed = 20, kStepPerRow = 19;
+    FreeReads free_reads;
+    for (int64_t g = g0; g < g1; ++g) {
+      int64_t cols = 0, swept = 0, hap_bases = 0;
+      const int32_t H = p.region_nhap[size_t(g)];
+      for (int32_t k = 0; k < H; ++k) {
+        const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
+        hap_bases += hp.len;
+        if (!hp.dup) {
+          cols += (hp.cut ? hp.cut - 1 : hp.len) - hp.lcp;  // the FP32 pass stops at its cut
+          swept += 1;
+        }
+      }
+      int64_t prefix_cols = 0, fwd_cols = cols, bwd_cols = 0;
+      for (int32_t k = 0; k < H; ++k) {
+        const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
+        if (!hp.dup) prefix_cols += hp.len - hp.lcp;
+      }
+      for (int32_t w = q.region_bw[size_t(g - g0)]; w < q.region_bw[size_t(g - g0) + 1]; ++w) {
+        cols += q.bsweeps[size_t(w)].right - q.bsweeps[size_t(w)].left;
+        bwd_cols += q.bsweeps[size_t(w)].right - q.bsweeps[size_t(w)].left;
+        swept += 1;
+      }
+      {
+        int64_t read_bases = 0;
+        for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r)
+          read_bases += in.read_off[r + 1] - in.read_off[r];
+        q.work_nominal += read_bases * hap_bases;
+        q.work_prefix += read_bases * prefix_cols;
+        q.work_forward += read_bases * fwd_cols;
+        q.work_backward += read_bases * bwd_cols;
+      }
+      auto item_cost = [&](int R, int64_t L) {
+        return (cols + swept * (L - 1)) * (kStepFixed + kStepPerRow * R);
+      };
+      auto single_cost = [&](int64_t n) {
+        const int R = rows_per_lane(n);
+        return item_cost(R, lanes.of(n, R));
+      };
+      auto pair_cost = [&](int64_t na, int64_t nb) {
+        for (int R = 1; R <= kMaxRowsPerLane; ++R) {  // rows_per_lane2 through the table
+          const int64_t L = lanes.of(na, R) + lanes.of(nb, R);
+          if (L <= 64) return item_cost(R, L);
+        }
+        return INT64_MAX;
+      };
+      const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
+      auto len = [&](int64_t r) { return in.read_off[r + 1] - in.read_off[r]; };
+      free_reads.reset(in.read_off, r0, r1);
+      while (!free_reads.empty()) {
+        const int64_t na = free_reads.longest_at_most(kMaxReadLen);
+        const int64_t a = free_reads.take(na);
+        const int64_t alone = single_cost(na);
+        int64_t best_gain = 0, nb = 0, last = 0;
+        for (int R = rows_per_lane(na); R <= kMaxRowsPerLane; ++R) {
+          const int64_t spare = 64 - lanes.of(na, R);
+          if (spare <= 0) continue;
+          const int64_t max_n = spare * R - 1;  // its rows plus one identity row
+          const int64_t n = free_reads.longest_at_most(max_n);
+          // The gain depends on the partner's length only, so a repeated candidate changes nothing.
+          if (n == 0 || n == last) continue;
+          last = n;
+          const int64_t gain = alone + single_cost(n) - pair_cost(na, n);
+          if (gain > best_gain) {
+            best_gain = gain;
+            nb = n;
+          }
+        }
+        const int64_t b = nb > 0 ? free_reads.take(nb) : -1;
+        const int R = b >= 0 ? rows_per_lane2(na, nb) : rows_per_lane(na);
+        const int64_t L = lanes_for(na, R) + (b >= 0 ? lanes_for(nb, R) : 0);
+        auto& bucket = q.bucket_items[R - 1];
+        bucket.push_back(int32_t(a));
+        bucket.push_back(int32_t(b));
+        q.bucket_pairs[R - 1] += int64_t(H) * (b >= 0 ? 2 : 1);
+        q.bucket_cells[R - 1] += (na + nb) * hap_bases;
+        q.bucket_work[R - 1] += (na + nb) * cols;
+        q.item_costs[R - 1].push_back(item_cost(R, L));
+      }
+    }
+  };
+  const int threads = host_threads(G, 256);
+  std::vector<Part> parts(static_cast<size_t>(threads));
+  run_parallel(threads, [&](int t) {
+    plan_range(G * t / threads, G * (t + 1) / threads, parts[size_t(t)]);
+  });
+  std::vector<int64_t> item_costs[kMaxRowsPerLane];
+  for (int t = 0; t < threads; ++t) {
+    Part& q = parts[size_t(t)];
+    const int64_t g0 = G * t / threads, g1 = G * (t + 1) / threads;
+    const int32_t snap_base = int32_t(p.snaps.size()), sweep_base = int32_t(p.bsweeps.size());
+    const int32_t write_base = int32_t(p.bwrites.size() / 2);
+    for (int64_t h = in.region_hap[g0]; h < in.region_hap[g1]; ++h) p.plan[size_t(h)].snap0 += snap_base;
+    for (BwdSweep& b : q.bsweeps) b.w0 += write_base;
+    for (size_t i = 0; i + 1 < q.region_bw.size(); ++i) p.region_bw.push_back(q.region_bw[i] + sweep_base);
+    p.snaps.insert(p.snaps.end(), q.snaps.begin(), q.snaps.end());
+    p.bsweeps.insert(p.bsweeps.end(), q.bsweeps.begin(), q.bsweeps.end());
+    p.bwrites.insert(p.bwrites.end(), q.bwrites.begin(), q.bwrites.end());
+    p.max_slots = std::max(p.max_slots, q.max_slots);
+    p.max_nodes = std::max(p.max_nodes, q.max_nodes);
+    p.max_hap_len = std::max(p.max_hap_len, q.max_hap_len);
+    p.work_nominal += q.work_nominal;
+    p.work_prefix += q.work_prefix;
+    p.work_forward += q.work_forward;
+    p.work_backward += q.work_backward;
+    for (int R = 0; R < kMaxRowsPerLane; ++R) {
+      p.bucket_items[R].insert(p.bucket_items[R].end(), q.bucket_items[R].begin(), q.bucket_items[R].end());
+      item_costs[R].insert(item_costs[R].end(), q.item_costs[R].begin(), q.item_costs[R].end());
+      p.bucket_pairs[R] += q.bucket_pairs[R];
+      p.bucket_cells[R] += q.bucket_cells[R];
+      p.bucket_work[R] += q.bucket_work[R];
     }
   }
   p.region_bw.push_back(int32_t(p.bsweeps.size()));
-  p.region_out[size_t(G)] = out_base;
-  p.n_pairs = out_base;
-  if (in.pairs_only) {
-    if (p.n_pairs > kMaxPairList) {
-      set_error(kInvalidArgument, "a pairs-only batch holds more than " +
-                                      std::to_string(kMaxPairList) + " pairs");
-      return kInvalidArgument;
-    }
-    return kOk;
-  }
+  if (in.pairs_only) return kOk;
 
-  // Work items.  Two reads of a region can share a wave: read A's rows fill lanes [0, L_A), read
-  // B's the lanes after, both swept against the same haplotype columns (B lags L_A steps).  A
-  // step costs about kStepFixed + kStepPerRow * R VALU instructions (wave-wide moves, the column
-  // word, then R cells), and a region's sweeps take sum(W) + H' (L - 1) steps (H' sweeps, forward
-  // over the columns between the shared prefix and the cut, and backward over shared suffixes).  Reads are paired greedily, longest first,
-  // each with the free read that lowers the modelled cost most; the candidates are, for every R,
-  // the longest free read that fits beside it.
-  constexpr int64_t kStepFixed = 20, kStepPerRow = 19;
-  FreeReads free_reads;
-  for (int64_t g = 0; g < G; ++g) {
-    int64_t cols = 0, swept = 0, hap_bases = 0;
-    const int32_t H = p.region_nhap[size_t(g)];
-    for (int32_t k = 0; k < H; ++k) {
-      const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
-      hap_bases += hp.len;
-      if (!hp.dup) {
-        cols += (hp.cut ? hp.cut - 1 : hp.len) - hp.lcp;  // the FP32 pass stops at its cut
-        swept += 1;
-      }
-    }
-    int64_t prefix_cols = 0, fwd_cols = cols, bwd_cols = 0;
-    for (int32_t k = 0; k < H; ++k) {
-      const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
-      if (!hp.dup) prefix_cols += hp.len - hp.lcp;
-    }
-    for (int32_t w = p.region_bw[size_t(g)]; w < p.region_bw[size_t(g) + 1]; ++w) {
-      cols += p.bsweeps[size_t(w)].right - p.bsweeps[size_t(w)].left;
-      bwd_cols += p.bsweeps[size_t(w)].right - p.bsweeps[size_t(w)].left;
-      swept += 1;
-    }
-    {
-      int64_t read_bases = 0;
-      for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r)
-        read_bases += in.read_off[r + 1] - in.read_off[r];
-      p.work_nominal += read_bases * hap_bases;
-      p.work_prefix += read_bases * prefix_cols;
-      p.work_forward += read_bases * fwd_cols;
-      p.work_backward += read_bases * bwd_cols;
-    }
-    auto item_cost = [&](int R, int64_t L) {
-      return (cols + swept * (L - 1)) * (kStepFixed + kStepPerRow * R);
-    };
-    auto single_cost = [&](int64_t n) {
-      const int R = rows_per_lane(n);
-      return item_cost(R, lanes.of(n, R));
-    };
-    auto pair_cost = [&](int64_t na, int64_t nb) {
-      for (int R = 1; R <= kMaxRowsPerLane; ++R) {  // rows_per_lane2 through the table
-        const int64_t L = lanes.of(na, R) + lanes.of(nb, R);
-        if (L <= 64) return item_cost(R, L);
-      }
-      return INT64_MAX;
-    };
-    const int64_t r0 = in.region_read[g], r1 = in.region_read[g + 1];
-    auto len = [&](int64_t r) { return in.read_off[r + 1] - in.read_off[r]; };
-    free_reads.reset(in.read_off, r0, r1);
-    while (!free_reads.empty()) {
-      const int64_t na = free_reads.longest_at_most(kMaxReadLen);
-      const int64_t a = free_reads.take(na);
-      const int64_t alone = single_cost(na);
-      int64_t best_gain = 0, nb = 0, last = 0;
-      for (int R = rows_per_lane(na); R <= kMaxRowsPerLane; ++R) {
-        const int64_t spare = 64 - lanes.of(na, R);
-        if (spare <= 0) continue;
-        const int64_t max_n = spare * R - 1;  // its rows plus one identity row
-        const int64_t n = free_reads.longest_at_most(max_n);
-        // The gain depends on the partner's length only, so a repeated candidate changes nothing.
-        if (n == 0 || n == last) continue;
-        last = n;
-        const int64_t gain = alone + single_cost(n) - pair_cost(na, n);
-        if (gain > best_gain) {
-          best_gain = gain;
-          nb = n;
-        }
-      }
-      const int64_t b = nb > 0 ? free_reads.take(nb) : -1;
-      const int R = b >= 0 ? rows_per_lane2(na, nb) : rows_per_lane(na);
-      const int64_t L = lanes_for(na, R) + (b >= 0 ? lanes_for(nb, R) : 0);
-      auto& bucket = p.bucket_items[R - 1];
-      bucket.push_back(int32_t(a));
-      bucket.push_back(int32_t(b));
-      p.bucket_pairs[R - 1] += int64_t(H) * (b >= 0 ? 2 : 1);
-      p.bucket_cells[R - 1] += (na + nb) * hap_bases;
-      p.bucket_work[R - 1] += (na + nb) * cols;
-      item_costs[R - 1].push_back(item_cost(R, L));
-    }
-  }
-  // Costliest items first, so the persistent grid's tail is short.
-  for (int R = 0; R < kMaxRowsPerLane; ++R) {
+  // Costliest items first, so the persistent grid's tail is short (a bucket per thread).
+  run_parallel(threads > 1 ? kMaxRowsPerLane : 1, [&](int t) {
+   for (int R = threads > 1 ? t : 0; R < (threads > 1 ? t + 1 : kMaxRowsPerLane); ++R) {
     auto& items = p.bucket_items[R];
     const auto& c = item_costs[R];
     std::vector<int32_t> idx(c.size());
@@ -589,7 +648,8 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       sorted[2 * i + 1] = items[2 * size_t(idx[i]) + 1];
     }
     items.swap(sorted);
-  }
+   }
+  });
   return kOk;
 }
 
