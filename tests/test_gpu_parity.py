@@ -622,6 +622,29 @@ def test_aggregated_calls_released_together_share_batches(gpu):
 
 
 # ---- the batched engine ------------------------------------------------------------------------
+def test_a_batch_planned_on_several_host_threads_matches_the_oracle_and_its_parts(gpu):
+    """1200 small regions are planned in ranges on several host threads and joined in region
+    order; the results match the oracle, and every pair has the bits it gets when its regions
+    are planned in batches of 200 (one planning thread each)."""
+    regions = []
+    for i in range(1200):
+        r = Region.generate(500 + i, 3 + i % 4, 40 + i % 50, 2 + i % 3, 90 + i % 40)
+        regions.append((r.region_reads_list(), r.region_haps_list()))
+    data = batch_from_regions(regions)
+    b = F.Batch(data)
+    b.run(F.MODE_FP64, share=True)
+    assert_rel(b.results(), batch_reference(data), what="fp64")
+    b.run(F.MODE_FP32_RESCUE, share=True)
+    whole = b.results()
+    b.close()
+    parts = []
+    for k in range(0, 1200, 200):
+        pb = F.Batch(batch_from_regions(regions[k:k + 200]))
+        pb.run(F.MODE_FP32_RESCUE, share=True)
+        parts.append(pb.results())
+        pb.close()
+    assert np.array_equal(whole, np.concatenate(parts))
+
 @pytest.mark.parametrize("wl,regions", [(0, 1), (1, 6), (2, 3), (3, 4), (4, 6)])
 def test_batch_workloads_match_the_oracle(gpu, wl, regions):
     data = workload(wl, seed=1234 + wl, n_regions=regions)
