@@ -26,6 +26,7 @@ class RunStats(C.Structure):
         ("bucket_ms", C.c_double * 8),
         ("bucket_cells", C.c_int64 * 8),
         ("bucket_work", C.c_int64 * 8),
+        ("dense_rescue_ms", C.c_double),
     ]
 
 

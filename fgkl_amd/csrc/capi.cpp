@@ -306,6 +306,7 @@ int fgkl_batch_run(fgkl_batch* batch, int mode, int share_prefixes, int use_grap
       stats->bucket_cells[i] = st.bucket_cells[i];
       stats->bucket_work[i] = st.bucket_work[i];
     }
+    stats->dense_rescue_ms = st.dense_rescue_ms;
   }
   return rc;
 }

@@ -55,6 +55,7 @@ constexpr int kBlocks[3] = {64, 128, 256};
 int block_index(int block) { return block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1; }
 
 constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
+constexpr int kCounters = 40;  // see Impl::counters
 // Uploads above this size go straight from the caller's arrays when no staging buffer fits them.
 constexpr size_t kDirectUploadBytes = size_t(64) << 20;
 // Pairs a one-pair-per-wave run takes in a single launch: 256 CUs hold 8192 waves at 32 per CU.
@@ -206,7 +207,11 @@ struct DeviceBatch::Impl {
   BwdSweep* bsweeps = nullptr;
   int32_t* bwrites = nullptr;
   int32_t* region_bw = nullptr;
-  int32_t* counters = nullptr;  // [0,8) main, [8,16) rescue, [16,24) rescue counts
+  // Work-queue heads and counts per rows-per-lane bucket: [0,8) main, [8,16) rescue,
+  // [16,24) rescue counts, [24,32) dense-read counts, [32,40) dense-rescue heads.
+  int32_t* counters = nullptr;
+  uint8_t* dense_flags = nullptr;              // per read, written by every FP32 main pass
+  int32_t* dense_items[kMaxRowsPerLane] = {};  // per bucket, (read, -1), up to 2 per item
   double* out = nullptr;
   void* scratch = nullptr;
   size_t scratch_bytes = 0, scratch_cap = 0;
@@ -441,6 +446,11 @@ struct DeviceBatch::Impl {
       a.rescue = 1;
       a.rescue_pairs = rescue_pairs[R - 1];
       a.rescue_count = counters + 16 + (R - 1);
+      if (!pd) {  // PD batches rescue pair by pair (see enqueue)
+        a.dense = dense_flags;
+        a.dense_items = dense_items[R - 1];
+        a.dense_count = counters + 24 + (R - 1);
+      }
     }
     return a;
   }
@@ -450,7 +460,7 @@ struct DeviceBatch::Impl {
 
 int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
   // Reset the work-queue heads and rescue counts with a kernel (graph-safe).
-  hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, stream, counters, 24, 0);
+  hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, stream, counters, kCounters, 0);
   HIP_TRY(hipGetLastError());
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
   // The primary sweep, one launch per rows-per-lane bucket, each bracketed by events when the
@@ -467,10 +477,30 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
     if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1) + 1], stream));
   }
   if (timed) HIP_TRY(hipEventRecord(ev[1], stream));
+  if (fp32 && mode == kModeFp32Rescue) {
+    // Dense reads first: each against every haplotype of its region, FP64 with prefix sharing,
+    // filling only its lost pairs.  The grid is the FP64 main pass's, which the scratch is sized
+    // for; its waves take the dense reads from a queue and the ones past the count exit at once.
+    // PD batches skip this: their haplotypes rarely share prefixes, and one wave walking a read
+    // through every haplotype measured slower than a wave per lost pair.
+    for (int R = 1; R <= kMaxRowsPerLane && !pd; ++R) {
+      if (!bucket_n[R - 1]) continue;
+      SweepArgs a = base_args();
+      a.items = dense_items[R - 1];
+      a.n_items = 0;
+      a.n_items_dev = counters + 24 + (R - 1);
+      a.counter = counters + 32 + (R - 1);
+      a.share = 1;
+      a.nan_only = 1;
+      if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
+    }
+    if (timed) HIP_TRY(hipEventRecord(ev[3], stream));
+  }
   if (fp32) {
     for (int R = 1; R <= kMaxRowsPerLane && mode == kModeFp32Rescue; ++R) {
       if (!bucket_n[R - 1]) continue;
       SweepArgs a = base_args();
+      a.dense = pd ? nullptr : dense_flags;  // pairs of dense reads are done
       a.items = rescue_pairs[R - 1];
       a.n_items = 0;
       a.n_items_dev = counters + 16 + (R - 1);
@@ -555,7 +585,11 @@ int DeviceBatch::assign(const BatchInput& in) {
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.rescue_pairs[R - 1], static_cast<const int32_t*>(nullptr),
             size_t(2 * d.bucket_pairs[R - 1]));
-  d.place(&d.counters, static_cast<const int32_t*>(nullptr), 32);
+  d.place(&d.counters, static_cast<const int32_t*>(nullptr), kCounters);
+  d.place(&d.dense_flags, static_cast<const uint8_t*>(nullptr), size_t(NR));
+  for (int R = 1; R <= kMaxRowsPerLane; ++R)
+    d.place(&d.dense_items[R - 1], static_cast<const int32_t*>(nullptr),
+            size_t(4 * d.bucket_n[R - 1]));
   d.place(&d.out, static_cast<const double*>(nullptr), size_t(p.n_pairs));
   std::vector<int32_t> all;
   d.n_all_pairs = 0;
@@ -682,6 +716,11 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
       HIP_TRY(hipEventElapsedTime(&first, d.ev[0], d.ev[1]));
       stats->primary_ms = first;
       stats->rescue_ms = total - first;
+      if (mode == kModeFp32Rescue) {
+        float dense = 0.f;
+        HIP_TRY(hipEventElapsedTime(&dense, d.ev[1], d.ev[3]));
+        stats->dense_rescue_ms = dense;
+      }
       for (int R = 1; R <= kMaxRowsPerLane; ++R) {
         if (!d.bucket_n[R - 1]) continue;
         float ms = 0.f;
@@ -787,7 +826,7 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
   const size_t out_bytes = sizeof(double) * size_t(d.plan.n_pairs);
   if (int rc = d.reserve_readback(8 + out_bytes)) return rc;
-  hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
+  hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, kCounters, 0);
   HIP_TRY(hipGetLastError());
   if (stats) HIP_TRY(hipEventRecord(d.ev[0], d.stream));
   // A batch whose pairs leave the GPU partly idle runs them in one launch at the largest rows per
@@ -865,7 +904,7 @@ int DeviceBatch::tune(int share, TuneReport* report) {
     for (size_t i = 0; i < cands.size(); ++i) {
       double ms = 1e30;
       for (int rep_i = 0; rep_i < 2; ++rep_i) {
-        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, 24, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, kCounters, 0);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(d.ev[0], d.stream));
         if (int rc = d.launch<float>(R, a, d.bucket_n[R - 1], cands[i])) return rc;

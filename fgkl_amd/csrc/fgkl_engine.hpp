@@ -23,6 +23,7 @@ struct RunStats {
   double bucket_ms[kMaxRowsPerLane] = {};  // primary sweep launch of each rows-per-lane bucket
   int64_t bucket_cells[kMaxRowsPerLane] = {};  // DP cells (read x haplotype bases) it covers
   int64_t bucket_work[kMaxRowsPerLane] = {};   // cells it computes with prefix + suffix sharing
+  double dense_rescue_ms = 0;  // of rescue_ms: dense reads rerun in FP64 with prefix sharing
 };
 
 // How the FP32 sweep of one rows-per-lane bucket is launched.  Every configuration computes

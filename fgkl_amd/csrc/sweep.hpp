@@ -79,6 +79,15 @@ struct SweepArgs {
   int32_t* rescue_pairs;
   int32_t* rescue_count;
   int32_t rescue;
+  // Dense rescue (lib.rs:488-520): a main-pass FP32 launch sets dense[read] to 1 for a read lost
+  // against a third or more of its region's haplotypes (0 otherwise) and lists it in
+  // dense_items; an FP64 launch with `nan_only` reruns those reads against every haplotype with
+  // prefix sharing and writes only the results still NaN; a pair-mode rescue with `dense` set
+  // skips their pairs.
+  uint8_t* dense;
+  int32_t* dense_items;
+  int32_t* dense_count;
+  int32_t nan_only;
   int32_t lds_words;           // LDS words per wave for the staged haplotype
   // Suffix sharing (FP32 plain main pass only, when `suffix` is set): the backward sweeps of
   // region g are bsweeps[region_bw[g] .. region_bw[g + 1]) with node writes in bwrites; node
@@ -679,6 +688,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       rb = -1;
       k_begin = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
       k_end = k_begin + 1;
+      if (a.dense && __builtin_amdgcn_readfirstlane(a.dense[ra])) continue;
     } else {
       rb = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
       k_begin = 0;
@@ -760,6 +770,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     }
 
     double prev_a = 0.0, prev_b = 0.0;
+    int lost_a = 0, lost_b = 0;
     for (int k = k_begin; k < k_end; ++k) {
       HapPlan p = a.plan[hap0 + k];
       p.len = __builtin_amdgcn_readfirstlane(p.len);
@@ -855,15 +866,19 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         res_a = finish(raw_a);
         res_b = finish(raw_b);
       }
+      lost_a += res_a != res_a;
+      lost_b += res_b != res_b;
       if (lane == 0) {
-        a.out[out_a + p.src] = res_a;
+        double* oa = a.out + out_a + p.src;
+        if (!a.nan_only || *oa != *oa) *oa = res_a;
         if (a.rescue && res_a != res_a) {
           const int idx = atomicAdd(a.rescue_count, 1);
           a.rescue_pairs[2 * idx] = ra;
           a.rescue_pairs[2 * idx + 1] = k;
         }
         if (two) {
-          a.out[out_b + p.src] = res_b;
+          double* ob = a.out + out_b + p.src;
+          if (!a.nan_only || *ob != *ob) *ob = res_b;
           if (a.rescue && res_b != res_b) {
             const int idx = atomicAdd(a.rescue_count, 1);
             a.rescue_pairs[2 * idx] = rb;
@@ -873,6 +888,25 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       }
       prev_a = res_a;
       prev_b = res_b;
+    }
+    // A main-pass item has run its reads against every haplotype: mark the dense ones.
+    if (a.dense && !a.pair_mode && lane == 0) {
+      const bool da = lost_a > 0 && 3 * lost_a >= H;
+      a.dense[ra] = da;
+      if (da) {
+        const int idx = atomicAdd(a.dense_count, 1);
+        a.dense_items[2 * idx] = ra;
+        a.dense_items[2 * idx + 1] = -1;
+      }
+      if (two) {
+        const bool db = lost_b > 0 && 3 * lost_b >= H;
+        a.dense[rb] = db;
+        if (db) {
+          const int idx = atomicAdd(a.dense_count, 1);
+          a.dense_items[2 * idx] = rb;
+          a.dense_items[2 * idx + 1] = -1;
+        }
+      }
     }
   }
 }

@@ -133,6 +133,8 @@ typedef struct fgkl_run_stats {
   double bucket_ms[8];    /* primary sweep launch per rows-per-lane bucket R=1..8 (eager runs) */
   int64_t bucket_cells[8]; /* DP cells (read bases x haplotype bases) each of those launches covers */
   int64_t bucket_work[8];  /* DP cells each computes with prefix and suffix sharing (FP32 pass) */
+  double dense_rescue_ms;  /* part of rescue_ms spent on reads lost against a third or more of
+                              their haplotypes, rerun in FP64 with prefix sharing (eager runs) */
 } fgkl_run_stats;
 
 int fgkl_batch_create(fgkl_batch** out, int pd, int64_t n_regions,
