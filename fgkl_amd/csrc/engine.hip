@@ -211,7 +211,8 @@ struct DeviceBatch::Impl {
   // [16,24) rescue counts, [24,32) dense-read counts, [32,40) dense-rescue heads.
   int32_t* counters = nullptr;
   uint8_t* dense_flags = nullptr;              // per read, written by every FP32 main pass
-  int32_t* dense_items[kMaxRowsPerLane] = {};  // per bucket, (read, -1), up to 2 per item
+  int32_t* dense_items[kMaxRowsPerLane] = {};  // (read, -1) by the read's own rows per lane
+  int64_t rows_reads[kMaxRowsPerLane] = {};    // reads of each rows per lane
   double* out = nullptr;
   void* scratch = nullptr;
   size_t scratch_bytes = 0, scratch_cap = 0;
@@ -396,10 +397,10 @@ struct DeviceBatch::Impl {
 
   // Scratch bytes a launch of bucket R with configuration `c` may touch.
   template <typename T>
-  int64_t scratch_need(int R, LaunchCfg c) {
+  int64_t scratch_need(int R, LaunchCfg c, int64_t max_items = -1) {
     const int rows = c.rows ? c.rows : R;
     int grid = 0;
-    if (grid_waves<T>(c, rows, bucket_n[R - 1], &grid)) return -1;
+    if (grid_waves<T>(c, rows, max_items < 0 ? bucket_n[R - 1] : max_items, &grid)) return -1;
     return int64_t(grid) * (c.block / 64) * scratch_stride<T>(rows) * int64_t(sizeof(T));
   }
   // Per wave: the snapshot slots, then (FP32 plain) the suffix-sharing node records.
@@ -448,8 +449,8 @@ struct DeviceBatch::Impl {
       a.rescue_count = counters + 16 + (R - 1);
       if (!pd) {  // PD batches rescue pair by pair (see enqueue)
         a.dense = dense_flags;
-        a.dense_items = dense_items[R - 1];
-        a.dense_count = counters + 24 + (R - 1);
+        for (int q = 0; q < kMaxRowsPerLane; ++q) a.dense_items[q] = dense_items[q];
+        a.dense_count = counters + 24;
       }
     }
     return a;
@@ -484,7 +485,7 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
     // PD batches skip this: their haplotypes rarely share prefixes, and one wave walking a read
     // through every haplotype measured slower than a wave per lost pair.
     for (int R = 1; R <= kMaxRowsPerLane && !pd; ++R) {
-      if (!bucket_n[R - 1]) continue;
+      if (!rows_reads[R - 1]) continue;
       SweepArgs a = base_args();
       a.items = dense_items[R - 1];
       a.n_items = 0;
@@ -492,7 +493,7 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
       a.counter = counters + 32 + (R - 1);
       a.share = 1;
       a.nan_only = 1;
-      if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
+      if (int rc = launch<double>(R, a, rows_reads[R - 1])) return rc;
     }
     if (timed) HIP_TRY(hipEventRecord(ev[3], stream));
   }
@@ -587,9 +588,12 @@ int DeviceBatch::assign(const BatchInput& in) {
             size_t(2 * d.bucket_pairs[R - 1]));
   d.place(&d.counters, static_cast<const int32_t*>(nullptr), kCounters);
   d.place(&d.dense_flags, static_cast<const uint8_t*>(nullptr), size_t(NR));
+  std::fill(std::begin(d.rows_reads), std::end(d.rows_reads), 0);
+  if (!in.pd && !in.pairs_only)
+    for (int64_t r = 0; r < NR; ++r) d.rows_reads[p.read_rows[size_t(r)] - 1] += 1;
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.dense_items[R - 1], static_cast<const int32_t*>(nullptr),
-            size_t(4 * d.bucket_n[R - 1]));
+            size_t(2 * d.rows_reads[R - 1]));
   d.place(&d.out, static_cast<const double*>(nullptr), size_t(p.n_pairs));
   std::vector<int32_t> all;
   d.n_all_pairs = 0;
@@ -639,6 +643,9 @@ int DeviceBatch::assign(const BatchInput& in) {
   int64_t need = 0;
   if (p.max_slots > 0 || p.max_nodes > 0) {
     for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      // The dense FP64 rescue of the reads with R rows per lane.
+      if (d.rows_reads[R - 1])
+        need = std::max(need, d.scratch_need<double>(R, LaunchCfg{}, d.rows_reads[R - 1]));
       if (!d.bucket_n[R - 1]) continue;
       for (const LaunchCfg& c : d.candidates(R, true))
         need = std::max(need, d.scratch_need<float>(R, c));

@@ -85,8 +85,8 @@ struct SweepArgs {
   // prefix sharing and writes only the results still NaN; a pair-mode rescue with `dense` set
   // skips their pairs.
   uint8_t* dense;
-  int32_t* dense_items;
-  int32_t* dense_count;
+  int32_t* dense_items[kMaxRowsPerLane];  // by the read's own rows per lane
+  int32_t* dense_count;                   // kMaxRowsPerLane counts
   int32_t nan_only;
   int32_t lds_words;           // LDS words per wave for the staged haplotype
   // Suffix sharing (FP32 plain main pass only, when `suffix` is set): the backward sweeps of
@@ -889,22 +889,25 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       prev_a = res_a;
       prev_b = res_b;
     }
-    // A main-pass item has run its reads against every haplotype: mark the dense ones.
+    // A main-pass item has run its reads against every haplotype: mark the dense ones, listed
+    // by their own rows per lane (a read paired into a wider bucket reruns alone at its own).
     if (a.dense && !a.pair_mode && lane == 0) {
       const bool da = lost_a > 0 && 3 * lost_a >= H;
       a.dense[ra] = da;
       if (da) {
-        const int idx = atomicAdd(a.dense_count, 1);
-        a.dense_items[2 * idx] = ra;
-        a.dense_items[2 * idx + 1] = -1;
+        const int q = (na + 64) / 64 - 1;
+        const int idx = atomicAdd(a.dense_count + q, 1);
+        a.dense_items[q][2 * idx] = ra;
+        a.dense_items[q][2 * idx + 1] = -1;
       }
       if (two) {
         const bool db = lost_b > 0 && 3 * lost_b >= H;
         a.dense[rb] = db;
         if (db) {
-          const int idx = atomicAdd(a.dense_count, 1);
-          a.dense_items[2 * idx] = rb;
-          a.dense_items[2 * idx + 1] = -1;
+          const int q = (nb + 64) / 64 - 1;
+          const int idx = atomicAdd(a.dense_count + q, 1);
+          a.dense_items[q][2 * idx] = rb;
+          a.dense_items[q][2 * idx + 1] = -1;
         }
       }
     }
