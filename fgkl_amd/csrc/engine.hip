@@ -212,7 +212,8 @@ struct DeviceBatch::Impl {
   int32_t* counters = nullptr;
   uint8_t* dense_flags = nullptr;              // per read, written by every FP32 main pass
   int32_t* dense_items[kMaxRowsPerLane] = {};  // (read, -1) by the read's own rows per lane
-  int64_t rows_reads[kMaxRowsPerLane] = {};    // reads of each rows per lane
+  int64_t rows_reads[kMaxRowsPerLane] = {};    // reads of each rows per lane (plain batches)
+  int64_t rows_pairs[kMaxRowsPerLane] = {};    // pairs of those reads (every batch)
   double* out = nullptr;
   void* scratch = nullptr;
   size_t scratch_bytes = 0, scratch_cap = 0;
@@ -445,8 +446,9 @@ struct DeviceBatch::Impl {
     a.suffix = fp32 && share && !pd && !plan.bsweeps.empty();
     if (fp32) {
       a.rescue = 1;
-      a.rescue_pairs = rescue_pairs[R - 1];
-      a.rescue_count = counters + 16 + (R - 1);
+      for (int q = 0; q < kMaxRowsPerLane; ++q) a.rescue_pairs[q] = rescue_pairs[q];
+      a.rescue_count = counters + 16;
+      a.rescue_by_rows = 1;
       if (!pd) {  // PD batches rescue pair by pair (see enqueue)
         a.dense = dense_flags;
         for (int q = 0; q < kMaxRowsPerLane; ++q) a.dense_items[q] = dense_items[q];
@@ -498,8 +500,9 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
     if (timed) HIP_TRY(hipEventRecord(ev[3], stream));
   }
   if (fp32) {
+    // Lost pairs by their read's own rows per lane, a wave each.
     for (int R = 1; R <= kMaxRowsPerLane && mode == kModeFp32Rescue; ++R) {
-      if (!bucket_n[R - 1]) continue;
+      if (!rows_pairs[R - 1]) continue;
       SweepArgs a = base_args();
       a.dense = pd ? nullptr : dense_flags;  // pairs of dense reads are done
       a.items = rescue_pairs[R - 1];
@@ -510,7 +513,7 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
       a.share = 0;
       a.scratch = nullptr;
       // The grid is sized for the worst case; waves beyond the rescued count exit at once.
-      if (int rc = launch<double>(R, a, bucket_pairs[R - 1])) return rc;
+      if (int rc = launch<double>(R, a, rows_pairs[R - 1])) return rc;
     }
   }
   return kOk;
@@ -583,9 +586,14 @@ int DeviceBatch::assign(const BatchInput& in) {
     d.bucket_work[R - 1] = p.bucket_work[R - 1];
     d.place(&d.bucket_items[R - 1], items.data(), items.size());
   }
+  // Pairs of the reads of each rows per lane: the capacity of the rescue lists.
+  std::fill(std::begin(d.rows_pairs), std::end(d.rows_pairs), 0);
+  if (!in.pairs_only)  // pair runs keep their own rescue list
+    for (int64_t r = 0; r < NR; ++r)
+      d.rows_pairs[p.read_rows[size_t(r)] - 1] += p.region_nhap[size_t(p.read_region[size_t(r)])];
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.rescue_pairs[R - 1], static_cast<const int32_t*>(nullptr),
-            size_t(2 * d.bucket_pairs[R - 1]));
+            size_t(2 * d.rows_pairs[R - 1]));
   d.place(&d.counters, static_cast<const int32_t*>(nullptr), kCounters);
   d.place(&d.dense_flags, static_cast<const uint8_t*>(nullptr), size_t(NR));
   std::fill(std::begin(d.rows_reads), std::end(d.rows_reads), 0);
@@ -854,7 +862,7 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     a.scratch = nullptr;
     if (fp32) {
       a.rescue = 1;
-      a.rescue_pairs = d.all_rescue;
+      a.rescue_pairs[0] = d.all_rescue;
       a.rescue_count = d.counters + 16;
       if (int rc = d.launch<float>(R, a, n, LaunchCfg{R, 256, false})) return rc;
     } else {

@@ -76,8 +76,11 @@ struct SweepArgs {
   void* scratch;
   int64_t scratch_wave_stride;  // elements of T per wave
   double* out;
-  int32_t* rescue_pairs;
+  // FP32 pairs lost to underflow: with `rescue_by_rows`, listed by the read's own rows per lane
+  // (rescue_pairs[R - 1], count rescue_count[R - 1]), else all in rescue_pairs[0].
+  int32_t* rescue_pairs[kMaxRowsPerLane];
   int32_t* rescue_count;
+  int32_t rescue_by_rows;
   int32_t rescue;
   // Dense rescue (lib.rs:488-520): a main-pass FP32 launch sets dense[read] to 1 for a read lost
   // against a third or more of its region's haplotypes (0 otherwise) and lists it in
@@ -872,17 +875,19 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         double* oa = a.out + out_a + p.src;
         if (!a.nan_only || *oa != *oa) *oa = res_a;
         if (a.rescue && res_a != res_a) {
-          const int idx = atomicAdd(a.rescue_count, 1);
-          a.rescue_pairs[2 * idx] = ra;
-          a.rescue_pairs[2 * idx + 1] = k;
+          const int q = a.rescue_by_rows ? (na + 64) / 64 - 1 : 0;
+          const int idx = atomicAdd(a.rescue_count + q, 1);
+          a.rescue_pairs[q][2 * idx] = ra;
+          a.rescue_pairs[q][2 * idx + 1] = k;
         }
         if (two) {
           double* ob = a.out + out_b + p.src;
           if (!a.nan_only || *ob != *ob) *ob = res_b;
           if (a.rescue && res_b != res_b) {
-            const int idx = atomicAdd(a.rescue_count, 1);
-            a.rescue_pairs[2 * idx] = rb;
-            a.rescue_pairs[2 * idx + 1] = k;
+            const int q = a.rescue_by_rows ? (nb + 64) / 64 - 1 : 0;
+            const int idx = atomicAdd(a.rescue_count + q, 1);
+            a.rescue_pairs[q][2 * idx] = rb;
+            a.rescue_pairs[q][2 * idx + 1] = k;
           }
         }
       }
