@@ -176,6 +176,32 @@ def test_aggregated_calls_validate_and_fail_like_single_calls():
         F.aggregate(0)
 
 
+@pytest.mark.skipif(F.device_count() > 0, reason="checks the no-GPU host")
+def test_the_environment_turns_aggregation_on_for_unchanged_callers():
+    """FGKL_AGGREGATE=<regions>[:<wait us>] routes per-region calls through the aggregator in a
+    process that never calls fgkl_aggregate_config (a JVM with the unchanged bindings)."""
+    import os
+    import subprocess
+    import sys
+
+    code = (
+        "import numpy as np, fgkl_amd as F\n"
+        "from fgkl_amd.synthetic import Region\n"
+        "d = Region.generate(3, 4, 30, 2, 60)\n"
+        "ro, ho = d.read_offsets.astype(np.int32), d.hap_offsets.astype(np.int32)\n"
+        "try:\n"
+        "    F.compute_native(d.bases, d.quals, d.ins, d.dels, d.gcp, ro, d.hap_bases, ho, False,"
+        " np.zeros(d.n_pairs()))\n"
+        "except F.NativeError as e:\n"
+        "    assert 'no HIP device' in str(e), e\n"
+        "print(F.aggregate_stats())\n")
+    for env, want in (("4:100", "(1, 1)"), ("", "(0, 0)")):
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                             env={**os.environ, "FGKL_AGGREGATE": env}, timeout=120)
+        assert out.returncode == 0, out.stderr
+        assert out.stdout.strip().splitlines()[-1] == want, (env, out.stdout)
+
+
 # ---- Python mirror validation (lib.rs:1070-1096, pd.rs:438-464) ----------------------------------
 def test_mirror_validates_like_the_reference_before_any_device_work():
     good = F.ReadRef(b"ACGT", [30] * 4, [45] * 4, [45] * 4, [10] * 4)
