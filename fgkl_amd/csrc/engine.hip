@@ -159,6 +159,12 @@ struct DeviceBatch::Impl {
   HostPlan plan;
   bool pd = false;
   hipStream_t stream = nullptr;
+  // Buckets with little work (side[R - 1]) run their primary sweep on a second stream beside
+  // the large ones, from their own part of the scratch (side_scratch_off bytes in).
+  hipStream_t side_stream = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool side[kMaxRowsPerLane] = {};
+  size_t side_scratch_off = 0;
   std::vector<void*> allocs;  // per-region mode lists, allocated on first use
   // The batch's arrays live in one device arena filled from one pinned staging buffer; both only
   // grow, so a batch re-assigned with inputs that fit reuses them (the per-call entry points do).
@@ -265,6 +271,10 @@ struct DeviceBatch::Impl {
     if (staging) (void)hipHostFree(staging);
     if (readback) (void)hipHostFree(readback);
     if (scratch) (void)hipFree(scratch);
+    if (side_stream) (void)hipStreamSynchronize(side_stream);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side_stream) (void)hipStreamDestroy(side_stream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -372,8 +382,10 @@ struct DeviceBatch::Impl {
   // Launches bucket R's sweep with configuration `c` (FP64 launches ignore `boost`).  The
   // scratch stride follows the kernel's rows per lane.
   template <typename T>
-  int launch(int R, SweepArgs a, int64_t max_items, LaunchCfg c = LaunchCfg{}) {
+  int launch(int R, SweepArgs a, int64_t max_items, LaunchCfg c = LaunchCfg{},
+             hipStream_t on = nullptr, size_t scratch_off = 0) {
     if (max_items <= 0) return kOk;
+    if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
     if (!std::is_same<T, float>::value) c.boost = false;
     int grid = 0;
@@ -384,14 +396,15 @@ struct DeviceBatch::Impl {
       // Every wave of the grid owns a stride of scratch: refuse a grid the batch's scratch
       // does not cover.
       const int64_t need = int64_t(grid) * (c.block / 64) * a.scratch_wave_stride * int64_t(sizeof(T));
-      if (need > int64_t(scratch_bytes)) {
+      a.scratch = static_cast<char*>(a.scratch) + scratch_off;
+      if (need + int64_t(scratch_off) > int64_t(scratch_bytes)) {
         set_error(kInternal, "a launch needs " + std::to_string(need) +
                                  " bytes of scratch, the batch holds " + std::to_string(scratch_bytes));
         return kInternal;
       }
     }
     hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows), dim3(grid), dim3(c.block),
-                       lds_words * 4 * (c.block / 64), stream, a, dev_tab<T>(st));
+                       lds_words * 4 * (c.block / 64), on, a, dev_tab<T>(st));
     HIP_TRY(hipGetLastError());
     return kOk;
   }
@@ -467,17 +480,34 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
   HIP_TRY(hipGetLastError());
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
   // The primary sweep, one launch per rows-per-lane bucket, each bracketed by events when the
-  // run is not captured into a graph.
+  // run is not captured into a graph.  The small buckets run on the side stream meanwhile
+  // (forked after the counters are cleared, joined before the rescue).
+  bool forked = false;
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
-    if (!bucket_n[R - 1]) continue;
-    const SweepArgs a = primary_args(R, share, fp32);
-    if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1)], stream));
-    if (fp32) {
-      if (int rc = launch<float>(R, a, bucket_n[R - 1], cfg[R - 1])) return rc;
-    } else {
-      if (int rc = launch<double>(R, a, bucket_n[R - 1])) return rc;
+    if (bucket_n[R - 1] && side[R - 1] && !forked) {
+      HIP_TRY(hipEventRecord(fork_ev, stream));
+      HIP_TRY(hipStreamWaitEvent(side_stream, fork_ev, 0));
+      forked = true;
     }
-    if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1) + 1], stream));
+  }
+  for (int pass = 0; pass < 2; ++pass) {  // side buckets first, so they start at once
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      if (!bucket_n[R - 1] || side[R - 1] != (pass == 0)) continue;
+      hipStream_t on = side[R - 1] ? side_stream : stream;
+      const size_t off = side[R - 1] ? side_scratch_off : 0;
+      const SweepArgs a = primary_args(R, share, fp32);
+      if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1)], on));
+      if (fp32) {
+        if (int rc = launch<float>(R, a, bucket_n[R - 1], cfg[R - 1], on, off)) return rc;
+      } else {
+        if (int rc = launch<double>(R, a, bucket_n[R - 1], LaunchCfg{}, on, off)) return rc;
+      }
+      if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1) + 1], on));
+    }
+  }
+  if (forked) {
+    HIP_TRY(hipEventRecord(join_ev, side_stream));
+    HIP_TRY(hipStreamWaitEvent(stream, join_ev, 0));
   }
   if (timed) HIP_TRY(hipEventRecord(ev[1], stream));
   if (fp32 && mode == kModeFp32Rescue) {
@@ -543,6 +573,9 @@ int DeviceBatch::assign(const BatchInput& in) {
   d.pairs_only = in.pairs_only;
   if (!d.stream) {
     HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&d.side_stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&d.fork_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&d.join_ev, hipEventDisableTiming));
     for (auto& e : d.ev) HIP_TRY(hipEventCreate(&e));
     for (auto& e : d.bev) HIP_TRY(hipEventCreate(&e));
   }
@@ -647,21 +680,38 @@ int DeviceBatch::assign(const BatchInput& in) {
     std::lock_guard<std::mutex> lock(g_dev_mutex);
     for (int R = 1; R <= kMaxRowsPerLane; ++R) d.cfg[R - 1] = d.st->tuned[d.pd][R - 1];
   }
-  // Scratch for the largest configuration any launch may use, so tuning never reallocates.
-  int64_t need = 0;
+  // Side buckets: those computing under 2% of the batch's cells, which cannot fill the GPU on
+  // their own (short-read and longest-read buckets of a mixed batch).
+  {
+    int64_t total = 0;
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) total += d.bucket_work[R - 1];
+    int n_big = 0;
+    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+      d.side[R - 1] = d.bucket_n[R - 1] > 0 && d.bucket_work[R - 1] * 50 < total;
+      n_big += d.bucket_n[R - 1] > 0 && !d.side[R - 1];
+    }
+    if (n_big == 0) std::fill(std::begin(d.side), std::end(d.side), false);
+  }
+  // Scratch for the largest configuration any launch may use, so tuning never reallocates: the
+  // main part for launches on the batch's stream, and after it the part the side buckets use
+  // while the large ones run.
+  int64_t need = 0, side_need = 0;
   if (p.max_slots > 0 || p.max_nodes > 0) {
     for (int R = 1; R <= kMaxRowsPerLane; ++R) {
       // The dense FP64 rescue of the reads with R rows per lane.
       if (d.rows_reads[R - 1])
         need = std::max(need, d.scratch_need<double>(R, LaunchCfg{}, d.rows_reads[R - 1]));
       if (!d.bucket_n[R - 1]) continue;
-      for (const LaunchCfg& c : d.candidates(R, true))
-        need = std::max(need, d.scratch_need<float>(R, c));
-      need = std::max(need, d.scratch_need<double>(R, LaunchCfg{}));
-      if (need < 0) break;
+      int64_t n = 0;
+      for (const LaunchCfg& c : d.candidates(R, true)) n = std::max(n, d.scratch_need<float>(R, c));
+      n = std::max(n, d.scratch_need<double>(R, LaunchCfg{}));
+      need = std::max(need, n);  // tuning runs every bucket on the batch's stream
+      if (d.side[R - 1]) side_need = std::max(side_need, n);
     }
   }
-  if (need < 0) return kInternal;
+  if (need < 0 || side_need < 0) return kInternal;
+  d.side_scratch_off = (size_t(need) + 255) & ~size_t(255);
+  if (side_need > 0) need = int64_t(d.side_scratch_off) + side_need;
   if (size_t(need) > d.scratch_cap) {
     if (d.scratch) HIP_TRY(hipFree(d.scratch));
     d.scratch = nullptr;
