@@ -512,9 +512,8 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: every GPU box of this build has one GPU, and the "
-              "path shards by region with no exchange (see DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one MI355X; the path shards by region with no exchange "
+              "between GPUs; run with --gpus 1", file=sys.stderr)
         return 2
     if args.sw_only:
         print(json.dumps({"smith_waterman": sw_bench(args)}), flush=True)

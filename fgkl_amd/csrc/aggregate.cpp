@@ -152,6 +152,12 @@ void aggregate_config(int max_regions, int max_wait_us) {
   std::lock_guard<std::mutex> lk(agg().mu);
   agg().max_regions = max_regions > 1 ? max_regions : 0;
   agg().max_wait_us = max_wait_us > 0 ? max_wait_us : 0;
+  if (agg().max_regions == 0) {
+    // Off: give back the device memory of the idle batches (a running one keeps its own).
+    for (Queue& q : agg().queues)
+      for (int slot = 0; slot < 2; ++slot)
+        if (!q.busy[slot]) q.batch[slot].reset();
+  }
 }
 
 bool aggregate_enabled() {

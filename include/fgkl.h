@@ -100,14 +100,17 @@ int fgkl_pairhmm_compute(fgkl_pairhmm* hmm,
                          double* out, int64_t out_len);
 
 /*
- * Aggregation of per-call regions (SURVEY's per-call -> batch-aggregator mode; no reference
- * counterpart).  With max_regions > 1, fgkl_pairhmm_compute_native / fgkl_pdhmm_compute_native
- * calls that arrive from several threads within max_wait_us of the first (up to max_regions)
- * run as one batch with prefix and suffix sharing; each call still returns when its own region's
- * likelihoods are written, and a region the batch cannot take (invalid input) fails only its own
- * call.  0 turns it off (the default).  The environment variable FGKL_AGGREGATE=<regions>[:<wait
- * us>] sets it for programs that cannot call this (a JVM with the unchanged Java bindings).
- * fgkl_aggregate_stats reports the batches run and the regions they held.
+ * Aggregation of per-call regions (no reference counterpart: fgkl serves each per-region call on
+ * its own).  With max_regions > 1, concurrent fgkl_pairhmm_compute_native /
+ * fgkl_pdhmm_compute_native calls of one kind (plain or PD, float or double) share batches of
+ * up to max_regions regions.  A call waits for company only while another batch of its kind is
+ * running, and for at most max_wait_us; on an idle GPU it runs at once.  Each call is validated
+ * on its own thread and returns when its own likelihoods are written.  A batch that fails reruns
+ * its regions one by one, so a bad region fails only its own call.  0 turns aggregation off (the
+ * default) and frees the device memory its idle batches hold.  The environment variable
+ * FGKL_AGGREGATE=<regions>[:<wait us>] sets it for programs that cannot call this (a JVM with
+ * the unchanged Java bindings).  fgkl_aggregate_stats reports the batches run and the regions
+ * they held.
  */
 int fgkl_aggregate_config(int max_regions, int max_wait_us);
 void fgkl_aggregate_stats(int64_t* batches, int64_t* regions);

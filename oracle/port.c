@@ -1,6 +1,6 @@
 /*
  * Lane-batched SIMD restatement of fgkl's CPU PairHMM backend: the CPU baseline that bench.py
- * times on the GPU box's host cores.
+ * times on the host cores beside the GPU.
  *
  * TEST / MEASUREMENT INFRASTRUCTURE ONLY.  Like the scalar oracle beside it, this is never
  * linked into, loaded by or called from the product library (fgkl_amd/).

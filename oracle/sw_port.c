@@ -1,6 +1,6 @@
 /*
- * Lane-batched SIMD Smith-Waterman: the CPU baseline bench.py times for the aligner on the GPU
- * box's host cores.
+ * Lane-batched SIMD Smith-Waterman: the CPU baseline bench.py times for the aligner on the host
+ * cores beside the GPU.
  *
  * TEST / MEASUREMENT INFRASTRUCTURE ONLY.  Never linked into, loaded by or called from the product
  * library (fgkl_amd/).
