@@ -281,9 +281,12 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
       // NORMAL copies the row's values (lme is m there), AFTER_DEL takes the maxima, INSIDE_DEL
       // holds the branch.
-      const T BM = sel_bits(I, s.bm[k], lme);
-      const T BX = sel_bits(I, s.bx[k], lxe);
-      const T BY = sel_bits(I, s.by[k], lye);
+      // Opaque to the optimiser: otherwise it distributes the next step's `branch & after` over
+      // the select and keeps both halves of each select live as well.
+      T BM = sel_bits(I, s.bm[k], lme);
+      T BX = sel_bits(I, s.bx[k], lxe);
+      T BY = sel_bits(I, s.by[k], lye);
+      asm volatile("" : "+v"(BM), "+v"(BX), "+v"(BY));
       s.m[k] = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
       s.y[k] = Y;
       s.bm[k] = BM;
