@@ -293,6 +293,35 @@ def aggregate_stats() -> tuple[int, int]:
     return b.value, r.value
 
 
+def release_idle() -> int:
+    """Frees the idle reusable batches of the per-call entry points and of aggregation; returns
+    how many (include/fgkl.h, fgkl_release_idle)."""
+    n = lib().fgkl_release_idle()
+    if n < 0:
+        raise NativeError(last_error())
+    return int(n)
+
+
+POOL_PAIRHMM, POOL_PDHMM, POOL_SMITH_WATERMAN = 0, 1, 2
+
+
+def pool_stats(kind: int = POOL_PAIRHMM) -> tuple[int, int, int]:
+    """(idle batches, checkouts not yet returned, checkouts served by an idle batch) of one
+    per-call pool."""
+    idle, out, reused = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(lib().fgkl_pool_stats(int(kind), C.byref(idle), C.byref(out), C.byref(reused)))
+    return idle.value, out.value, reused.value
+
+
+def selftest_pool(threads: int, rounds: int) -> dict:
+    """Runs the pool on the CPU with `threads` threads of `rounds` checkouts each (all of which
+    exit before it reports) and returns what it saw."""
+    out = (C.c_int64 * 5)()
+    _check(lib().fgkl_selftest_pool(int(threads), int(rounds), out))
+    keys = ("created", "destroyed_while_threads_ran", "idle", "outstanding", "reused")
+    return dict(zip(keys, (int(v) for v in out)))
+
+
 # ---- batched engine --------------------------------------------------------------------------
 class Batch:
     """Many regions resident in device memory (include/fgkl.h, fgkl_batch_*)."""

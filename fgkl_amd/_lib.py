@@ -115,6 +115,9 @@ SIGNATURES = {
     "fgkl_grad_free": (None, [C.c_void_p]),
     "fgkl_aggregate_config": (C.c_int, [C.c_int, C.c_int]),
     "fgkl_aggregate_stats": (None, [i64p, i64p]),
+    "fgkl_release_idle": (C.c_int64, []),
+    "fgkl_pool_stats": (C.c_int, [C.c_int, i64p, i64p, i64p]),
+    "fgkl_selftest_pool": (C.c_int, [C.c_int, C.c_int, i64p]),
     "fgkl_sw_backend_name": (C.c_char_p, []),
     "fgkl_sw_align_native": (C.c_int, [u8p, C.c_int64, u8p, C.c_int64, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, C.c_int32, i32p, C.c_char_p,

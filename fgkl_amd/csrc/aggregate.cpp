@@ -1,7 +1,7 @@
 // Aggregation of per-call regions: concurrent JNI-shaped calls (one region each, as HaplotypeCaller
-// threads make them) are gathered for up to a short wait and run as one batch through the batched
-// engine (prefix and suffix sharing, one launch set per rows-per-lane bucket), each caller blocking
-// until its own region's likelihoods are written.
+// threads make them) are gathered for up to a short wait and run as one batch, every pair in a
+// wave of its own as a lone call runs it, each caller blocking until its own region's likelihoods
+// are written.  A region's results are therefore the same bits whatever it was gathered with.
 //
 // Leader / follower: a queued caller that finds no batch being gathered for its kind, and one of
 // the kind's two batch slots free, becomes the leader.  While the other slot's batch runs it waits
@@ -49,9 +49,11 @@ struct Aggregator {
   int64_t batches = 0, regions = 0;
 };
 
+// Allocated once and never destroyed: its batches must not be freed by a static destructor at
+// process exit, when the HIP runtime may already be gone (fgkl_release_idle frees idle ones).
 Aggregator& agg() {
-  static Aggregator a;
-  return a;
+  static Aggregator* const a = new Aggregator;
+  return *a;
 }
 
 std::once_flag g_env_once;
@@ -72,8 +74,6 @@ void read_env() {
   }
 }
 
-constexpr int64_t kPairRunMaxCells = int64_t(15) * 1000 * 1000 * 1000;
-
 template <typename T>
 void append(std::vector<T>& dst, const T* src, int64_t n) {
   if (n > 0) dst.insert(dst.end(), src, src + n);
@@ -85,12 +85,11 @@ void run_jobs(std::unique_ptr<DeviceBatch>& batch, const std::vector<Job*>& jobs
   const RegionCall& first = *jobs[0]->call;
   std::vector<int64_t> region_read{0}, region_hap{0}, read_off{0}, hap_off{0};
   std::vector<uint8_t> bases, quals, ins, del, gcp, hap_bases, hap_flags;
-  int64_t pairs = 0, cells = 0;
+  int64_t pairs = 0;
   for (const Job* j : jobs) {
     const RegionCall& c = *j->call;
     const int64_t rb = c.read_off[c.n_reads], hb = c.hap_off[c.n_haps];
     pairs += c.n_reads * c.n_haps;
-    cells += rb * hb;
     for (int64_t r = 1; r <= c.n_reads; ++r) read_off.push_back(read_off.back() - c.read_off[r - 1] + c.read_off[r]);
     for (int64_t h = 1; h <= c.n_haps; ++h) hap_off.push_back(hap_off.back() - c.hap_off[h - 1] + c.hap_off[h]);
     region_read.push_back(region_read.back() + c.n_reads);
@@ -117,15 +116,15 @@ void run_jobs(std::unique_ptr<DeviceBatch>& batch, const std::vector<Job*>& jobs
   in.hap_off = hap_off.data();
   in.hap_bases = hap_bases.data();
   in.hap_flags = first.pd ? hap_flags.data() : nullptr;
-  // A batch of a few regions finishes sooner with every pair in a wave of its own than with
-  // prefix-shared sweeps, whose waves each walk a read through all of its region's haplotypes
-  // (a floor of a few milliseconds); measured on the batched uneven workload the two meet near
-  // 128 regions, 1.5e10 cells.
-  in.pairs_only = pairs <= kMaxPairList && cells <= kPairRunMaxCells;
+  // The run a lone call gets (compute_region): every pair in its own wave, or, for a single
+  // region beyond the pair list, the batched run with prefix and suffix sharing.  Pair-per-wave
+  // results do not depend on the lane layout or the launch, so a region gets the same bits in
+  // any gathered batch.
+  in.pairs_only = pairs <= kMaxPairList;
   const int mode = first.double_precision ? kModeFp64 : kModeFp32Rescue;
   int rc = batch ? batch->assign(in) : DeviceBatch::create(in, batch);
   std::vector<double> out;
-  if (rc == kOk) rc = in.pairs_only ? batch->run_latency(mode, nullptr) : batch->run(mode, 1, 0, nullptr);
+  if (rc == kOk) rc = batch->run_latency(mode, nullptr);
   if (rc == kOk) {
     out.resize(size_t(batch->n_pairs()));
     rc = batch->results(out.data(), int64_t(out.size()));
@@ -160,6 +159,17 @@ void aggregate_config(int max_regions, int max_wait_us) {
   }
 }
 
+int64_t aggregate_release_idle() {
+  std::vector<std::unique_ptr<DeviceBatch>> drop;
+  {
+    std::lock_guard<std::mutex> lk(agg().mu);
+    for (Queue& q : agg().queues)
+      for (int slot = 0; slot < 2; ++slot)
+        if (!q.busy[slot] && q.batch[slot]) drop.push_back(std::move(q.batch[slot]));
+  }
+  return int64_t(drop.size());  // destroyed here, outside the lock
+}
+
 bool aggregate_enabled() {
   std::call_once(g_env_once, read_env);
   std::lock_guard<std::mutex> lk(agg().mu);
@@ -192,7 +202,18 @@ int aggregate_run(const RegionCall& call) {
       while (int(q.pending.size()) < a.max_regions && (q.busy[0] || q.busy[1]) &&
              a.cv.wait_until(lk, deadline) != std::cv_status::timeout) {
       }
-      const size_t take = std::min(q.pending.size(), size_t(std::max(a.max_regions, 1)));
+      // Calls in arrival order, up to max_regions and the one-pair-per-wave list: every gathered
+      // region then runs as a lone call runs it (each pair in its own wave, no sharing), so its
+      // bits do not depend on the calls it was gathered with.  A region beyond the list runs on
+      // its own, as a lone call does.
+      size_t take = 0;
+      int64_t pairs = 0;
+      while (take < q.pending.size() && take < size_t(std::max(a.max_regions, 1))) {
+        const RegionCall& c = *q.pending[take]->call;
+        if (take > 0 && pairs + c.n_reads * c.n_haps > kMaxPairList) break;
+        pairs += c.n_reads * c.n_haps;
+        ++take;
+      }
       std::vector<Job*> jobs(q.pending.begin(), q.pending.begin() + long(take));
       q.pending.erase(q.pending.begin(), q.pending.begin() + long(take));
       const int slot = q.busy[0] ? 1 : 0;

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fgkl.h"
@@ -12,6 +13,7 @@
 #include "fgkl_engine.hpp"
 #include "fgkl_grad.hpp"
 #include "fgkl_internal.hpp"
+#include "fgkl_pool.hpp"
 #include "fgkl_replay.hpp"
 #include "fgkl_sw.hpp"
 #include "fgkl_synthetic.hpp"
@@ -48,6 +50,17 @@ int dpp_shl_selftest(int32_t* out64);
 }
 
 namespace {
+
+// The per-call pools: plain and PD PairHMM batches, Smith-Waterman batches.  Allocated once and
+// never destroyed (see fgkl_pool.hpp).
+Pool<DeviceBatch>& batch_pool(int pd) {
+  static Pool<DeviceBatch>* const pools = new Pool<DeviceBatch>[2];
+  return pools[pd ? 1 : 0];
+}
+Pool<SwBatch>& sw_pool() {
+  static Pool<SwBatch>* const pool = new Pool<SwBatch>;
+  return *pool;
+}
 
 int fail(int code, const std::string& msg) {
   set_error(code, msg);
@@ -105,19 +118,20 @@ int compute_region(int pd, int double_precision, int double_fallback, int64_t n_
   in.hap_bases = hap_bases;
   in.hap_flags = hap_flags;
   in.pairs_only = n_reads * n_haps <= kMaxPairList;
-  // One reusable batch per thread and kind: consecutive calls reuse its device memory, stream
-  // and events (the reference keeps its runners per thread too, kernel.rs:144-191).
-  thread_local std::unique_ptr<DeviceBatch> t_batch[2];
-  std::unique_ptr<DeviceBatch>& b = t_batch[pd ? 1 : 0];
-  if (!b) {
-    if (int rc = DeviceBatch::create(in, b)) return rc;
-  } else if (int rc = b->assign(in)) {
-    return rc;
-  }
+  // A reusable batch of this kind, checked out of the process-wide pool for this call:
+  // consecutive calls reuse its device memory, stream and events (the reference keeps its runners
+  // per thread for the same reason, kernel.rs:144-191), and no thread owns one when it exits.
+  Checkout<DeviceBatch> co(batch_pool(pd));
+  std::unique_ptr<DeviceBatch>& b = co.get();
+  int rc = b ? b->assign(in) : DeviceBatch::create(in, b);
   const int mode = double_precision ? kModeFp64 : (double_fallback ? kModeFp32Rescue : kModeFp32NoRescue);
-  if (int rc = b->run_latency(mode, nullptr)) return rc;
-  if (fallbacks) *fallbacks = uint64_t(b->last_rescued());
-  return b->results(out, out_len);
+  if (rc == kOk) rc = b->run_latency(mode, nullptr);
+  if (rc == kOk) {
+    if (fallbacks) *fallbacks = uint64_t(b->last_rescued());
+    rc = b->results(out, out_len);
+  }
+  if (rc == kInternal) co.drop();  // a device error: the next call starts from a fresh batch
+  return rc;
 }
 
 int compute_native(int pd, const uint8_t* bases, int64_t bases_len, const uint8_t* quals,
@@ -558,9 +572,6 @@ void fgkl_grad_free(fgkl_grad* g) { delete g; }
 
 namespace {
 
-// One reusable batch per thread for the single-pair entry (crates/jni/src/lib.rs:23-26).
-thread_local std::unique_ptr<SwBatch> t_sw;
-
 // Alignment::cigar_string: "<len><op>" per element.
 std::string cigar_text(const uint32_t* ops, int64_t n) {
   static const char kOps[] = "MIDNSHP=X";
@@ -589,20 +600,26 @@ int fgkl_sw_align_native(const uint8_t* reference, int64_t reference_len,
     if (reference_len == 0 || alternate_len == 0)  // Error::EmptySequence
       return fail(kInvalidArgument, "sequences must be non-empty");
     if (!offset || !cigar || cigar_cap < 1) return fail(kInvalidArgument, "no room for the result");
-    if (!t_sw && !device_available())
+    // A reusable one-pair batch checked out of the process-wide pool for this call (the
+    // reference keeps one aligner per thread, crates/jni/src/lib.rs:23-26).
+    Checkout<SwBatch> co(sw_pool());
+    std::unique_ptr<SwBatch>& sw = co.get();
+    if (!sw && !device_available())
       return fail(kInternal, "no HIP device is available: the fgkl GPU engine needs an MI355X (gfx950)");
     const int64_t ro[2] = {0, reference_len}, ao[2] = {0, alternate_len};
-    if (!t_sw) {
-      if (int rc = SwBatch::create(1, ro, reference, ao, alternate, t_sw)) return rc;
-    } else if (int rc = t_sw->assign(1, ro, reference, ao, alternate)) {
-      return rc;
-    }
     const int32_t params[4] = {match_value, mismatch_penalty, gap_open, gap_extend};
-    if (int rc = t_sw->run(params, strategy, nullptr)) return rc;
-    std::vector<uint32_t> ops(static_cast<size_t>(t_sw->cigar_bound()));
+    std::vector<uint32_t> ops;
     int64_t starts[2] = {0, 0};
     int32_t off = 0;
-    if (int rc = t_sw->results(&off, starts, ops.data(), int64_t(ops.size()))) return rc;
+    int rc = sw ? sw->assign(1, ro, reference, ao, alternate)
+                : SwBatch::create(1, ro, reference, ao, alternate, sw);
+    if (rc == kOk) rc = sw->run(params, strategy, nullptr);
+    if (rc == kOk) {
+      ops.resize(static_cast<size_t>(sw->cigar_bound()));
+      rc = sw->results(&off, starts, ops.data(), int64_t(ops.size()));
+    }
+    if (rc == kInternal) co.drop();
+    if (rc != kOk) return rc;
     const std::string text = cigar_text(ops.data(), starts[1]);
     if (int64_t(text.size()) + 1 > cigar_cap)
       return fail(kInvalidArgument, "CIGAR text needs " + std::to_string(text.size() + 1) + " bytes");
@@ -671,3 +688,69 @@ int fgkl_aggregate_config(int max_regions, int max_wait_us) {
 }
 
 void fgkl_aggregate_stats(int64_t* batches, int64_t* regions) { aggregate_stats(batches, regions); }
+
+// ---------------------------------------------------------------------------------------------
+// Pools of reusable per-call batches
+
+namespace {
+
+// Counts its own destructions, for the pool self-test.
+struct PoolProbe {
+  explicit PoolProbe(std::atomic<int64_t>* d) : destroyed(d) {}
+  ~PoolProbe() { destroyed->fetch_add(1); }
+  std::atomic<int64_t>* destroyed;
+};
+
+}  // namespace
+
+int64_t fgkl_release_idle(void) {
+  try {
+    return batch_pool(0).release_idle() + batch_pool(1).release_idle() +
+           sw_pool().release_idle() + aggregate_release_idle();
+  } catch (const std::exception& e) {
+    fail(kInternal, std::string("fgkl native code failed: ") + e.what());
+    return -1;
+  }
+}
+
+int fgkl_pool_stats(int kind, int64_t* idle, int64_t* checked_out, int64_t* reused) {
+  if (kind == 0 || kind == 1)
+    batch_pool(kind).stats(idle, checked_out, reused);
+  else if (kind == 2)
+    sw_pool().stats(idle, checked_out, reused);
+  else
+    return fail(kInvalidArgument, "pool kind must be 0 (PairHMM), 1 (PD-HMM) or 2 (Smith-Waterman)");
+  return kOk;
+}
+
+int fgkl_selftest_pool(int threads, int rounds, int64_t* out5) {
+  if (threads < 1 || rounds < 1 || !out5) return fail(kInvalidArgument, "bad self-test arguments");
+  std::atomic<int64_t> created{0}, destroyed{0};
+  Pool<PoolProbe> pool;
+  {
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+      ts.emplace_back([&] {
+        for (int r = 0; r < rounds; ++r) {
+          Checkout<PoolProbe> co(pool);
+          if (!co.get()) {
+            co.get().reset(new PoolProbe(&destroyed));
+            created.fetch_add(1);
+          }
+          std::this_thread::yield();
+        }
+      });
+    for (auto& th : ts) th.join();  // every worker has exited here
+  }
+  int64_t idle = 0, out = 0, reused = 0;
+  pool.stats(&idle, &out, &reused);
+  out5[0] = created.load();
+  out5[1] = destroyed.load();  // destroyed while the workers ran and exited
+  out5[2] = idle;
+  out5[3] = out;
+  out5[4] = reused;
+  pool.release_idle();
+  if (destroyed.load() != created.load())
+    return fail(kInternal, "pool self-test: release_idle left objects alive");
+  return kOk;
+}

@@ -566,6 +566,9 @@ int DeviceBatch::assign(const BatchInput& in) {
   d.out_on_host = false;
   d.drop_graphs();
   d.free_lazy();
+  // A previous assign that failed validation may have left arrays placed but not committed.
+  d.pieces.clear();
+  d.layout_bytes = 0;
   d.plan = HostPlan{};
   if (int rc = build_plan(in, d.plan)) return rc;
   if (int rc = device_state(&d.st)) return rc;

@@ -28,6 +28,8 @@ struct RegionCall {
 void aggregate_config(int max_regions, int max_wait_us);
 bool aggregate_enabled();
 void aggregate_stats(int64_t* batches, int64_t* regions);
+// Frees the device memory of the batches no aggregated call is using; returns how many.
+int64_t aggregate_release_idle();
 // Runs the region in a batch with the calls that arrive within the wait; returns its status.
 int aggregate_run(const RegionCall& call);
 // Runs one region on its own (the per-call latency path); defined with the C-ABI.

@@ -116,6 +116,24 @@ int fgkl_aggregate_config(int max_regions, int max_wait_us);
 void fgkl_aggregate_stats(int64_t* batches, int64_t* regions);
 
 /*
+ * Reusable per-call batches (no reference counterpart; the reference keeps one runner per
+ * thread, crates/pairhmm/src/kernel.rs:144-191).  The per-call entry points check a device batch
+ * out of a process-wide pool for each call and return it afterwards, so calls reuse device
+ * memory, streams and events, and a thread that exits holds none: nothing makes HIP calls at
+ * thread exit.  Pooled batches are never freed implicitly (not at thread exit, not from static
+ * destructors at process exit); fgkl_release_idle frees the idle ones (per-call pools and idle
+ * aggregation batches) and returns how many, or -1 on error.  fgkl_pool_stats reports, for kind
+ * 0 (PairHMM), 1 (PD-HMM) or 2 (Smith-Waterman), the idle batches, the checkouts not yet
+ * returned and the checkouts served by an idle batch.  fgkl_selftest_pool exercises the pool on
+ * the CPU with `threads` threads of `rounds` checkouts each, all of which exit before it reads
+ * out5 = {objects created, objects destroyed while the threads ran and exited, idle objects,
+ * checkouts outstanding, checkouts served by an idle object}.
+ */
+int64_t fgkl_release_idle(void);
+int fgkl_pool_stats(int kind, int64_t* idle, int64_t* checked_out, int64_t* reused);
+int fgkl_selftest_pool(int threads, int rounds, int64_t* out5);
+
+/*
  * Batched engine (no reference counterpart: the reference computes one region per call on the
  * calling thread, crates/pairhmm/src/lib.rs:377-397).  A batch holds many regions resident in
  * device memory; region g owns reads region_reads[g]..region_reads[g+1] and haplotypes

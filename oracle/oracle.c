@@ -186,6 +186,9 @@ double fgklo_pd_log10_likelihood(const uint8_t* hap, const uint8_t* flags, int64
     for (int64_t j = 0; j < cols; ++j) Y[j] = initial;
     int state = NORMAL; /* carried from the end of one row into the next (pd_reference.rs:46) */
     for (int64_t i = 1; i < rows; ++i) {
+#ifdef FGKLO_PLANT_PD_ROW_RESET
+        state = NORMAL; /* a planted mistake, built only by tests/test_oracle.py to show its KAT fails */
+#endif
         double t[6], p[2];
         fgklo_transitions(ins_gop[i - 1], del_gop[i - 1], gcp[i - 1], t);
         fgklo_priors(quals[i - 1], p);

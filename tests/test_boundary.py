@@ -142,6 +142,25 @@ def test_without_a_gpu_the_engine_fails_loudly():
         F.PairHmm()
 
 
+def test_pooled_batches_survive_the_threads_that_used_them():
+    """The per-call entry points check reusable batches out of a process-wide pool for one call
+    each.  Sixteen threads of 50 checkouts each run and exit: nothing is destroyed while they run
+    or at their exit (no HIP call at thread exit), every checkout is returned, at most one object
+    per thread was ever created, later checkouts reuse idle ones, and release_idle frees them."""
+    st = F.selftest_pool(16, 50)
+    assert st["destroyed_while_threads_ran"] == 0
+    assert st["outstanding"] == 0
+    assert 1 <= st["created"] <= 16
+    assert st["idle"] == st["created"]
+    assert st["reused"] == 16 * 50 - st["created"]
+    # the product's pools start empty on a host that never ran a call
+    if F.device_count() == 0:
+        assert F.pool_stats(F.POOL_PAIRHMM) == (0, 0, 0)
+        assert F.release_idle() == 0
+    with pytest.raises(F.InvalidArgument):
+        F.pool_stats(7)
+
+
 def test_aggregated_calls_validate_and_fail_like_single_calls():
     """With aggregation on, each call is still checked on its own thread before it is queued, and a
     batch that cannot run hands every caller the error of its own region."""

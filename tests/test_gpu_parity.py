@@ -380,6 +380,26 @@ def test_pd_flags_on_the_last_columns_carry_their_state_into_the_next_row(gpu):
     assert_close(out, expected, FLOAT_TOL, "pd last columns float")
 
 
+def test_pd_carried_deletion_state_keeps_later_rows_from_skipping(gpu):
+    """The oracle's hand-worked carry case (haplotype GCTT, DEL_END on column 3, DEL_START on
+    column 4, read GCT): rows 2-3 start inside a deletion, so the skip of column 3 does not count
+    and the likelihood is that of the one perfect alignment; without the trailing DEL_START it
+    counts twice.  Reads of 3..9 rows and lane layouts with identity rows at every offset."""
+    import math
+
+    haps = [F.PdHaplotype(b"GCTT", [0, 0, F.DEL_END, F.DEL_START]),
+            F.PdHaplotype(b"GCTT", [0, 0, F.DEL_END, 0]),
+            F.PdHaplotype(b"GCTTAGCTT", [0, 0, F.DEL_END, F.DEL_START, 0, 0, F.DEL_END, 0,
+                                         F.DEL_START])]
+    reads = [read(b"GCT"), read(b"GCTA"), read(b"GCTAGCT"), read(b"CTTAGCTTA")]
+    expected = pd_reference_all(reads, haps)
+    assert abs(expected[1] - expected[0] - math.log10(2.0)) < 1e-4
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what="pd carry double")
+    out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, "pd carry float")
+
+
 def test_pd_adjacent_and_single_base_deletions_match_the_reference(gpu):
     hap = b"ACGTTGCAAGGCTTAGGCTTACGATCGATTACA"
     n = len(hap)
@@ -475,7 +495,10 @@ def test_native_entry_matches_reference_in_both_precisions(gpu):
             assert_close(out, expected, tol, "native float")
 
 
-def test_concurrent_native_calls_give_identical_results(gpu):  # FgklPairHmmTest.java:193-211
+@pytest.mark.parametrize("aggregate", [0, 8])
+def test_concurrent_native_calls_give_identical_results(gpu, aggregate):  # FgklPairHmmTest.java:193-211
+    """Eight concurrent calls of one region give bitwise the results of a lone call, with each
+    call on its own and with the calls gathered into shared batches."""
     data = Region.generate(8, 200, 150, 30, 400)
     ro, ho = _native_arrays(data)
     n = data.n_pairs()
@@ -489,21 +512,151 @@ def test_concurrent_native_calls_give_identical_results(gpu):  # FgklPairHmmTest
     expected = run()
     results = [None] * 8
     errors = []
+    barrier = threading.Barrier(8)
 
     def worker(i):
         try:
+            barrier.wait()
             results[i] = run()
         except Exception as e:  # pragma: no cover - reported below
             errors.append(e)
 
-    threads = [threading.Thread(target=worker, args=(i,)) for i in range(8)]
+    b0, r0 = F.aggregate_stats()
+    F.aggregate(aggregate, 20000)
+    try:
+        threads = [threading.Thread(target=worker, args=(i,)) for i in range(8)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    finally:
+        F.aggregate(0)
+    assert not errors, errors
+    for r in results:
+        assert np.array_equal(r, expected)
+    b1, r1 = F.aggregate_stats()
+    assert r1 - r0 == (8 if aggregate else 0)
+
+
+def _region_native(data, g):
+    """Region g of a batch as the JNI-shaped entry takes it."""
+    r0, r1 = int(data.region_reads[g]), int(data.region_reads[g + 1])
+    h0, h1 = int(data.region_haps[g]), int(data.region_haps[g + 1])
+    lo, hi = int(data.read_offsets[r0]), int(data.read_offsets[r1])
+    hlo, hhi = int(data.hap_offsets[h0]), int(data.hap_offsets[h1])
+    ro = (data.read_offsets[r0:r1 + 1] - lo).astype(np.int32)
+    ho = (data.hap_offsets[h0:h1 + 1] - hlo).astype(np.int32)
+    arrs = [a[lo:hi] for a in (data.bases, data.quals, data.ins, data.dels, data.gcp)]
+    return arrs, ro, data.hap_bases[hlo:hhi], data.hap_flags[hlo:hhi], ho, (r1 - r0) * (h1 - h0)
+
+
+@pytest.mark.parametrize("pd", [False, True])
+def test_aggregated_regions_get_their_lone_call_bits_in_large_gathered_batches(gpu, pd):
+    """lib.rs:971-1017, pd.rs:400-426 for the aggregator.  The largest regions of a draw of the
+    batched uneven workload (PD: with flags), 1.6e10 cells together and reads of mixed lengths,
+    run as one gathered batch the way the aggregator runs one (every pair in a wave of its own,
+    each rows-per-lane bucket at its own rows), and arrive together from one thread each with
+    aggregation on; each region gets exactly the bits of a lone call (one launch at the region's
+    largest rows per lane), in both precisions."""
+    draw = workload(3 if pd else 1, seed=23, n_regions=600)
+    rr, hh = draw.read_offsets[draw.region_reads], draw.hap_offsets[draw.region_haps]
+    cells = (rr[1:] - rr[:-1]) * (hh[1:] - hh[:-1])
+    picked, total = [], 0
+    for g in np.argsort(-cells, kind="stable"):
+        picked.append(int(g))
+        total += int(cells[g])
+        if total > 1.6e10:
+            break
+    regions = [(draw.region_reads_list(g),
+                draw.region_pd_haps_list(g) if pd else draw.region_haps_list(g)) for g in picked]
+    data = batch_from_regions(regions, pd=pd)
+    assert data.cells() > 1.5e10 and data.n_pairs() <= 1 << 20
+    native = [_region_native(data, g) for g in range(data.n_regions)]
+
+    def call(g, dp):
+        arrs, ro, hb, hf, ho, n = native[g]
+        out = np.zeros(n)
+        if pd:
+            F.pd_compute_native(*arrs, ro, hb, hf, ho, dp, out)
+        else:
+            F.compute_native(*arrs, ro, hb, ho, dp, out)
+        return out
+
+    base = data.out_base()
+    b = F.Batch(data)
+    for dp in (False, True):
+        alone = [call(g, dp) for g in range(len(regions))]
+        b.run_pairs(F.MODE_FP64 if dp else F.MODE_FP32_RESCUE)
+        whole = b.results()
+        for g in range(len(regions)):
+            assert np.array_equal(whole[base[g]:base[g + 1]], alone[g]), f"batch: region {g} double={dp}"
+            assert np.all(np.isfinite(alone[g]))
+
+        got, errors = {}, []
+        barrier = threading.Barrier(len(regions))
+
+        def worker(g):
+            try:
+                barrier.wait()
+                got[g] = call(g, dp)
+            except Exception as e:  # pragma: no cover - reported below
+                errors.append(e)
+
+        b0, r0 = F.aggregate_stats()
+        F.aggregate(len(regions), 200000)
+        try:
+            threads = [threading.Thread(target=worker, args=(g,)) for g in range(len(regions))]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+        finally:
+            F.aggregate(0)
+        assert not errors, errors
+        b1, r1 = F.aggregate_stats()
+        assert r1 - r0 == len(regions)
+        assert b1 - b0 < len(regions), "no two calls shared a batch"
+        for g in range(len(regions)):
+            assert np.array_equal(got[g], alone[g]), f"aggregated: region {g} double={dp}"
+    b.close()
+
+
+def test_per_call_batches_are_pooled_across_threads_that_exit(gpu):
+    """Per-call batches are checked out of a process-wide pool: four threads that call and exit
+    leave their batches idle in the pool (nothing is freed at thread exit), later calls reuse
+    them with the same results, and release_idle frees them."""
+    data = Region.generate(5, 40, 120, 6, 260)
+    ro, ho = _native_arrays(data)
+
+    def run():
+        out = np.zeros(data.n_pairs())
+        F.compute_native(data.bases, data.quals, data.ins, data.dels, data.gcp, ro,
+                         data.hap_bases, ho, False, out)
+        return out
+
+    expected = run()
+    F.release_idle()
+    barrier = threading.Barrier(4)
+    results = []
+
+    def worker():
+        barrier.wait()
+        results.append(run())
+
+    threads = [threading.Thread(target=worker) for _ in range(4)]
     for t in threads:
         t.start()
     for t in threads:
         t.join()
-    assert not errors, errors
-    for r in results:
-        assert np.array_equal(r, expected)
+    idle, out, reused0 = F.pool_stats(F.POOL_PAIRHMM)
+    assert out == 0 and 1 <= idle <= 4
+    assert all(np.array_equal(r, expected) for r in results)
+    assert np.array_equal(run(), expected)
+    _, _, reused1 = F.pool_stats(F.POOL_PAIRHMM)
+    assert reused1 == reused0 + 1
+    assert F.release_idle() >= idle
+    assert F.pool_stats(F.POOL_PAIRHMM)[0] == 0
+    assert np.array_equal(run(), expected)
 
 
 def test_pd_native_entry_matches_reference(gpu):

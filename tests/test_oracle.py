@@ -264,6 +264,60 @@ def test_pd_single_base_deletion_on_one_column_is_max_of_skip_and_keep():
                O.log10_likelihood(hap, r)) < 1e-12
 
 
+# pd_reference.rs:46: the deletion state is carried from the end of one row into the next.
+# Hand-worked case: haplotype GCTT with column 3 flagged DEL_END and column 4 DEL_START, read GCT
+# (Q30, ins/del 45, GCP 10).  Row 1 starts NORMAL; column 3 (DEL_END) leaves it AFTER_DEL and
+# column 4 (DEL_START) INSIDE_DEL, so rows 2 and 3 start inside a deletion whose branch values
+# hold column 0's zeros until column 3 ends it.  The read can therefore not skip column 3: the
+# skip at column 4 takes max(branch, M) on the diagonal, and row 2's branch at column 3 is zero.
+# The only perfect alignment is G C T on columns 1-3:
+#     L = (C/4) * im * pm * mm * pm * mm * pm   (im = 1 - 10^-1, pm = 1 - 10^-3, mm = mm(45, 45))
+# plus paths with a mismatch or a gap, each below 3.4e-4 of it (e/3 = 3.3e-4, 10^-4.5 gaps).
+# Resetting the state at each row would start rows 2 and 3 NORMAL, their branch at column 3 would
+# hold M of column 2, and the skip G C _ T would add a second perfect path: L twice as large.
+# Two-row reads cannot show the difference (a branch value of row 2 is read only by row 3).
+PD_CARRY_KAT = (b"GCTT", [0, 0, F.DEL_END, F.DEL_START], b"GCT")
+
+
+def _pd_carry_main_path():
+    pm = 1.0 - o.fgklo_error_prob(30)
+    mm = o.fgklo_match_to_match(45, 45)
+    im = 1.0 - o.fgklo_error_prob(10)
+    return math.log10(pm ** 3 * mm ** 2 * im / 4.0)
+
+
+def test_pd_deletion_state_carries_into_the_next_row():
+    hap, flags, rb = PD_CARRY_KAT
+    got = O.pd_log10_likelihood(hap, flags, read(rb))
+    assert 0.0 <= got - _pd_carry_main_path() < 1e-4, got
+    # the same flags without the trailing DEL_START leave rows 2-3 NORMAL: the skip counts
+    no_start = O.pd_log10_likelihood(hap, [0, 0, F.DEL_END, 0], read(rb))
+    assert abs(no_start - (_pd_carry_main_path() + math.log10(2.0))) < 1e-4, no_start
+
+
+def test_a_planted_per_row_state_reset_fails_the_carry_kat(tmp_path):
+    """The oracle rebuilt with the deletion state reset at every row (a planted mistake) misses
+    the known answer by log10 2: the check above is not vacuous."""
+    import ctypes as C
+    import subprocess
+
+    so = tmp_path / "planted.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11",
+                    "-DFGKLO_PLANT_PD_ROW_RESET", "-o", str(so), str(O.ORACLE_DIR / "oracle.c"),
+                    "-lm"], check=True)
+    planted = C.CDLL(str(so))
+    planted.fgklo_pd_log10_likelihood.restype = C.c_double
+    planted.fgklo_pd_log10_likelihood.argtypes = o.fgklo_pd_log10_likelihood.argtypes
+    hap, flags, rb = PD_CARRY_KAT
+    h, f = O._u8(hap), O._u8(flags)
+    r = read(rb)
+    b, q, i, d, g = (O._u8(x) for x in (r.bases, r.quals, r.ins_gop, r.del_gop, r.gcp))
+    bad = planted.fgklo_pd_log10_likelihood(O._p(h), O._p(f), len(h), O._p(b), O._p(q), O._p(i),
+                                            O._p(d), O._p(g), len(b))
+    assert abs(bad - (_pd_carry_main_path() + math.log10(2.0))) < 1e-4, bad
+    assert bad - O.pd_log10_likelihood(hap, flags, r) > 0.3
+
+
 # ---- gradient oracle ---------------------------------------------------------------------------------
 def _cont_inputs(rd):
     e = np.array([o.fgklo_error_prob(q) for q in rd.quals])
