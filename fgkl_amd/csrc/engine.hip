@@ -50,12 +50,19 @@ template <> struct Kernels<float> {
 template <> struct Kernels<double> {
   static KernelD get(bool pd, bool, int rows) { return kSweepF64[pd][rows - 1]; }
 };
+template <typename T> struct LongKernel;
+template <> struct LongKernel<float> {
+  static KernelF get(bool pd) { return kLongF32[pd]; }
+};
+template <> struct LongKernel<double> {
+  static KernelD get(bool pd) { return kLongF64[pd]; }
+};
 
 constexpr int kBlocks[3] = {64, 128, 256};
 int block_index(int block) { return block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1; }
 
 constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
-constexpr int kCounters = 40;  // see Impl::counters
+constexpr int kCounters = 48;  // see Impl::counters
 // Uploads above this size go straight from the caller's arrays when no staging buffer fits them.
 constexpr size_t kDirectUploadBytes = size_t(64) << 20;
 // Pairs a one-pair-per-wave run takes in a single launch: 256 CUs hold 8192 waves at 32 per CU.
@@ -72,6 +79,8 @@ struct DeviceState {
   int occ_lds[2][2][2][kMaxRowsPerLane][3] = {};
   // Launch configurations the autotuner chose for FP32 buckets [pd][R-1] (rows 0: none yet).
   LaunchCfg tuned[2][kMaxRowsPerLane];
+  // Resident one-wave blocks per CU of long_kernel [fp64][pd] (0: not queried yet).
+  int occ_long[2][2] = {};
 };
 
 std::mutex g_dev_mutex;
@@ -214,8 +223,18 @@ struct DeviceBatch::Impl {
   int32_t* bwrites = nullptr;
   int32_t* region_bw = nullptr;
   // Work-queue heads and counts per rows-per-lane bucket: [0,8) main, [8,16) rescue,
-  // [16,24) rescue counts, [24,32) dense-read counts, [32,40) dense-rescue heads.
+  // [16,24) rescue counts, [24,32) dense-read counts, [32,40) dense-rescue heads; long pairs:
+  // 40 main head, 41 rescue head, 42 rescue count.
   int32_t* counters = nullptr;
+  // Pairs of regions beyond the staged layout (long_kernel): (read, sorted haplotype), their
+  // FP32 rescue list, and the boundary-row scratch (two buffers of (longest + 1) columns per
+  // wave, sized for FP64 and the largest grid).
+  int32_t* long_pairs = nullptr;
+  int32_t* long_rescue = nullptr;
+  int64_t n_long = 0;
+  void* long_scratch = nullptr;
+  size_t long_scratch_cap = 0;
+  int long_grid = 0;
   uint8_t* dense_flags = nullptr;              // per read, written by every FP32 main pass
   int32_t* dense_items[kMaxRowsPerLane] = {};  // (read, -1) by the read's own rows per lane
   int64_t rows_reads[kMaxRowsPerLane] = {};    // reads of each rows per lane (plain batches)
@@ -271,6 +290,7 @@ struct DeviceBatch::Impl {
     if (staging) (void)hipHostFree(staging);
     if (readback) (void)hipHostFree(readback);
     if (scratch) (void)hipFree(scratch);
+    if (long_scratch) (void)hipFree(long_scratch);
     if (side_stream) (void)hipStreamSynchronize(side_stream);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
@@ -430,6 +450,66 @@ struct DeviceBatch::Impl {
     return snap_elems<T>(rows) + (nodes ? int64_t(plan.max_nodes) * ns * 64 : 0);
   }
 
+  // Elements of T per wave of long_kernel's boundary scratch: two buffers of (longest + 1)
+  // columns of M, X, Y (and PD branch values).
+  int64_t long_stride() const { return 2 * (int64_t(plan.long_max_hap) + 1) * (pd ? 6 : 3); }
+
+  // Resident waves of long_kernel<T> on the device.
+  template <typename T>
+  int long_resident(int* waves) {
+    const int ti = std::is_same<T, double>::value ? 1 : 0;
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    int& occ = st->occ_long[ti][pd];
+    if (occ == 0) {
+      int nb = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, LongKernel<T>::get(pd), 64, 0));
+      occ = std::max(nb, 1);
+    }
+    *waves = occ * st->num_cus;
+    return kOk;
+  }
+
+  // Launches long_kernel<T> over `max_items` items of `a` (pair mode, one wave per block).
+  template <typename T>
+  int launch_long(SweepArgs a, int64_t max_items) {
+    if (max_items <= 0) return kOk;
+    int resident = 0;
+    if (int rc = long_resident<T>(&resident)) return rc;
+    const int grid = int(std::min<int64_t>(std::min<int64_t>(resident, max_items), long_grid));
+    a.scratch = long_scratch;
+    a.scratch_wave_stride = long_stride();
+    a.pair_mode = 1;
+    hipLaunchKernelGGL(LongKernel<T>::get(pd), dim3(grid), dim3(64), 0, stream, a, dev_tab<T>(st));
+    HIP_TRY(hipGetLastError());
+    return kOk;
+  }
+
+  // The long pairs' pass: FP32 with their rescue list (or FP64), counters 40 and 42.
+  int enqueue_long(bool fp32) {
+    if (!n_long) return kOk;
+    SweepArgs a = base_args();
+    a.items = long_pairs;
+    a.n_items = int32_t(n_long);
+    a.counter = counters + 40;
+    if (fp32) {
+      a.rescue = 1;
+      a.rescue_pairs[0] = long_rescue;
+      a.rescue_count = counters + 42;
+      return launch_long<float>(a, n_long);
+    }
+    return launch_long<double>(a, n_long);
+  }
+  // The FP64 rescue of the long pairs the FP32 pass lost.
+  int enqueue_long_rescue() {
+    if (!n_long) return kOk;
+    SweepArgs a = base_args();
+    a.items = long_rescue;
+    a.n_items = 0;
+    a.n_items_dev = counters + 42;
+    a.counter = counters + 41;
+    return launch_long<double>(a, n_long);
+  }
+
   // Largest rows per lane bucket R may run at.  PD batches keep R: with prefix sharing a PD
   // read's last bits can change with its lane layout (seen as 4e-8 in log10 on 2 of 7444
   // pairs), while plain reads are bitwise the same at R and R + 1.
@@ -509,6 +589,7 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
     HIP_TRY(hipEventRecord(join_ev, side_stream));
     HIP_TRY(hipStreamWaitEvent(stream, join_ev, 0));
   }
+  if (int rc = enqueue_long(fp32)) return rc;
   if (timed) HIP_TRY(hipEventRecord(ev[1], stream));
   if (fp32 && mode == kModeFp32Rescue) {
     // Dense reads first: each against every haplotype of its region, FP64 with prefix sharing,
@@ -545,6 +626,8 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
       // The grid is sized for the worst case; waves beyond the rescued count exit at once.
       if (int rc = launch<double>(R, a, rows_pairs[R - 1])) return rc;
     }
+    if (mode == kModeFp32Rescue)
+      if (int rc = enqueue_long_rescue()) return rc;
   }
   return kOk;
 }
@@ -626,7 +709,8 @@ int DeviceBatch::assign(const BatchInput& in) {
   std::fill(std::begin(d.rows_pairs), std::end(d.rows_pairs), 0);
   if (!in.pairs_only)  // pair runs keep their own rescue list
     for (int64_t r = 0; r < NR; ++r)
-      d.rows_pairs[p.read_rows[size_t(r)] - 1] += p.region_nhap[size_t(p.read_region[size_t(r)])];
+      if (p.read_rows[size_t(r)] > 0)  // long-kernel reads have their own
+        d.rows_pairs[p.read_rows[size_t(r)] - 1] += p.region_nhap[size_t(p.read_region[size_t(r)])];
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.rescue_pairs[R - 1], static_cast<const int32_t*>(nullptr),
             size_t(2 * d.rows_pairs[R - 1]));
@@ -634,11 +718,15 @@ int DeviceBatch::assign(const BatchInput& in) {
   d.place(&d.dense_flags, static_cast<const uint8_t*>(nullptr), size_t(NR));
   std::fill(std::begin(d.rows_reads), std::end(d.rows_reads), 0);
   if (!in.pd && !in.pairs_only)
-    for (int64_t r = 0; r < NR; ++r) d.rows_reads[p.read_rows[size_t(r)] - 1] += 1;
+    for (int64_t r = 0; r < NR; ++r)
+      if (p.read_rows[size_t(r)] > 0) d.rows_reads[p.read_rows[size_t(r)] - 1] += 1;
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.dense_items[R - 1], static_cast<const int32_t*>(nullptr),
             size_t(2 * d.rows_reads[R - 1]));
   d.place(&d.out, static_cast<const double*>(nullptr), size_t(p.n_pairs));
+  d.n_long = int64_t(p.long_pairs.size() / 2);
+  d.place(&d.long_pairs, p.long_pairs.data(), p.long_pairs.size());
+  d.place(&d.long_rescue, static_cast<const int32_t*>(nullptr), p.long_pairs.size());
   std::vector<int32_t> all;
   d.n_all_pairs = 0;
   d.all_rows = 0;
@@ -723,6 +811,22 @@ int DeviceBatch::assign(const BatchInput& in) {
     d.scratch_cap = size_t(need);
   }
   d.scratch_bytes = size_t(need);
+  // Boundary-row scratch of long_kernel, for the larger of its FP32 and FP64 grids.
+  d.long_grid = 0;
+  if (d.n_long) {
+    int wf = 0, wd = 0;
+    if (int rc = d.long_resident<float>(&wf)) return rc;
+    if (int rc = d.long_resident<double>(&wd)) return rc;
+    d.long_grid = int(std::min<int64_t>(std::max(wf, wd), d.n_long));
+    const size_t bytes = size_t(d.long_grid) * size_t(d.long_stride()) * sizeof(double);
+    if (bytes > d.long_scratch_cap) {
+      if (d.long_scratch) HIP_TRY(hipFree(d.long_scratch));
+      d.long_scratch = nullptr;
+      d.long_scratch_cap = 0;
+      HIP_TRY(hipMalloc(&d.long_scratch, bytes));
+      d.long_scratch_cap = bytes;
+    }
+  }
   return kOk;
 }
 
@@ -769,11 +873,13 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
     float total = 0.f;
     HIP_TRY(hipEventElapsedTime(&total, d.ev[0], d.ev[2]));
     stats->total_ms = total;
-    int32_t counts[32];
+    int32_t counts[kCounters];
     HIP_TRY(hipMemcpy(counts, d.counters, sizeof(counts), hipMemcpyDeviceToHost));
     int64_t rescued = 0;
-    if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue)
+    if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue) {
       for (int R = 0; R < kMaxRowsPerLane; ++R) rescued += counts[16 + R];
+      rescued += counts[42];
+    }
     stats->rescued_pairs = rescued;
     for (int R = 0; R < kMaxRowsPerLane; ++R) {
       stats->bucket_cells[R] = d.bucket_cells[R];
@@ -832,7 +938,9 @@ int DeviceBatch::run_per_region(RunStats* stats) {
     }
     d.region_pair_off[size_t(G * kMaxRowsPerLane)] = int64_t(pairs.size() / 2);
     if (int rc = d.upload(&d.region_pairs, pairs.data(), pairs.size())) return rc;
-    d.n_region_launches = G * kMaxRowsPerLane;
+    // One work-queue head per (region, rows per lane) launch, then one per region for its
+    // long-kernel pairs.
+    d.n_region_launches = G * (kMaxRowsPerLane + 1);
     if (int rc = d.alloc(&d.region_counters, size_t(d.n_region_launches))) return rc;
   }
   const int n = int(d.n_region_launches);
@@ -852,6 +960,14 @@ int DeviceBatch::run_per_region(RunStats* stats) {
       a.pair_mode = 1;
       a.scratch = nullptr;
       if (int rc = d.launch<double>(R, a, hi - lo)) return rc;
+    }
+    const int64_t llo = p.region_long[size_t(g)], lhi = p.region_long[size_t(g + 1)];
+    if (lhi > llo) {
+      SweepArgs a = d.base_args();
+      a.items = d.long_pairs + 2 * llo;
+      a.n_items = int32_t(lhi - llo);
+      a.counter = d.region_counters + G * kMaxRowsPerLane + g;
+      if (int rc = d.launch_long<double>(a, lhi - llo)) return rc;
     }
     // One synchronous call per region, as HaplotypeCaller makes them.
     HIP_TRY(hipStreamSynchronize(d.stream));
@@ -902,7 +1018,7 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
   // time to results); a larger one launches each rows-per-lane bucket at its own rows, each with
   // its own work-queue head, the FP32 launches sharing one rescue list.
   const bool one_launch = d.n_all_pairs <= kOneLaunchPairs;
-  for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+  for (int R = 1; R <= kMaxRowsPerLane && d.all_rows > 0; ++R) {
     const int64_t first = one_launch ? 0 : d.all_off[R - 1];
     const int64_t n = one_launch ? (R == d.all_rows ? d.n_all_pairs : 0) : d.all_off[R] - first;
     if (!n) continue;
@@ -922,9 +1038,10 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
       if (int rc = d.launch<double>(R, a, n)) return rc;
     }
   }
+  if (int rc = d.enqueue_long(fp32)) return rc;
   const int R = d.all_rows;
   if (stats) HIP_TRY(hipEventRecord(d.ev[1], d.stream));
-  if (mode == kModeFp32Rescue) {
+  if (mode == kModeFp32Rescue && R > 0) {
     SweepArgs b = d.base_args();
     b.items = d.all_rescue;
     b.n_items = 0;
@@ -935,14 +1052,22 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     b.scratch = nullptr;
     if (int rc = d.launch<double>(R, b, d.n_all_pairs)) return rc;
   }
+  if (mode == kModeFp32Rescue)
+    if (int rc = d.enqueue_long_rescue()) return rc;
   if (stats) HIP_TRY(hipEventRecord(d.ev[2], d.stream));
   auto* rb = static_cast<char*>(d.readback);
-  if (fp32)
+  if (fp32) {  // the rescued counts of the staged pairs and of the long pairs
     HIP_TRY(hipMemcpyAsync(rb, d.counters + 16, sizeof(int32_t), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipMemcpyAsync(rb + 4, d.counters + 42, sizeof(int32_t), hipMemcpyDeviceToHost, d.stream));
+  }
   HIP_TRY(hipMemcpyAsync(rb + 8, d.out, out_bytes, hipMemcpyDeviceToHost, d.stream));
   HIP_TRY(hipStreamSynchronize(d.stream));
   int32_t rescued = 0;
-  if (fp32) std::memcpy(&rescued, rb, sizeof(rescued));
+  if (fp32) {
+    int32_t c[2];
+    std::memcpy(c, rb, sizeof(c));
+    rescued = c[0] + c[1];
+  }
   d.last_rescued = rescued;
   d.out_on_host = true;
   if (stats) {

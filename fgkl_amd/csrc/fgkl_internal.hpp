@@ -111,9 +111,15 @@ struct BatchInput {
 constexpr int64_t kMaxPairList = int64_t(1) << 20;
 
 // A wave holds the n read rows plus at least one identity row that reproduces DP row 0, at
-// R rows per lane over 64 lanes: reads of up to 64 * 8 - 1 = 511 bases.
+// R rows per lane over 64 lanes: reads of up to 64 * 8 - 1 = 511 bases run in the staged sweep.
 constexpr int kMaxRowsPerLane = 8;
 constexpr int kMaxReadLen = 64 * kMaxRowsPerLane - 1;
+// Haplotypes the staged sweep takes: its LDS holds the longest haplotype of a launch per wave,
+// and 2048 columns keep four waves of a 256-thread block within 34 KiB.
+constexpr int kMaxStagedHap = 2048;
+// A region shares prefixes (suffixes) only while it needs at most this many snapshot slots
+// (suffix node records): every wave's scratch holds the batch's largest count.
+constexpr int kMaxRegionSlots = 128;
 
 struct HostPlan {
   int64_t n_reads = 0, n_haps = 0, n_pairs = 0;
@@ -135,6 +141,14 @@ struct HostPlan {
   // DP cells of the batch: nominal (every read base x haplotype base), computed with prefix
   // sharing alone, and with suffix sharing (forward, backward) in the FP32 pass.
   int64_t work_nominal = 0, work_prefix = 0, work_forward = 0, work_backward = 0;
+  // Regions with a read of more than kMaxReadLen bases or a haplotype of more than
+  // kMaxStagedHap run every pair in long_kernel, unshared: pairs (read, sorted haplotype) in
+  // region order, region g's from region_long[g] to region_long[g + 1]; their reads have
+  // read_rows 0 and appear in no bucket.
+  std::vector<int32_t> long_pairs;
+  std::vector<int64_t> region_long;
+  int32_t long_max_hap = 0;
+  int64_t long_cells = 0;
   // Work per rows-per-lane bucket (index R-1), costliest first: items of two int32, a read and
   // a second read of the same region sharing the wave (-1 when the read is alone).
   std::vector<int32_t> bucket_items[kMaxRowsPerLane];

@@ -315,11 +315,8 @@ int build_plan(const BatchInput& in, HostPlan& p) {
                                         " has no bases");
         return kInvalidArgument;
       }
-      if (n > kMaxReadLen) {
-        set_error(kInvalidArgument, "read " + std::to_string(r - in.region_read[g]) + " has " +
-                                        std::to_string(n) +
-                                        " bases; the GPU engine supports reads of at most " +
-                                        std::to_string(kMaxReadLen));
+      if (n > (int64_t(1) << 30)) {
+        set_error(kInvalidArgument, "read too long");
         return kInvalidArgument;
       }
     }
@@ -391,6 +388,11 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     std::vector<BwdSweep> bsweeps;
     std::vector<int32_t> bwrites, region_bw;
     int32_t max_slots = 0, max_nodes = 0, max_hap_len = 0;
+    std::vector<int32_t> long_pairs;
+    std::vector<int64_t> region_long_pairs;  // pairs of each region in long_pairs
+    std::vector<char> region_is_long;
+    int32_t long_max_hap = 0;
+    int64_t long_cells = 0;
     int64_t work_nominal = 0, work_prefix = 0, work_forward = 0, work_backward = 0;
     std::vector<int32_t> bucket_items[kMaxRowsPerLane];
     int64_t bucket_pairs[kMaxRowsPerLane] = {}, bucket_cells[kMaxRowsPerLane] = {},
@@ -407,11 +409,19 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       q.region_bw.push_back(int32_t(q.bsweeps.size()));
       p.region_hap0[size_t(g)] = int32_t(h0);
       p.region_nhap[size_t(g)] = int32_t(H);
+      // A region the staged sweep cannot hold runs every pair in long_kernel, unshared.
+      bool is_long = false;
+      for (int64_t r = r0; r < r1; ++r)
+        is_long |= in.read_off[r + 1] - in.read_off[r] > kMaxReadLen;
+      for (int64_t h = h0; h < h1; ++h) is_long |= rh.len(h) > kMaxStagedHap;
       for (int64_t r = r0; r < r1; ++r) {
         p.read_region[size_t(r)] = int32_t(g);
         p.read_local[size_t(r)] = int32_t(r - r0);
-        p.read_rows[size_t(r)] = rows_per_lane(in.read_off[r + 1] - in.read_off[r]);
+        p.read_rows[size_t(r)] =
+            is_long ? 0 : rows_per_lane(in.read_off[r + 1] - in.read_off[r]);
       }
+      q.region_long_pairs.push_back(is_long ? (r1 - r0) * H : 0);
+      q.region_is_long.push_back(char(is_long));
       if (H == 0) continue;
 
       // Stable sort, as Rust's sort_by (kernel.rs:55, pd_kernel.rs:72-76).
@@ -433,6 +443,10 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       std::map<int64_t, int32_t> slot_of;
       for (auto& kv : counts)
         if (kv.first > 0) slot_of.emplace(kv.first, int32_t(slot_of.size()));
+      // Beyond kMaxRegionSlots snapshot columns the region shares no prefixes: every haplotype
+      // sweeps from column 0 (duplicates still copy).  A long region shares nothing.
+      const bool share_prefix = !is_long && slot_of.size() <= size_t(kMaxRegionSlots);
+      if (!share_prefix) slot_of.clear();
       std::vector<int32_t> last_len(slot_of.size(), 0);
       q.max_slots = std::max<int32_t>(q.max_slots, int32_t(slot_of.size()));
 
@@ -450,9 +464,15 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         hp.nsnap = 0;
         hp.cut = 0;
         hp.cut_node = 0;
+        if (is_long) {
+          hp.lcp = 0;
+          hp.dup = 0;
+          q.long_max_hap = std::max(q.long_max_hap, hp.len);
+          continue;
+        }
         q.max_hap_len = std::max(q.max_hap_len, hp.len);
         if (hp.dup) continue;
-        const int64_t start = k == 0 ? 0 : lcp[size_t(k)];
+        const int64_t start = k == 0 || !share_prefix ? 0 : lcp[size_t(k)];
         hp.lcp = int32_t(start);
         if (k > 0) counts[start] -= 1;  // kernel.rs:248-250
         if (start > 0) {
@@ -468,6 +488,22 @@ int build_plan(const BatchInput& in, HostPlan& p) {
           }
         }
       }
+      if (is_long) {
+        int64_t read_bases = 0, hap_bases = 0;
+        for (int64_t r = r0; r < r1; ++r) {
+          read_bases += in.read_off[r + 1] - in.read_off[r];
+          for (int64_t k = 0; k < H; ++k) {
+            q.long_pairs.push_back(int32_t(r));
+            q.long_pairs.push_back(int32_t(k));
+          }
+        }
+        for (int64_t h = h0; h < h1; ++h) hap_bases += rh.len(h);
+        q.long_cells += read_bases * hap_bases;
+        q.work_nominal += read_bases * hap_bases;
+        q.work_prefix += read_bases * hap_bases;
+        q.work_forward += read_bases * hap_bases;
+        continue;
+      }
       if (!in.pd && H > 1 && !in.pairs_only) {
         // Suffix sharing for plain haplotypes (the reference shares suffixes in its plain kernel
         // only, kernel.rs:53-81); the kernel applies it in the FP32 pass with prefix sharing.
@@ -481,6 +517,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
           dp[size_t(k)] = char(hp.dup);
         }
         const SuffixPlan sp = plan_suffixes(hb, hl, lcp, dp);
+        if (sp.nodes - 1 > kMaxRegionSlots) continue;  // too many node records: no suffix sharing
         for (int64_t k = 0; k < H; ++k) {
           const int32_t d = sp.depth[size_t(k)];
           if (d > 0) {
@@ -523,6 +560,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     for (int64_t g = g0; g < g1; ++g) {
       int64_t cols = 0, swept = 0, hap_bases = 0;
       const int32_t H = p.region_nhap[size_t(g)];
+      if (q.region_is_long[size_t(g - g0)]) continue;  // every pair computed in long_kernel
       for (int32_t k = 0; k < H; ++k) {
         const HapPlan& hp = p.plan[size_t(p.region_hap0[size_t(g)] + k)];
         hap_bases += hp.len;
@@ -605,6 +643,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     plan_range(G * t / threads, G * (t + 1) / threads, parts[size_t(t)]);
   });
   std::vector<int64_t> item_costs[kMaxRowsPerLane];
+  p.region_long.push_back(0);  // region g's long pairs: [region_long[g], region_long[g + 1])
   for (int t = 0; t < threads; ++t) {
     Part& q = parts[size_t(t)];
     const int64_t g0 = G * t / threads, g1 = G * (t + 1) / threads;
@@ -619,6 +658,11 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     p.max_slots = std::max(p.max_slots, q.max_slots);
     p.max_nodes = std::max(p.max_nodes, q.max_nodes);
     p.max_hap_len = std::max(p.max_hap_len, q.max_hap_len);
+    for (int64_t pairs : q.region_long_pairs)
+      p.region_long.push_back(p.region_long.back() + pairs);
+    p.long_pairs.insert(p.long_pairs.end(), q.long_pairs.begin(), q.long_pairs.end());
+    p.long_max_hap = std::max(p.long_max_hap, q.long_max_hap);
+    p.long_cells += q.long_cells;
     p.work_nominal += q.work_nominal;
     p.work_prefix += q.work_prefix;
     p.work_forward += q.work_forward;

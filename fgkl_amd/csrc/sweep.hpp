@@ -922,6 +922,172 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Pairs beyond the staged layout: reads of more than 511 bases, haplotypes longer than the LDS
+// staging holds, and every other pair of their regions (which then share nothing).  One
+// (read, haplotype) pair per wave, from column 0.  The read's rows are laid out as in sweep_kernel
+// at 8 rows per lane over V = ceil((n + 1) / 8) virtual lanes (identity rows at the top of
+// virtual lane 0), and the wave sweeps them 64 virtual lanes (one strip) at a time, top strip
+// first.  A strip's last lane stores its last row's values at every column into the wave's
+// boundary buffer in HBM; the next strip's lane 0 takes them as the row above its first row.
+// Lane 0's inputs (the haplotype column word, the boundary row) are loaded 64 columns at a time,
+// one column per lane, and handed to lane 0 with readlane, so neither length is bounded by LDS.
+
+// Values of one boundary column: M, X, Y and (PD) the branch values.
+template <bool PD> constexpr int bnd_values() { return PD ? 6 : 3; }
+
+// The row above lane 0's first row comes from `above` (uniform); every other lane's from the lane
+// before it, as shift_up moves it.
+template <typename T, int R, bool PD>
+__device__ __forceinline__ void shift_up_from(const Lane<T, R, PD>& s, Cell<T>& u,
+                                              const Cell<T>& above) {
+  u.m = shr1(above.m, s.m[R - 1]);
+  u.x = shr1(above.x, s.x[R - 1]);
+  u.y = shr1(above.y, s.y[R - 1]);
+  if constexpr (PD) {
+    u.bm = shr1(above.bm, s.bm[R - 1]);
+    u.bx = shr1(above.bx, s.bx[R - 1]);
+    u.by = shr1(above.by, s.by[R - 1]);
+  }
+}
+
+template <typename T, bool PD>
+__global__ void __launch_bounds__(64) long_kernel(SweepArgs a, DevTab<T> tab) {
+  constexpr int R = kMaxRowsPerLane;
+  constexpr int NB = bnd_values<PD>();
+  const int lane = threadIdx.x & 63;
+  const int64_t wave_global = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  T* const bnd = static_cast<T*>(a.scratch) + wave_global * a.scratch_wave_stride;
+  const int64_t bnd_half = a.scratch_wave_stride / 2;  // one boundary buffer: (W + 1) x NB
+  const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
+
+  for (;;) {
+    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, 1)) >> 6;
+    if (item >= n_items) break;
+    const int ra = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
+    const int k = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
+    const int64_t o = a.read_off[ra];
+    const int n = __builtin_amdgcn_readfirstlane(int(a.read_off[ra + 1] - o));
+    const int g = __builtin_amdgcn_readfirstlane(a.read_region[ra]);
+    const int hap0 = __builtin_amdgcn_readfirstlane(a.region_hap0[g]);
+    const int H = __builtin_amdgcn_readfirstlane(a.region_nhap[g]);
+    const int64_t out_idx = a.region_out[g] + int64_t(a.read_local[ra]) * H;
+    HapPlan p = a.plan[hap0 + k];
+    const int W = __builtin_amdgcn_readfirstlane(p.len);
+    const int64_t hoff = p.off;
+    const int src = __builtin_amdgcn_readfirstlane(p.src);
+    const int V = (n + R) / R;        // virtual lanes: n rows plus at least one identity row
+    const int pad = R * V - n;        // identity rows at the top of virtual lane 0
+    const int S = (V + 63) / 64;      // strips
+    const T init = T(Num<T>::kScale / double(W));  // kernel.rs:640
+
+    T* bin = bnd;
+    T* bout = bnd + bnd_half;
+    double raw = 0.0;
+    for (int st = 0; st < S; ++st) {
+      const int vl = 64 * st + lane;
+      Rows<T, R, PD> rw;
+      load_rows<T, R, PD>(a, tab, o, V, pad, vl, rw);
+      const int L = min(64, V - 64 * st);  // live lanes of this strip
+      const bool last_strip = st == S - 1;
+      Lane<T, R, PD> s;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        s.m[r] = T(0);
+        s.x[r] = T(0);
+        // Column 0: identity rows carry row 0's deletion value C/m; real rows are zero.
+        s.y[r] = (rw.sh[r] == kIdentShift && vl < V) ? init : T(0);
+        if constexpr (PD) {
+          s.bm[r] = T(0);
+          s.bx[r] = T(0);
+          s.by[r] = T(0);
+        }
+      }
+      s.acc = T(0);
+      s.w = 0;
+      Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};
+      // Identity rows fill virtual lane 0: read row 1's diagonal in column 1 is DP row 0's C/m.
+      if (st == 0 && lane == 1 && pad == R) P.y = init;
+      const int steps = W + L - 1;
+      // Lane l's loads for a chunk starting at step cb: the word of column cb + l + 1 and the
+      // boundary row at that column (zero past column W and in the top strip).
+      auto load_word = [&](int cb) -> uint32_t {
+        const int j = cb + lane;  // 0-based column index
+        if (j >= W) return 0u;
+        return PD ? a.hap_words[hoff + j] : uint32_t(a.hap_bytes[hoff + j]);
+      };
+      auto load_bnd = [&](int cb, T* v) {
+        const int c = cb + lane + 1;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) v[q] = (st > 0 && c <= W) ? bin[int64_t(c) * NB + q] : T(0);
+      };
+      uint32_t wv_next = load_word(0);
+      T bv_next[NB];
+      load_bnd(0, bv_next);
+      for (int cb = 0; cb < steps; cb += 64) {
+        const uint32_t wv = wv_next;
+        T bv[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) bv[q] = bv_next[q];
+        wv_next = load_word(cb + 64);
+        load_bnd(cb + 64, bv_next);
+        const int nsteps = min(64, steps - cb);
+        for (int i = 0; i < nsteps; ++i) {
+          const int t = cb + i;
+          s.w = shr1(uint32_t(__builtin_amdgcn_readlane(int(wv), i)), s.w);
+          Cell<T> above{T(0), T(0), T(0), T(0), T(0), T(0)};
+          above.m = readlane(bv[0], i);
+          above.x = readlane(bv[1], i);
+          above.y = readlane(bv[2], i);
+          if constexpr (PD) {
+            above.bm = readlane(bv[3], i);
+            above.bx = readlane(bv[4], i);
+            above.by = readlane(bv[5], i);
+          }
+          Cell<T> Q;
+          shift_up_from(s, Q, above);
+          const int c = t - lane + 1;  // the column this lane computes at step t
+          if (c >= 1 && c <= W) {
+            cells(s, rw, P, Q);
+            P = Q;
+            // A strip's last lane hands its last row at column c to the next strip.
+            if (!last_strip && lane == 63) {
+              T* d = bout + int64_t(c) * NB;
+              d[0] = s.m[R - 1];
+              d[1] = s.x[R - 1];
+              d[2] = s.y[R - 1];
+              if constexpr (PD) {
+                d[3] = s.bm[R - 1];
+                d[4] = s.bx[R - 1];
+                d[5] = s.by[R - 1];
+              }
+            }
+          }
+        }
+      }
+      if (last_strip) raw = double(readlane(s.acc, L - 1));
+      // The next strip reads what this one's last lane stored.
+      wave_sync_global();
+      T* tmp = bin;
+      bin = bout;
+      bout = tmp;
+    }
+    double res;
+    if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted))
+      res = __builtin_nan("");  // kernel.rs:897-914
+    else
+      res = log10(raw) - Num<T>::kLog10Scale;
+    if (lane == 0) {
+      a.out[out_idx + src] = res;
+      if (a.rescue && res != res) {
+        const int idx = atomicAdd(a.rescue_count, 1);
+        a.rescue_pairs[0][2 * idx] = ra;
+        a.rescue_pairs[0][2 * idx + 1] = k;
+      }
+    }
+  }
+}
+
 using KernelF = void (*)(SweepArgs, DevTab<float>);
 using KernelD = void (*)(SweepArgs, DevTab<double>);
 
@@ -929,5 +1095,8 @@ using KernelD = void (*)(SweepArgs, DevTab<double>);
 // for FP64 (sweep_f64.hip).
 extern const KernelF kSweepF32[2][2][kMaxRowsPerLane];
 extern const KernelD kSweepF64[2][kMaxRowsPerLane];
+// long_kernel by [PD].
+extern const KernelF kLongF32[2];
+extern const KernelD kLongF64[2];
 
 }  // namespace fgkl

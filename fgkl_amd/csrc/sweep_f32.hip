@@ -16,4 +16,6 @@ const KernelF kSweepF32[2][2][kMaxRowsPerLane] = {
 
 #undef FGKL_F32_ROW
 
+const KernelF kLongF32[2] = {long_kernel<float, false>, long_kernel<float, true>};
+
 }  // namespace fgkl

@@ -14,4 +14,6 @@ const KernelD kSweepF64[2][kMaxRowsPerLane] = {FGKL_F64_ROW(false), FGKL_F64_ROW
 
 #undef FGKL_F64_ROW
 
+const KernelD kLongF64[2] = {long_kernel<double, false>, long_kernel<double, true>};
+
 }  // namespace fgkl

@@ -125,12 +125,16 @@ def test_empty_inputs_produce_no_output_without_touching_a_gpu():  # FgklPairHmm
                      np.array([0], np.int32), False, np.zeros(0))
 
 
-def test_reads_longer_than_the_engine_supports_are_a_clear_error():
-    n = 513
-    b = np.full(n, ord("A"), np.uint8)
-    with pytest.raises((F.InvalidArgument, F.NativeError)):
-        F.compute_native(b, b, b, b, b, np.array([0, n], np.int32), b, np.array([0, n], np.int32),
-                         False, np.zeros(1))
+@pytest.mark.skipif(F.device_count() > 0, reason="checks the no-GPU host")
+def test_long_reads_and_haplotypes_pass_validation():
+    """lib.rs:55-67 validates only emptiness and matching lengths: reads of more than 511 bases
+    and haplotypes of thousands are valid (without a GPU the call then fails for want of one)."""
+    for n, m in ((513, 600), (2000, 2100), (150, 5000)):
+        b = np.full(n, ord("A"), np.uint8)
+        h = np.full(m, ord("A"), np.uint8)
+        with pytest.raises(F.NativeError, match="no HIP device"):
+            F.compute_native(b, b, b, b, b, np.array([0, n], np.int32), h,
+                             np.array([0, m], np.int32), False, np.zeros(1))
 
 
 @pytest.mark.skipif(F.device_count() > 0, reason="checks the no-GPU host")
