@@ -247,7 +247,7 @@ class PdPairHmm(_Engine):
 
 # ---- the JNI-shaped entry points (crates/jni/src/lib.rs:159-254) ------------------------------
 def _native(pd, bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_flags, hap_offsets,
-            double_precision, likelihoods):
+            double_precision, likelihoods, noalloc=False):
     arrs = [_u8(x) for x in (bases, quals, ins, dels, gcp)]
     ro = np.ascontiguousarray(read_offsets, dtype=np.int32)
     ho = np.ascontiguousarray(hap_offsets, dtype=np.int32)
@@ -261,7 +261,10 @@ def _native(pd, bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_flags
         args += [_ptr(hf, C.c_uint8), len(hf)]
     args += [_ptr(ho, C.c_int32), len(ho), int(bool(double_precision)),
              likelihoods.ctypes.data_as(C.POINTER(C.c_double)), len(likelihoods)]
-    fn = lib().fgkl_pdhmm_compute_native if pd else lib().fgkl_pairhmm_compute_native
+    if noalloc:
+        fn = lib().fgkl_pairhmm_compute_native_noalloc
+    else:
+        fn = lib().fgkl_pdhmm_compute_native if pd else lib().fgkl_pairhmm_compute_native
     _check(fn(*args))
 
 
@@ -270,6 +273,14 @@ def compute_native(bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_of
     """FgklPairHmm.computeNative (crates/jni/src/lib.rs:159-195)."""
     _native(False, bases, quals, ins, dels, gcp, read_offsets, hap_bases, None, hap_offsets,
             double_precision, likelihoods)
+
+
+def compute_native_noalloc(bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_offsets,
+                           double_precision, likelihoods: np.ndarray) -> None:
+    """compute_native without allocating device memory (include/fgkl.h,
+    fgkl_pairhmm_compute_native_noalloc)."""
+    _native(False, bases, quals, ins, dels, gcp, read_offsets, hap_bases, None, hap_offsets,
+            double_precision, likelihoods, noalloc=True)
 
 
 def pd_compute_native(bases, quals, ins, dels, gcp, read_offsets, hap_bases, hap_flags,

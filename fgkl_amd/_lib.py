@@ -67,6 +67,11 @@ SIGNATURES = {
         [u8p, C.c_int64] * 5 + [i32p, C.c_int64, u8p, C.c_int64, u8p, C.c_int64, i32p,
                                  C.c_int64, C.c_int, f64p, C.c_int64],
     ),
+    "fgkl_pairhmm_compute_native_noalloc": (
+        C.c_int,
+        [u8p, C.c_int64] * 5 + [i32p, C.c_int64, u8p, C.c_int64, i32p, C.c_int64, C.c_int,
+                                 f64p, C.c_int64],
+    ),
     "fgkl_pairhmm_new": (C.c_void_p, [C.c_int, C.c_int, C.c_int]),
     "fgkl_pairhmm_free": (None, [C.c_void_p]),
     "fgkl_pairhmm_fallback_pairs": (C.c_uint64, [C.c_void_p]),

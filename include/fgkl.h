@@ -74,6 +74,31 @@ int fgkl_pdhmm_compute_native(const uint8_t* bases, int64_t bases_len,
                               double* likelihoods, int64_t likelihoods_len);
 
 /*
+ * The allocation-free variant of fgkl_pairhmm_compute_native (no counterpart in the reference:
+ * it is for callers that cannot allocate device memory, such as serving loops and harnesses that
+ * forbid hipMalloc).  Same arguments, validation, messages and results (bit for bit) as
+ * fgkl_pairhmm_compute_native; it calls no hipMalloc, hipFree or other allocating routine.
+ * Every device byte beyond the caller's arrays is a static array of the library: 32 MiB of
+ * scratch per GPU, held from module load to process exit, plus the Phred tables.  The first call
+ * on a device (with that device current) looks up their addresses and uploads the tables.
+ * Limits: reads of at most 511 bases, haplotypes of at most 2048, at most 1048576 pairs, and a
+ * region whose arrays, pair lists and results fit the 32 MiB; beyond them it returns
+ * FGKL_INVALID_ARGUMENT with a message naming the limit (fgkl_pairhmm_compute_native takes them).
+ * Calls on one device share its scratch: this library serialises them, and work another process
+ * or caller enqueues on that memory meanwhile would race.
+ */
+int fgkl_pairhmm_compute_native_noalloc(const uint8_t* bases, int64_t bases_len,
+                                        const uint8_t* quals, int64_t quals_len,
+                                        const uint8_t* ins_gop, int64_t ins_gop_len,
+                                        const uint8_t* del_gop, int64_t del_gop_len,
+                                        const uint8_t* gcp, int64_t gcp_len,
+                                        const int32_t* read_offsets, int64_t read_offsets_len,
+                                        const uint8_t* hap_bases, int64_t hap_bases_len,
+                                        const int32_t* hap_offsets, int64_t hap_offsets_len,
+                                        int double_precision, double* likelihoods,
+                                        int64_t likelihoods_len);
+
+/*
  * A configured PairHMM or PD-HMM, replacing PairHmm::new / PdPairHmm::new with Config
  * {precision, double_fallback} (crates/pairhmm/src/lib.rs:213-265, pd.rs:41-66).
  * pd selects the PD kernel; double_precision the FP64 engine; double_fallback recomputes
