@@ -510,10 +510,9 @@ struct DeviceBatch::Impl {
     return launch_long<double>(a, n_long);
   }
 
-  // Largest rows per lane bucket R may run at.  PD batches keep R: with prefix sharing a PD
-  // read's last bits can change with its lane layout (seen as 4e-8 in log10 on 2 of 7444
-  // pairs), while plain reads are bitwise the same at R and R + 1.
-  int max_rows(int R) const { return pd ? R : std::min(R + 1, kMaxRowsPerLane); }
+  // Largest rows per lane bucket R may run at.  A read's rows, plain or PD, see the same
+  // operands in the same order at R and R + 1, so both give the same bits.
+  int max_rows(int R) const { return std::min(R + 1, kMaxRowsPerLane); }
 
   // The FP32 configurations of bucket R that the tuner times (the default first), or, with
   // `all`, every one set_launch accepts.

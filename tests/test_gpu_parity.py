@@ -218,6 +218,31 @@ def test_a_read_gets_the_same_bits_alone_and_in_any_batch_size(gpu):  # lib.rs:9
                 assert np.array_equal(allr[i * len(haps):(i + 1) * len(haps)], solo[i]), (p, n, i)
 
 
+@pytest.mark.parametrize("share", [True, False])
+def test_a_pd_read_gets_the_same_bits_whatever_shares_its_batch(gpu, share):  # pd.rs:400-426
+    """Three PD reads alone, and with 29 reads of random bases (which underflow in FP32 and are
+    rescued) in one region, as a batch with and without prefix sharing: the three reads' results
+    are the same bits in both precisions.  The random reads change the pairing of reads into
+    waves and the rows per lane the three run at."""
+    region = PdRegion.generate(31, 70, 120, 24, 300)
+    haps = region.region_pd_haps_list()
+    alone = region.region_reads_list()[:3]
+    rng = Rng(77)
+    others = [F.ReadRef(bytes(rng.base() for _ in range(120)), [30] * 120, [45] * 120,
+                        [45] * 120, [10] * 120) for _ in range(29)]
+    for mode in (F.MODE_FP32_RESCUE, F.MODE_FP64):
+        outs = []
+        for reads in (alone, alone + others):
+            b = F.Batch(batch_from_regions([(reads, haps)], pd=True))
+            st = b.run(mode, share=share)
+            outs.append(b.results())
+            b.close()
+        n = len(alone) * len(haps)
+        assert np.array_equal(outs[0], outs[1][:n]), (mode, share)
+        if mode == F.MODE_FP32_RESCUE:
+            assert st.rescued_pairs > 0, "the random reads must underflow"
+
+
 def test_a_read_matching_only_the_previous_haplotype_is_recomputed_in_double(gpu):  # lib.rs:1019-1056
     rng = Rng(8)
     reference = bytes(rng.base() for _ in range(300))
@@ -871,14 +896,15 @@ def test_two_reads_sharing_a_wave_match_the_oracle(gpu, pd):
     b.close()
 
 
-def _launch_space(R, pd=False):
-    # PD buckets keep R rows per lane (their last bits may depend on the lane layout).
-    rows = [R] + ([R + 1] if R < 8 and not pd else [])
+def _launch_space(R):
+    rows = [R] + ([R + 1] if R < 8 else [])
     return [(r, b, o) for r in rows for b in (256, 128, 64) for o in (False, True)]
 
 
 @pytest.mark.parametrize("wl", [1, 3])
 def test_every_launch_configuration_gives_the_same_bits(gpu, wl):
+    """Every rows per lane (R and R + 1), block size and build of every FP32 bucket gives the
+    bits of the default launch, with prefix (and, plain, suffix) sharing; PD included."""
     data = workload(wl, seed=808 + wl, n_regions=5)
     b = F.Batch(data)
     for R in range(1, 9):
@@ -890,7 +916,7 @@ def test_every_launch_configuration_gives_the_same_bits(gpu, wl):
     used = [R for R in range(1, 9) if st.bucket_cells[R - 1] > 0]
     assert len(used) >= 3
     for R in used:
-        for rows, block, boost in _launch_space(R, pd=data.pd):
+        for rows, block, boost in _launch_space(R):
             b.set_launch(R, rows, block, boost)
             assert b.get_launch(R) == (rows, block, boost)
             b.run(F.MODE_FP32_RESCUE, share=True)
@@ -929,8 +955,10 @@ def test_tuning_keeps_results_and_is_inherited_by_later_batches(gpu):
     with pytest.raises(F.InvalidArgument):
         b.set_launch(2, block=512)
     pd = F.Batch(workload(3, seed=4244, n_regions=2))
+    pd.set_launch(1, rows=2)  # PD buckets take R + 1 like plain ones
+    assert pd.get_launch(1)[0] == 2
     with pytest.raises(F.InvalidArgument):
-        pd.set_launch(1, rows=2)
+        pd.set_launch(1, rows=3)
     pd.close()
     later.close()
     b.close()
