@@ -31,9 +31,21 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "PairHMM GCUPS, batched FP32+FP64-rescue vs plain FP64, 1 GPU, |Δlog10L|≤1e-4"
 FLOPS_PER_CELL = 12          # 4 mul + 4 FMA per cell (reference.rs:49-54)
-PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed) peak
-PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3     # MI355X published vector FP32 peak (it counts packed FP32 math)
+PEAK_FP64_TFLOPS = 78.6      # MI355X published vector FP64 peak
 ADJOINT_FLOPS_PER_CELL = 36  # forward recompute 12 + reverse recurrence 12 + gradient terms 12
+
+
+# The measured vector issue rates (valu_peak), filled in by main before any workload runs.
+MEASURED: dict = {}
+
+
+def vs_measured(achieved_tflops: float, key: str) -> dict:
+    """The achieved rate against the measured issue rate of the same instruction kind."""
+    peak = MEASURED.get(key)
+    if not peak:
+        return {}
+    return {"peak_measured": peak, "frac_of_measured": round(achieved_tflops / peak, 4)}
 
 
 def log(msg: str) -> None:
@@ -139,6 +151,33 @@ def cpu_baseline(data, args, fp32_res) -> dict:
     }
 
 
+def valu_peak() -> dict:
+    """The vector ALU's measured issue rate on this device (fgkl_measure_valu_rate): unpacked
+    FP32 FMAs, FP64 FMAs and 32-bit integer ops from a full grid of dependence-free chains.  It
+    is the one issue model every roofline here is also quoted against, beside the published
+    peaks: the library's kernels run with packed FP32 disabled."""
+    import fgkl_amd as F
+
+    f32, f32_ms = F.measure_valu_rate(F.VALU_FP32_FMA)
+    f64, f64_ms = F.measure_valu_rate(F.VALU_FP64_FMA)
+    u32, u32_ms = F.measure_valu_rate(F.VALU_U32)
+    cus = 256
+    out = {
+        "fp32_fma_tflops": round(2 * f32 / 1e12, 2),
+        "fp64_fma_tflops": round(2 * f64 / 1e12, 2),
+        "u32_tops": round(u32 / 1e12, 2),
+        "fp32_wave_instructions_per_s": float(f"{f32 / 64:.4g}"),
+        "cycles_per_wave64_fp32_at_2400mhz": round(cus * 4 * 2.4e9 / (f32 / 64), 3),
+        "cycles_per_wave64_fp64_at_2400mhz": round(cus * 4 * 2.4e9 / (f64 / 64), 3),
+        "cycles_per_wave64_u32_at_2400mhz": round(cus * 4 * 2.4e9 / (u32 / 64), 3),
+        "launch_ms": {"fp32": round(f32_ms, 3), "fp64": round(f64_ms, 3), "u32": round(u32_ms, 3)},
+        "method": "256 CUs x 8 waves per SIMD, 16 independent chains per lane, packed FP32 off",
+    }
+    log(f"valu peak: fp32 fma {out['fp32_fma_tflops']} TFLOP/s, fp64 fma "
+        f"{out['fp64_fma_tflops']} TFLOP/s, u32 {out['u32_tops']} Tops/s")
+    return out
+
+
 def adjoint_bench(args) -> dict:
     """The gradient engine on the adjoint workload: FP64 forward recompute, reverse sweep and
     gradient accumulation, timed by HIP events around its kernels."""
@@ -167,7 +206,8 @@ def adjoint_bench(args) -> dict:
         "forward_ms": round(fwd_ms, 3), "forward_gcups": round(cells / fwd_ms / 1e6, 2),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "flops_per_cell": ADJOINT_FLOPS_PER_CELL},
+                     "flops_per_cell": ADJOINT_FLOPS_PER_CELL,
+                     **vs_measured(achieved, "fp64_fma_tflops")},
         "finite": bool(np.isfinite(ge).all() and np.isfinite(gi).all() and np.isfinite(gd).all()),
     }
 
@@ -218,7 +258,8 @@ def side_bench(args, config: int) -> dict:
         "fp64_mode_gcups": round(cells / (st64.total_ms * 1e-3) / 1e9, 2),
         "roofline_fp32_sweeps": {"bound": "valu", "achieved": round(achieved, 3),
                                  "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4)},
+                                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                                 **vs_measured(achieved, "fp32_fma_tflops")},
         "parity": {"max_abs_dlog10L_vs_fp64": float(d.max()), "pairs_over_1e-4": int((d > 1e-4).sum()),
                    "nonfinite": int((~np.isfinite(eager)).sum())},
     }
@@ -423,7 +464,7 @@ def sw_bench(args) -> dict:
     achieved = best * 1e9 * SW_VALU_PER_CELL / 1e12  # lane ops: one cell per lane per row
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": SW_PEAK_INT32_TOPS,
                        "unit": "Tops/s (32-bit lane ops)", "frac": round(achieved / SW_PEAK_INT32_TOPS, 4),
-                       "valu_per_cell": SW_VALU_PER_CELL}
+                       "valu_per_cell": SW_VALU_PER_CELL, **vs_measured(achieved, "u32_tops")}
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         from _oracle import cigar_text, sw_align, sw_port_backend, sw_port_raw
@@ -498,6 +539,10 @@ def main() -> int:
     ap.add_argument("--sw-large-pairs", type=int, default=16384)
     ap.add_argument("--sw-large-only", action="store_true",
                     help="time only the larger Smith-Waterman batch (profiling runs)")
+    ap.add_argument("--pd-only", action="store_true",
+                    help="run only the PD-HMM workload and print its JSON (profiling runs)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the gradient workload and print its JSON (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -515,8 +560,15 @@ def main() -> int:
         print("bench.py measures one MI355X; the path shards by region with no exchange "
               "between GPUs; run with --gpus 1", file=sys.stderr)
         return 2
+    MEASURED.update(valu_peak())
     if args.sw_only:
         print(json.dumps({"smith_waterman": sw_bench(args)}), flush=True)
+        return 0
+    if args.pd_only:
+        print(json.dumps({"pd": side_bench(args, 3)}), flush=True)
+        return 0
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench(args)}), flush=True)
         return 0
 
     import fgkl_amd as F
@@ -578,6 +630,7 @@ def main() -> int:
 
     result = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1,
+        "valu_peak_measured": dict(MEASURED),
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (f64 rescue)", "data": "synthetic",
@@ -612,6 +665,10 @@ def main() -> int:
         "achieved_computed": round(FLOPS_PER_CELL * bw[dom] / (bucket_ms[dom] * 1e-3) / 1e12, 3),
         "frac_computed": round(FLOPS_PER_CELL * bw[dom] / (bucket_ms[dom] * 1e-3) / 1e12
                                / PEAK_FP32_TFLOPS, 4),
+        **vs_measured(achieved, "fp32_fma_tflops"),
+        "frac_computed_of_measured": round(
+            FLOPS_PER_CELL * bw[dom] / (bucket_ms[dom] * 1e-3) / 1e12
+            / MEASURED["fp32_fma_tflops"], 4) if MEASURED.get("fp32_fma_tflops") else None,
         "launches": launches,
     }
     if tune is not None:
@@ -629,6 +686,19 @@ def main() -> int:
     result["rescue"] = {"pairs": rescued, "fraction": round(rescued / n_pairs, 6),
                         "fp64_time_share": round(rescue_ms / eager_ms, 4) if eager_ms else None}
     assert np.array_equal(graph_res, fp32_res), "graph replay differs from the eager run"
+
+    # The batched FP64 mode on the same inputs (every pair in FP64, prefix sharing, one launch
+    # set): it separates the FP32 gain from the sharing and batching gains in the speedup over
+    # the per-call FP64 engine.
+    st64 = batch.run(F.MODE_FP64, share=True)
+    fp64_batched = batch.results()
+    result["fp64_batched"] = {
+        "gcups": round(cells / (st64.total_ms * 1e-3) / 1e9, 2), "ms": round(st64.total_ms, 3),
+        "definition": "FP64 engine, every pair, prefix sharing, one launch per rows-per-lane "
+                      "bucket (HIP events)",
+        "max_abs_dlog10L_fp32_rescue_vs_fp64_batched": float(np.abs(fp32_res - fp64_batched).max())}
+    result["speedup_vs_fp64_batched"] = round(gcups / result["fp64_batched"]["gcups"], 2)
+    log(f"fp64 batched: {st64.total_ms:.1f} ms, {result['fp64_batched']['gcups']:.1f} GCUPS")
 
     if not args.no_baseline:
         st = batch.run(F.MODE_FP64_PER_REGION)

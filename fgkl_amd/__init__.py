@@ -313,6 +313,17 @@ def release_idle() -> int:
     return int(n)
 
 
+VALU_FP32_FMA, VALU_FP64_FMA, VALU_U32 = 0, 1, 2
+
+
+def measure_valu_rate(kind: int) -> tuple[float, float]:
+    """(lane operations per second, best launch ms) of a full grid issuing one kind of vector ALU
+    instruction (include/fgkl.h, fgkl_measure_valu_rate)."""
+    rate, ms = C.c_double(), C.c_double()
+    _check(lib().fgkl_measure_valu_rate(int(kind), C.byref(rate), C.byref(ms)))
+    return rate.value, ms.value
+
+
 POOL_PAIRHMM, POOL_PDHMM, POOL_SMITH_WATERMAN = 0, 1, 2
 
 

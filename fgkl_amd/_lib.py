@@ -108,6 +108,7 @@ SIGNATURES = {
     "fgkl_replay_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, i64p, i64p, i64p, u8p, u8p,
                                     u8p, u8p, u8p, i64p, u8p, u8p, f64p, C.c_int64]),
     "fgkl_synth_splitmix": (None, [C.c_uint64, C.c_int64, u64p]),
+    "fgkl_measure_valu_rate": (C.c_int, [C.c_int, f64p, f64p]),
     "fgkl_selftest_dpp": (C.c_int, [i32p]),
     "fgkl_selftest_dpp_shl": (C.c_int, [i32p]),
     "fgkl_grad_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, i64p, i64p, i64p, u8p, i64p,

@@ -278,6 +278,14 @@ int fgkl_replay_write(const char* path, int pd, int64_t n_regions, const int64_t
 /* SplitMix64 (synthetic.rs:19-40): n successive next_u64 draws from seed. */
 void fgkl_synth_splitmix(uint64_t seed, int64_t n, uint64_t* out);
 
+/*
+ * Measurement (no reference counterpart): the rate at which a full grid of waves issues vector
+ * ALU instructions on the current device, compiled as the library's kernels are (packed FP32
+ * off).  kind 0: FP32 FMA (v_fma_f32), 1: FP64 FMA (v_fma_f64), 2: 32-bit integer add and xor.
+ * Writes lane operations per second (an FMA counts once) and the best launch time in ms.
+ */
+int fgkl_measure_valu_rate(int kind, double* lane_ops_per_second, double* best_ms);
+
 /* Checks the DPP lane shift the kernels rely on: out[i] receives 1000+i-1, out[0] = -1. */
 int fgkl_selftest_dpp(int32_t* out64);
 /* Checks the opposite DPP lane shift (the adjoint's): out[i] receives 1000+i+1, out[63] = -1. */
