@@ -178,37 +178,59 @@ def valu_peak() -> dict:
     return out
 
 
+def _adjoint_run(prob, e, di, dd, g):
+    prob.backward(e, di, dd, g)  # warm-up
+    runs = [prob.backward(e, di, dd, g, timing=True) for _ in range(2)]
+    ms = min(r[-1] for r in runs)
+    for a, b in zip(runs[0][:5], runs[1][:5]):
+        assert np.array_equal(a, b), "adjoint runs differ"
+    fwd_ms = min(prob.forward(e, di, dd, g, timing=True)[1] for _ in range(2))
+    return runs[-1][:5], ms, fwd_ms
+
+
 def adjoint_bench(args) -> dict:
-    """The gradient engine on the adjoint workload: FP64 forward recompute, reverse sweep and
-    gradient accumulation, timed by HIP events around its kernels."""
+    """The gradient engine on the adjoint workload, timed by HIP events around its kernels: the
+    FP32 mode (forward recompute, tape and reverse sweep in FP32, gradient terms accumulated in
+    FP64, FP64 rerun of pairs below 1e-28) and the FP64 mode, and the FP32 mode's distance from
+    the FP64 mode's gradients."""
     from fgkl_amd.grad import GradProblem, probabilities
     from fgkl_amd.synthetic import workload
 
     data = workload(4, seed=args.seed, n_regions=args.adjoint_regions)
     cells = data.cells()
     e, di, dd, g = probabilities(data)
-    prob = GradProblem(data)
-    prob.backward(e, di, dd, g)  # warm-up
-    runs = [prob.backward(e, di, dd, g, timing=True) for _ in range(2)]
-    ms = min(r[-1] for r in runs)
-    ll, ge, gi, gd, gg, _ = runs[-1]
-    assert np.array_equal(ge, runs[0][1]) and gg == runs[0][4], "adjoint runs differ"
-    fwd_ms = min(prob.forward(e, di, dd, g, timing=True)[1] for _ in range(2))
-    achieved = ADJOINT_FLOPS_PER_CELL * cells / (ms * 1e-3) / 1e12
-    log(f"adjoint: {data.n_regions} regions, {data.n_pairs()} pairs, {cells:.3e} cells; "
-        f"backward {ms:.1f} ms ({cells / ms / 1e6:.1f} GCUPS), forward {fwd_ms:.1f} ms")
-    prob.close()
+    out = {}
+    for precision in ("float", "double"):
+        prob = GradProblem(data, precision=precision)
+        res, ms, fwd_ms = _adjoint_run(prob, e, di, dd, g)
+        prob.close()
+        achieved = ADJOINT_FLOPS_PER_CELL * cells / (ms * 1e-3) / 1e12
+        peak = PEAK_FP32_TFLOPS if precision == "float" else PEAK_FP64_TFLOPS
+        key = "fp32_fma_tflops" if precision == "float" else "fp64_fma_tflops"
+        log(f"adjoint {precision}: {data.n_regions} regions, {data.n_pairs()} pairs, "
+            f"{cells:.3e} cells; backward {ms:.1f} ms ({cells / ms / 1e6:.1f} GCUPS), "
+            f"forward {fwd_ms:.1f} ms")
+        out[precision] = (res, {
+            "dtype": "f32 (f64 accumulation, f64 rescue)" if precision == "float" else "f64",
+            "backward_ms": round(ms, 3), "gcups": round(cells / ms / 1e6, 2),
+            "forward_ms": round(fwd_ms, 3), "forward_gcups": round(cells / fwd_ms / 1e6, 2),
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": peak,
+                         "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                         "frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 4),
+                         "flops_per_cell": ADJOINT_FLOPS_PER_CELL, **vs_measured(achieved, key)},
+            "finite": bool(all(np.isfinite(x).all() for x in res[1:4])),
+        })
+    (r32, f32), (r64, f64) = out["float"], out["double"]
+    dist = {"max_abs_dlog10L": float(np.max(np.abs(r32[0] - r64[0]))) if len(r64[0]) else 0.0}
+    for name, a, b in (("e", r32[1], r64[1]), ("d_ins", r32[2], r64[2]), ("d_del", r32[3], r64[3])):
+        dist[f"max_rel_d_grad_{name}"] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    dist["rel_d_grad_g"] = float(abs(r32[4] - r64[4]) / max(abs(r64[4]), 1e-300))
+    log(f"adjoint float vs double: {dist}")
     return {
         "workload": "adjoint regions: R~U(1,200) reads of U(50,250) bp, H~U(2,8) haplotypes of "
                     "read_max+U(50,400) bp",
-        "regions": data.n_regions, "pairs": data.n_pairs(), "cells": cells, "dtype": "f64",
-        "backward_ms": round(ms, 3), "gcups": round(cells / ms / 1e6, 2),
-        "forward_ms": round(fwd_ms, 3), "forward_gcups": round(cells / fwd_ms / 1e6, 2),
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "flops_per_cell": ADJOINT_FLOPS_PER_CELL,
-                     **vs_measured(achieved, "fp64_fma_tflops")},
-        "finite": bool(np.isfinite(ge).all() and np.isfinite(gi).all() and np.isfinite(gd).all()),
+        "regions": data.n_regions, "pairs": data.n_pairs(), "cells": cells,
+        **f32, "fp64": f64, "fp32_vs_fp64": dist,
     }
 
 

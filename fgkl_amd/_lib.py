@@ -113,6 +113,7 @@ SIGNATURES = {
     "fgkl_selftest_dpp_shl": (C.c_int, [i32p]),
     "fgkl_grad_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, i64p, i64p, i64p, u8p, i64p,
                                    u8p]),
+    "fgkl_grad_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "fgkl_grad_pairs": (C.c_int64, [C.c_void_p]),
     "fgkl_grad_read_bases": (C.c_int64, [C.c_void_p]),
     "fgkl_grad_forward": (C.c_int, [C.c_void_p, f64p, f64p, f64p, C.c_double, f64p, f64p]),

@@ -3,12 +3,12 @@
 // probabilities and the global gap-continuation probability.  The reference has no gradient;
 // the model is its FP64 forward recurrence (reference.rs:20-64) in a continuous
 // parameterisation: priors (1 - e, e / 3) with N matching everything, M->M = 1 - d_ins - d_del,
-// M->I = d_ins, M->D = d_del, I->I = D->D = g, I->M = D->M = 1 - g, row 0 deletions 2^1020 / m.
+// M->I = d_ins, M->D = d_del, I->I = D->D = g, I->M = D->M = 1 - g, row 0 deletions C / m.
 //
 // Layout (one work item per wave64, a read against every haplotype of its region):
 //  * Read rows 1..n are cut into strips of 64; lane l of strip s owns row 64 s + l + 1.  One row
-//    per lane keeps the FP64 working set of the adjoint in registers.  DP row 0 is stored as the
-//    boundary row above strip 0.
+//    per lane keeps the adjoint's working set in registers.  DP row 0 is stored as the boundary
+//    row above strip 0.
 //  * Pass F sweeps each strip along anti-diagonals (at step t lane l is on column t - l + 1,
 //    the row above arriving by DPP wave_shr:1 from lane l-1; lane 0 takes it from the previous
 //    strip's last row, which lane 63 stored) and stores every lane's state once every K steps:
@@ -17,7 +17,13 @@
 //    block is recomputed from its checkpoint into a register tape, then swept in reverse.  The
 //    adjoint of a cell needs the row below's values one and two steps later, which arrive by
 //    DPP wave_shl:1 from lane l+1 (lane 63 reads the lower strip's top row, stored by its lane
-//    0).  Gradient terms accumulate in FP64 registers per row and are written once per strip.
+//    0).  Gradient terms are summed per block in the working precision and accumulated in FP64
+//    registers per row, written once per strip.
+//  * Two precisions.  FP64: C = 2^1020, K = 8, every pair.  FP32 (the forward, the tape and the
+//    adjoint): C = 2^120, K = 16, the adjoint seeded with w / L so that it is d ln L / d cell
+//    (its values then stay within FP32's normal range from the last row to the first); a pair
+//    whose FP32 likelihood falls below 1e-28 (kernel.rs:897-914's rescue threshold) is flagged,
+//    and the FP64 kernel reruns the flagged pairs of each read and adds their terms.
 //  * Checkpoint memory is n/64 strips x (m+63)/K blocks x 6 values per lane; the tape is K+1
 //    cells per lane in registers.  Work: forward twice plus the reverse sweep.
 #include <hip/hip_runtime.h>
@@ -35,11 +41,25 @@
 namespace fgkl {
 namespace {
 
-constexpr int kTape = 8;         // steps per checkpoint block, recomputed into registers
-constexpr double kInitial = 0x1p1020;  // reference.rs:12
-constexpr double kLog10Initial = 1020.0 * 0.301029995663981195213738894724493027;
+// Per precision: the row-0 scale C (reference.rs:12 for FP64, simd.rs:106 for FP32), the
+// smallest accepted scaled likelihood, and the checkpoint block of K steps.
+template <typename T> struct GT;
+template <> struct GT<double> {
+  static constexpr double kInit = 0x1p1020;
+  static constexpr double kLog10Init = 1020.0 * 0.301029995663981195213738894724493027;
+  static constexpr double kMinAccepted = 0.0;
+  static constexpr int kTape = 8;
+};
+template <> struct GT<float> {
+  static constexpr double kInit = 0x1p120;
+  static constexpr double kLog10Init = 120.0 * 0.301029995663981195213738894724493027;
+  static constexpr double kMinAccepted = 1e-28;
+  static constexpr int kTape = 16;
+};
+
 constexpr double kInvLn10 = 0.434294481903251827651128918916605082;
 constexpr int kHapPad = 80;      // zero bytes before the staged haplotype (columns down to -79)
+constexpr int kSlot = 32;        // bytes of one dummy slot (holds any staged record)
 
 struct GradArgs {
   const uint8_t* bases;
@@ -52,17 +72,25 @@ struct GradArgs {
   const int32_t* region_nhap;
   const int64_t* hap_off;
   const uint8_t* hap_bases;
-  const int32_t* items;      // reads, most work first
+  const int32_t* items;      // reads, most work first (or, rescuing, the reads with lost pairs)
   int32_t n_items;
+  const int32_t* n_items_dev;  // when set, the item count is read from device memory
   int32_t* counter;
   const double* weights;     // per pair, or null for all ones
   double* log10_out;         // per pair, or null
   double *g_err, *g_ins, *g_del;
   double* dg_part;           // per read
-  double* scratch;
-  int64_t wave_stride;       // doubles of scratch per wave
-  int64_t ck_off, bnd_off, abnd_off;  // sections of a wave's scratch, in doubles
-  int64_t dummy_off;         // 64 x 4 doubles that absorb the stores of lanes with none to make
+  // FP32: lost[pair] is set to 1 for a pair below the threshold (0 otherwise), and each read
+  // with one is appended to lost_items.  Rescuing (FP64): only pairs with lost[pair] run, and
+  // their terms are added to the gradients already written.
+  uint8_t* lost;
+  int32_t* lost_items;
+  int32_t* lost_count;
+  int32_t rescue;
+  char* scratch;
+  int64_t wave_stride;       // bytes of scratch per wave
+  int64_t ck_off, bnd_off, abnd_off;  // sections of a wave's scratch, in bytes
+  int64_t dummy_off;         // 64 slots that absorb the stores of lanes with none to make
   int32_t nb_max;            // checkpoint blocks per strip
   int32_t bnd_cols;          // columns of a stored boundary row
   int32_t lds_bytes;         // per wave
@@ -70,41 +98,42 @@ struct GradArgs {
 
 // A lane's read row: its priors and gap-open parameters, the row below's (the adjoint reads
 // them), and the global gap continuation g and 1 - g, which every row shares.
-struct GRow {
-  double pm, px, mm, mi, md, mmB, miB;
-  double g, ig;
+template <typename T> struct GRow {
+  T pm, px, mm, mi, md, mmB, miB;
+  T g, ig;
   uint32_t rb;
 };
 
-struct St {
-  double m, x, y;
+template <typename T> struct St {
+  T m, x, y;
 };
-struct alignas(32) St4 {
-  double m, x, y, pad;
+template <typename T> struct alignas(4 * sizeof(T)) St4 {
+  T m, x, y, pad;
 };
-struct alignas(16) Adj2 {
-  double ax, q;
+template <typename T> struct alignas(2 * sizeof(T)) Adj2 {
+  T ax, q;
 };
 
 // Row v (1-based) of the read at `o`; rows past the read are zero and stay zero.  Every lane
 // loads (from a clamped, valid index) and selects, so no lane-dependent branch is taken.
+template <typename T>
 __device__ __forceinline__ void load_grow(const GradArgs& a, int64_t o, int n, int v, double g,
-                                          GRow& rw) {
+                                          GRow<T>& rw) {
   const bool real = v <= n, below = v < n;
   const int64_t i = o + (real ? v : n) - 1;
   const int64_t j = below ? i + 1 : i;
   const double e = a.err[i], di = a.ins[i], dd = a.del[i];
   const double di2 = a.ins[j], dd2 = a.del[j];
   const uint32_t b = a.bases[i];
-  rw.pm = real ? 1.0 - e : 0.0;
-  rw.px = real ? e / 3.0 : 0.0;
-  rw.mm = real ? 1.0 - di - dd : 0.0;
-  rw.mi = real ? di : 0.0;
-  rw.md = real ? dd : 0.0;
-  rw.mmB = below ? 1.0 - di2 - dd2 : 0.0;
-  rw.miB = below ? di2 : 0.0;
-  rw.g = g;
-  rw.ig = 1.0 - g;
+  rw.pm = T(real ? 1.0 - e : 0.0);
+  rw.px = T(real ? e / 3.0 : 0.0);
+  rw.mm = T(real ? 1.0 - di - dd : 0.0);
+  rw.mi = T(real ? di : 0.0);
+  rw.md = T(real ? dd : 0.0);
+  rw.mmB = T(below ? 1.0 - di2 - dd2 : 0.0);
+  rw.miB = T(below ? di2 : 0.0);
+  rw.g = T(g);
+  rw.ig = T(1.0 - g);
   rw.rb = real ? b : 0x100u;
 }
 
@@ -112,112 +141,125 @@ __device__ __forceinline__ bool is_match(uint32_t hb, uint32_t rb) {
   return hb == rb || hb == uint32_t('N') || rb == uint32_t('N');
 }
 
-__device__ __forceinline__ St shr_st(const St4& old, const St& src) {
-  return St{shr1(old.m, src.m), shr1(old.x, src.x), shr1(old.y, src.y)};
+template <typename T>
+__device__ __forceinline__ St<T> shr_st(const St4<T>& old, const St<T>& src) {
+  return St<T>{shr1(old.m, src.m), shr1(old.x, src.x), shr1(old.y, src.y)};
 }
 
 // A wave's LDS: the staged boundary rows and haplotype, and the per-block staging of the values
 // one lane hands to the neighbouring strip.  The lane that owns a boundary writes its staging
 // slot each step; every other lane writes a private dummy slot, so the writes need no branch.
-struct Lds {
-  St4* stage;     // kTape + 1 columns of the row above the strip (lane 0's neighbour)
-  Adj2* astage;   // kTape columns of the adjoint of the row below the strip (lane 63's)
-  St4* fout;      // kTape columns of lane 63's state, for the strip below
-  Adj2* aout;     // kTape columns of lane 0's adjoint, for the strip above
-  uint8_t* dummy; // 64 + kTape slots of 32 bytes
-  uint8_t* hap;   // kHapPad zeros, the haplotype, zeros
+template <typename T> struct Lds {
+  St4<T>* stage;   // K + 1 columns of the row above the strip (lane 0's neighbour)
+  Adj2<T>* astage; // K columns of the adjoint of the row below the strip (lane 63's)
+  St4<T>* fout;    // K columns of lane 63's state, for the strip below
+  Adj2<T>* aout;   // K columns of lane 0's adjoint, for the strip above
+  uint8_t* dummy;  // 64 + K slots of kSlot bytes
+  uint8_t* hap;    // kHapPad zeros, the haplotype, zeros
 };
 
-constexpr int kLdsFixed = (kTape + 1) * 32 + kTape * 16 + kTape * 32 + kTape * 16 + (64 + kTape) * 32;
+template <typename T>
+constexpr int lds_fixed() {
+  constexpr int K = GT<T>::kTape;
+  return (K + 1) * int(sizeof(St4<T>)) + K * int(sizeof(Adj2<T>)) + K * int(sizeof(St4<T>)) +
+         K * int(sizeof(Adj2<T>)) + (64 + K) * kSlot;
+}
 
-__device__ __forceinline__ Lds lds_parts(uint8_t* base) {
-  Lds l;
-  l.stage = reinterpret_cast<St4*>(base);
-  base += (kTape + 1) * 32;
-  l.astage = reinterpret_cast<Adj2*>(base);
-  base += kTape * 16;
-  l.fout = reinterpret_cast<St4*>(base);
-  base += kTape * 32;
-  l.aout = reinterpret_cast<Adj2*>(base);
-  base += kTape * 16;
+template <typename T>
+__device__ __forceinline__ Lds<T> lds_parts(uint8_t* base) {
+  constexpr int K = GT<T>::kTape;
+  Lds<T> l;
+  l.stage = reinterpret_cast<St4<T>*>(base);
+  base += (K + 1) * sizeof(St4<T>);
+  l.astage = reinterpret_cast<Adj2<T>*>(base);
+  base += K * sizeof(Adj2<T>);
+  l.fout = reinterpret_cast<St4<T>*>(base);
+  base += K * sizeof(St4<T>);
+  l.aout = reinterpret_cast<Adj2<T>*>(base);
+  base += K * sizeof(Adj2<T>);
   l.dummy = base;
-  base += (64 + kTape) * 32;
+  base += (64 + K) * kSlot;
   l.hap = base;
   return l;
 }
 
-__device__ __forceinline__ double prior_of(const Lds& l, int c, const GRow& rw) {
+template <typename T>
+__device__ __forceinline__ T prior_of(const Lds<T>& l, int c, const GRow<T>& rw) {
   return is_match(l.hap[kHapPad + c - 1], rw.rb) ? rw.pm : rw.px;
 }
 
-// kTape forward steps from `st` (the state before step t0) with diagonal `P`.  FM freezes lanes
+// K forward steps from `st` (the state before step t0) with diagonal `P`.  FM freezes lanes
 // that have not reached column 1; ACC adds M + X of columns 1..m; OUT stages lane 63's state.
 // tp[k] receives the state before step k (TAPE).
-template <bool FM, bool ACC, bool OUT, bool TAPE>
-__device__ __forceinline__ void fblock(St& st, St& P, double& acc, St* tp, const GRow& rw,
-                                       const Lds& l, St4* out, int t0, int lane, int m) {
+template <typename T, bool FM, bool ACC, bool OUT, bool TAPE>
+__device__ __forceinline__ void fblock(St<T>& st, St<T>& P, T& acc, St<T>* tp, const GRow<T>& rw,
+                                       const Lds<T>& l, St4<T>* out, int t0, int lane, int m) {
+  constexpr int K = GT<T>::kTape;
 #pragma unroll
-  for (int k = 0; k < kTape; ++k) {
+  for (int k = 0; k < K; ++k) {
     if constexpr (TAPE) tp[k] = st;
     const int c = t0 + k - lane + 1;
-    const St Q = shr_st(l.stage[k + 1], st);
-    const double prior = prior_of(l, c, rw);
-    const double Y = fma(st.y, rw.g, st.m * rw.md);
-    const double M = prior * fma(P.y, rw.ig, fma(P.x, rw.ig, P.m * rw.mm));
-    const double X = fma(Q.x, rw.g, Q.m * rw.mi);
+    const St<T> Q = shr_st(l.stage[k + 1], st);
+    const T prior = prior_of(l, c, rw);
+    const T Y = fma(st.y, rw.g, st.m * rw.md);
+    const T M = prior * fma(P.y, rw.ig, fma(P.x, rw.ig, P.m * rw.mm));
+    const T X = fma(Q.x, rw.g, Q.m * rw.mi);
     if constexpr (FM) {
       const bool live = c >= 1;
       st.m = live ? M : st.m;
       st.x = live ? X : st.x;
       st.y = live ? Y : st.y;
     } else {
-      st = St{M, X, Y};
+      st = St<T>{M, X, Y};
     }
     P = Q;
     if constexpr (ACC) {
       if constexpr (FM)
-        acc += (c >= 1 && c <= m) ? st.m + st.x : 0.0;
+        acc += (c >= 1 && c <= m) ? st.m + st.x : T(0);
       else
-        acc += (c <= m) ? st.m + st.x : 0.0;
+        acc += (c <= m) ? st.m + st.x : T(0);
     }
-    if constexpr (OUT) out[k] = St4{st.m, st.x, st.y, 0.0};
+    if constexpr (OUT) out[k] = St4<T>{st.m, st.x, st.y, T(0)};
   }
-  if constexpr (TAPE) tp[kTape] = st;
+  if constexpr (TAPE) tp[K] = st;
 }
 
-// Stages columns t0 .. t0+kTape of a stored boundary row (clamped to 0..m) for lane 0; lanes
-// past kTape write their copy into the dummy area.
-__device__ __forceinline__ void stage_above(const Lds& l, const double* bnd, int t0, int m,
+// Stages columns t0 .. t0+K of a stored boundary row (clamped to 0..m) for lane 0; lanes past K
+// write their copy into the dummy area.
+template <typename T>
+__device__ __forceinline__ void stage_above(const Lds<T>& l, const T* bnd, int t0, int m,
                                             int lane) {
+  constexpr int K = GT<T>::kTape;
   wave_sync_lds();
-  const int j = min(lane, kTape);
+  const int j = min(lane, K);
   const int col = min(t0 + j, m);
-  const double4 w = *reinterpret_cast<const double4*>(bnd + size_t(col) * 4);
-  St4* dst = lane <= kTape ? &l.stage[j] : reinterpret_cast<St4*>(l.dummy + lane * 32);
-  *dst = St4{w.x, w.y, w.z, 0.0};
+  const St4<T> w = *reinterpret_cast<const St4<T>*>(bnd + size_t(col) * 4);
+  St4<T>* dst = lane <= K ? &l.stage[j] : reinterpret_cast<St4<T>*>(l.dummy + lane * kSlot);
+  *dst = St4<T>{w.m, w.x, w.y, T(0)};
   wave_sync_lds();
 }
 
 // Pass F over strip s: returns the lane's sum of M + X over columns 1..m when ACC (row n's lane
 // is the one that matters).
-template <bool CK, bool ACC>
-__device__ double forward_strip(const GradArgs& a, const Lds& l, const GRow& rw, int lane, int s,
-                                int v, int m, int L, int nb, bool has_next, double* wscr) {
-  const double* bnd_in = wscr + a.bnd_off + size_t(s) * a.bnd_cols * 4;
-  double* bnd_out = wscr + a.bnd_off + size_t(s + 1) * a.bnd_cols * 4;
-  double* ck = wscr + a.ck_off + size_t(s) * a.nb_max * 6 * 64;
-  St4* out = lane == 63 ? l.fout : reinterpret_cast<St4*>(l.dummy + lane * 32);
-  St st{0.0, 0.0, 0.0};  // column 0 below DP row 0
-  St P{0.0, 0.0, 0.0};
-  double acc = 0.0;
-  double* gdummy = wscr + a.dummy_off + lane * 4;
-  if (has_next) *reinterpret_cast<double4*>(bnd_out) = make_double4(0.0, 0.0, 0.0, 0.0);
+template <typename T, bool CK, bool ACC>
+__device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw, int lane, int s,
+                           int m, int L, int nb, bool has_next, char* wscr) {
+  constexpr int K = GT<T>::kTape;
+  const T* bnd_in = reinterpret_cast<const T*>(wscr + a.bnd_off) + size_t(s) * a.bnd_cols * 4;
+  T* bnd_out = reinterpret_cast<T*>(wscr + a.bnd_off) + size_t(s + 1) * a.bnd_cols * 4;
+  T* ck = reinterpret_cast<T*>(wscr + a.ck_off) + size_t(s) * a.nb_max * 6 * 64;
+  St4<T>* out = lane == 63 ? l.fout : reinterpret_cast<St4<T>*>(l.dummy + lane * kSlot);
+  St<T> st{T(0), T(0), T(0)};  // column 0 below DP row 0
+  St<T> P{T(0), T(0), T(0)};
+  T acc = T(0);
+  St4<T>* gdummy = reinterpret_cast<St4<T>*>(wscr + a.dummy_off + lane * kSlot);
+  if (has_next) *reinterpret_cast<St4<T>*>(bnd_out) = St4<T>{T(0), T(0), T(0), T(0)};
   for (int b = 0; b < nb; ++b) {
-    const int t0 = b * kTape;
+    const int t0 = b * K;
     stage_above(l, bnd_in, t0, m, lane);
     if (b == 0) P = shr_st(l.stage[0], st);  // the diagonal of step 0: column 0 of the row above
     if constexpr (CK) {
-      double* c = ck + size_t(b) * 6 * 64 + lane;
+      T* c = ck + size_t(b) * 6 * 64 + lane;
       c[0 * 64] = st.m;
       c[1 * 64] = st.x;
       c[2 * 64] = st.y;
@@ -228,150 +270,169 @@ __device__ double forward_strip(const GradArgs& a, const Lds& l, const GRow& rw,
     const bool fm = t0 < L - 1;
     if (has_next) {
       if (fm)
-        fblock<true, ACC, true, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, true, ACC, true, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
       else
-        fblock<false, ACC, true, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, false, ACC, true, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
       wave_sync_lds();
-      // Lanes 0..kTape-1 store lane 63's columns t0+lane-62 that lie in 1..m.
+      // Lanes 0..K-1 store lane 63's columns t0+lane-62 that lie in 1..m.
       const int c = t0 + lane - 62;
-      const bool ok = lane < kTape && c >= 1 && c <= m;
-      const St4 w = l.fout[min(lane, kTape - 1)];
-      double* dst = ok ? bnd_out + size_t(max(c, 0)) * 4 : gdummy;
-      *reinterpret_cast<double4*>(dst) = make_double4(w.m, w.x, w.y, 0.0);
+      const bool ok = lane < K && c >= 1 && c <= m;
+      const St4<T> w = l.fout[min(lane, K - 1)];
+      St4<T>* dst = ok ? reinterpret_cast<St4<T>*>(bnd_out + size_t(max(c, 0)) * 4) : gdummy;
+      *dst = St4<T>{w.m, w.x, w.y, T(0)};
     } else {
       if (fm)
-        fblock<true, ACC, false, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, true, ACC, false, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
       else
-        fblock<false, ACC, false, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, false, ACC, false, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
     }
   }
   return acc;
 }
 
+// Gradient terms of one block, summed in the working precision (added to the FP64 accumulators
+// once per block).
+template <typename T> struct GSum {
+  T ge, gmm, gim, gmi, gmd, gg;
+};
+
 // The reverse sweep of one block over the tape.  BM zeroes the adjoint outside columns 1..m.
-template <bool BM>
-__device__ __forceinline__ void rblock(const St* tp, const St& P0, const GRow& rw, const Lds& l,
-                                       Adj2* aout, int t0, int lane, int m, double seedv,
-                                       double& AX, double& AY, double& Q, double& Bq, double& ge,
-                                       double& gmm, double& gim, double& gmi, double& gmd,
-                                       double& gg) {
-  St U = shr_st(l.stage[kTape], tp[kTape - 1]);  // up values of the last step
+template <typename T, bool BM>
+__device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const GRow<T>& rw,
+                                       const Lds<T>& l, Adj2<T>* aout, int t0, int lane, int m,
+                                       T seedv, T& AX, T& AY, T& Q, T& Bq, GSum<T>& gs) {
+  constexpr int K = GT<T>::kTape;
+  St<T> U = shr_st(l.stage[K], tp[K - 1]);  // up values of the last step
 #pragma unroll
-  for (int k = kTape - 1; k >= 0; --k) {
+  for (int k = K - 1; k >= 0; --k) {
     const int col = t0 + k - lane + 1;
     const bool mt = is_match(l.hap[kHapPad + col - 1], rw.rb);
-    const double prior = mt ? rw.pm : rw.px;
-    const double ds = mt ? -1.0 : (1.0 / 3.0);  // d prior / d e
+    const T prior = mt ? rw.pm : rw.px;
+    const T ds = mt ? T(-1) : T(1.0 / 3.0);  // d prior / d e
     // The row below: its insertion adjoint in this column and Q one column to the right.
-    const Adj2 below = l.astage[k];
-    const double AXb = shl1(below.ax, AX);
-    const double Qn = shl1(below.q, Q);
-    const double Qd = Bq;
+    const Adj2<T> below = l.astage[k];
+    const T AXb = shl1(below.ax, AX);
+    const T Qn = shl1(below.q, Q);
+    const T Qd = Bq;
     Bq = Qn;
-    const St D = k ? shr_st(l.stage[k], tp[k - 1]) : P0;
-    const St& Lf = tp[k];
-    const double pr = Qd * rw.ig;
-    double ay = fma(AY, rw.g, pr);
-    const double amp = fma(Qd, rw.mmB, AY * rw.md);
-    double ax = fma(AXb, rw.g, pr) + seedv;
-    double am = fma(AXb, rw.miB, amp) + seedv;
+    const St<T> D = k ? shr_st(l.stage[k], tp[k - 1]) : P0;
+    const St<T>& Lf = tp[k];
+    const T pr = Qd * rw.ig;
+    T ay = fma(AY, rw.g, pr);
+    const T amp = fma(Qd, rw.mmB, AY * rw.md);
+    T ax = fma(AXb, rw.g, pr) + seedv;
+    T am = fma(AXb, rw.miB, amp) + seedv;
     if constexpr (BM) {
       const bool live = col >= 1 && col <= m;
-      ay = live ? ay : 0.0;
-      ax = live ? ax : 0.0;
-      am = live ? am : 0.0;
+      ay = live ? ay : T(0);
+      ax = live ? ax : T(0);
+      am = live ? am : T(0);
     }
-    const double q = am * prior;
-    const double xy = D.x + D.y;
-    const double spre = fma(D.m, rw.mm, xy * rw.ig);  // M before the prior
-    ge = fma(am * spre, ds, ge);
-    gmm = fma(q, D.m, gmm);
-    gim = fma(q, xy, gim);
-    gmi = fma(ax, U.m, gmi);
-    gmd = fma(ay, Lf.m, gmd);
-    gg = fma(ax, U.x, fma(ay, Lf.y, gg));
+    const T q = am * prior;
+    const T xy = D.x + D.y;
+    const T spre = fma(D.m, rw.mm, xy * rw.ig);  // M before the prior
+    gs.ge = fma(am * spre, ds, gs.ge);
+    gs.gmm = fma(q, D.m, gs.gmm);
+    gs.gim = fma(q, xy, gs.gim);
+    gs.gmi = fma(ax, U.m, gs.gmi);
+    gs.gmd = fma(ay, Lf.m, gs.gmd);
+    gs.gg = fma(ax, U.x, fma(ay, Lf.y, gs.gg));
     AY = ay;
     AX = ax;
     Q = q;
     U = D;
-    aout[k] = Adj2{ax, q};
+    aout[k] = Adj2<T>{ax, q};
   }
 }
 
 // Pass B over strip s.  `dgl` collects the lane's gap-continuation terms.
-__device__ void backward_strip(const GradArgs& a, const Lds& l, const GRow& rw, int lane, int s,
-                               int v, int n, int m, int L, int nb, double seedv, int64_t o,
-                               double* wscr, double& dgl) {
-  const double* bnd_in = wscr + a.bnd_off + size_t(s) * a.bnd_cols * 4;
-  const double* ck = wscr + a.ck_off + size_t(s) * a.nb_max * 6 * 64;
+template <typename T>
+__device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw, int lane,
+                               int s, int v, int n, int m, int L, int nb, T seedv, int64_t o,
+                               char* wscr, double& dgl) {
+  constexpr int K = GT<T>::kTape;
+  const T* bnd_in = reinterpret_cast<const T*>(wscr + a.bnd_off) + size_t(s) * a.bnd_cols * 4;
+  const T* ck = reinterpret_cast<const T*>(wscr + a.ck_off) + size_t(s) * a.nb_max * 6 * 64;
   const int abnd_cols = a.bnd_cols + 1;
-  const Adj2* ab_in = reinterpret_cast<const Adj2*>(wscr + a.abnd_off) + size_t((s + 1) & 1) * abnd_cols;
-  Adj2* ab_out = reinterpret_cast<Adj2*>(wscr + a.abnd_off) + size_t(s & 1) * abnd_cols;
-  Adj2* aout = lane == 0 ? l.aout : reinterpret_cast<Adj2*>(l.dummy + lane * 32);
-  double* gdummy = wscr + a.dummy_off + lane * 4;
-  ab_out[0] = Adj2{0.0, 0.0};
-  ab_out[m + 1] = Adj2{0.0, 0.0};
-  double AX = 0.0, AY = 0.0, Q = 0.0, Bq = 0.0;  // columns beyond m: zero adjoints
+  Adj2<T>* abnd = reinterpret_cast<Adj2<T>*>(wscr + a.abnd_off);
+  const Adj2<T>* ab_in = abnd + size_t((s + 1) & 1) * abnd_cols;
+  Adj2<T>* ab_out = abnd + size_t(s & 1) * abnd_cols;
+  Adj2<T>* aout = lane == 0 ? l.aout : reinterpret_cast<Adj2<T>*>(l.dummy + lane * kSlot);
+  char* gdummy = wscr + a.dummy_off + lane * kSlot;
+  ab_out[0] = Adj2<T>{T(0), T(0)};
+  ab_out[m + 1] = Adj2<T>{T(0), T(0)};
+  T AX = T(0), AY = T(0), Q = T(0), Bq = T(0);  // columns beyond m: zero adjoints
   double ge = 0.0, gmm = 0.0, gim = 0.0, gmi = 0.0, gmd = 0.0, gg = 0.0;
   for (int b = nb - 1; b >= 0; --b) {
-    const int t0 = b * kTape;
+    const int t0 = b * K;
     {
-      const Adj2 w = ab_in[min(max(t0 + lane - 62, 0), m + 1)];
-      Adj2* dst = lane < kTape ? &l.astage[lane] : reinterpret_cast<Adj2*>(l.dummy + lane * 32);
+      const Adj2<T> w = ab_in[min(max(t0 + lane - 62, 0), m + 1)];
+      Adj2<T>* dst = lane < K ? &l.astage[lane] : reinterpret_cast<Adj2<T>*>(l.dummy + lane * kSlot);
       wave_sync_lds();
       *dst = w;
     }
     stage_above(l, bnd_in, t0, m, lane);
     // Recompute the block from its checkpoint into the tape: tp[k] is the state before step k.
-    const double* c = ck + size_t(b) * 6 * 64 + lane;
-    St tp[kTape + 1];
-    St st{c[0 * 64], c[1 * 64], c[2 * 64]};
-    const St P0{c[3 * 64], c[4 * 64], c[5 * 64]};
-    St P = P0;
-    double unused = 0.0;
+    const T* c = ck + size_t(b) * 6 * 64 + lane;
+    St<T> tp[K + 1];
+    St<T> st{c[0 * 64], c[1 * 64], c[2 * 64]};
+    const St<T> P0{c[3 * 64], c[4 * 64], c[5 * 64]};
+    St<T> P = P0;
+    T unused = T(0);
     if (t0 < L - 1)
-      fblock<true, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
+      fblock<T, true, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
     else
-      fblock<false, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
-    if (t0 < L - 1 || t0 + kTape > m)
-      rblock<true>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, ge, gmm, gim, gmi, gmd, gg);
+      fblock<T, false, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
+    GSum<T> gs{T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (t0 < L - 1 || t0 + K > m)
+      rblock<T, true>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
     else
-      rblock<false>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, ge, gmm, gim, gmi, gmd, gg);
+      rblock<T, false>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
+    ge += double(gs.ge);
+    gmm += double(gs.gmm);
+    gim += double(gs.gim);
+    gmi += double(gs.gmi);
+    gmd += double(gs.gmd);
+    gg += double(gs.gg);
     wave_sync_lds();
     {
       const int col = t0 + lane + 1;  // lane 0's column at step t0 + lane
-      const bool ok = lane < kTape && col <= m;
-      const Adj2 w = l.aout[min(lane, kTape - 1)];
-      Adj2* dst = ok ? &ab_out[col] : reinterpret_cast<Adj2*>(gdummy);
+      const bool ok = lane < K && col <= m;
+      const Adj2<T> w = l.aout[min(lane, K - 1)];
+      Adj2<T>* dst = ok ? &ab_out[col] : reinterpret_cast<Adj2<T>*>(gdummy);
       *dst = w;
     }
   }
   const bool real = v <= n;
   const int64_t i = o + (real ? v : n) - 1;
-  double* pe = real ? a.g_err + i : gdummy;
-  double* pi = real ? a.g_ins + i : gdummy + 1;
-  double* pd = real ? a.g_del + i : gdummy + 2;
+  double* pe = real ? a.g_err + i : reinterpret_cast<double*>(gdummy);
+  double* pi = real ? a.g_ins + i : reinterpret_cast<double*>(gdummy) + 1;
+  double* pd = real ? a.g_del + i : reinterpret_cast<double*>(gdummy) + 2;
   *pe += ge * kInvLn10;
   *pi += (gmi - gmm) * kInvLn10;  // M->M = 1 - d_ins - d_del
   *pd += (gmd - gmm) * kInvLn10;
   dgl += real ? gg - gim : 0.0;   // I->M = D->M = 1 - g
 }
 
-template <bool BWD>
+template <typename T, bool BWD>
 __global__ void __launch_bounds__(256) grad_kernel(GradArgs a) {
+  constexpr int K = GT<T>::kTape;
+  constexpr bool kFp32 = std::is_same<T, float>::value;
   extern __shared__ __attribute__((aligned(32))) uint8_t glds[];
   const int lane = threadIdx.x & 63;
   const int wave_in_block = threadIdx.x >> 6;
   const int64_t wave_global = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave_in_block;
-  const Lds l = lds_parts(glds + wave_in_block * a.lds_bytes);
-  double* wscr = a.scratch + wave_global * a.wave_stride;
+  const Lds<T> l = lds_parts<T>(glds + wave_in_block * a.lds_bytes);
+  char* wscr = a.scratch + wave_global * a.wave_stride;
+  char* gslot = wscr + a.dummy_off + lane * kSlot;
   const double g = a.gcp[0];
+  const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
 
   for (;;) {
     // Every lane adds one, so a claim advances the counter by 64 and every lane's old value lies
     // in [64 i, 64 i + 63]; no lane-dependent branch reaches the readfirstlane.
     const int item = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, 1)) >> 6;
-    if (item >= a.n_items) break;
+    if (item >= n_items) break;
     const int r = __builtin_amdgcn_readfirstlane(a.items[item]);
     const int64_t o = a.read_off[r];
     const int n = __builtin_amdgcn_readfirstlane(int(a.read_off[r + 1] - o));
@@ -380,23 +441,24 @@ __global__ void __launch_bounds__(256) grad_kernel(GradArgs a) {
     const int H = __builtin_amdgcn_readfirstlane(a.region_nhap[gi]);
     const int64_t out_base = a.region_out[gi] + int64_t(a.read_local[r]) * H;
     const int S = (n + 63) / 64;  // strips of 64 read rows
-    if constexpr (BWD) {
-      double* gdummy = wscr + a.dummy_off + lane * 4;
+    if (BWD && !a.rescue) {  // a rescue adds to what the FP32 pass wrote
       for (int s = 0; s < S; ++s) {
         const int v = 64 * s + lane + 1;
         const bool real = v <= n;
         const int64_t i = o + (real ? v : n) - 1;
-        *(real ? a.g_err + i : gdummy) = 0.0;
-        *(real ? a.g_ins + i : gdummy + 1) = 0.0;
-        *(real ? a.g_del + i : gdummy + 2) = 0.0;
+        *(real ? a.g_err + i : reinterpret_cast<double*>(gslot)) = 0.0;
+        *(real ? a.g_ins + i : reinterpret_cast<double*>(gslot) + 1) = 0.0;
+        *(real ? a.g_del + i : reinterpret_cast<double*>(gslot) + 2) = 0.0;
       }
     }
     double dgl = 0.0;
+    int any_lost = 0;
     for (int h = 0; h < H; ++h) {
+      if (a.rescue && !__builtin_amdgcn_readfirstlane(int(a.lost[out_base + h]))) continue;
       const int64_t hoff = a.hap_off[hap0 + h];
       const int m = __builtin_amdgcn_readfirstlane(int(a.hap_off[hap0 + h + 1] - hoff));
       wave_sync_lds();
-      const int hap_total = kHapPad + m + 64 + kTape + kHapPad;
+      const int hap_total = kHapPad + m + 64 + K + kHapPad;
       for (int i0 = 0; i0 < hap_total; i0 += 64) {
         const int i = i0 + lane;
         const int c = i - kHapPad;
@@ -404,13 +466,15 @@ __global__ void __launch_bounds__(256) grad_kernel(GradArgs a) {
         const uint8_t hb = a.hap_bases[hoff + (in ? c : 0)];
         *(i < hap_total ? &l.hap[i] : l.dummy + lane) = in ? hb : uint8_t(0);
       }
-      // The row above strip 0 is DP row 0: no match or insertion mass, deletions 2^1020 / m in
-      // every column (reference.rs:27-28).
-      const double init = kInitial / double(m);
+      // The row above strip 0 is DP row 0: no match or insertion mass, deletions C / m in every
+      // column (reference.rs:27-28).
+      const T init = T(GT<T>::kInit / double(m));
+      T* bnd0 = reinterpret_cast<T*>(wscr + a.bnd_off);
       for (int c0 = 0; c0 <= m; c0 += 64) {
         const int c = c0 + lane;
-        double* dst = c <= m ? wscr + a.bnd_off + size_t(c) * 4 : wscr + a.dummy_off + lane * 4;
-        *reinterpret_cast<double4*>(dst) = make_double4(0.0, 0.0, init, 0.0);
+        St4<T>* dst = c <= m ? reinterpret_cast<St4<T>*>(bnd0 + size_t(c) * 4)
+                             : reinterpret_cast<St4<T>*>(gslot);
+        *dst = St4<T>{T(0), T(0), init, T(0)};
       }
       wave_sync_lds();
       wave_sync_global();
@@ -421,45 +485,53 @@ __global__ void __launch_bounds__(256) grad_kernel(GradArgs a) {
       for (int s = 0; s < S; ++s) {
         const int v = 64 * s + lane + 1;
         const int L = min(64, n - 64 * s);
-        const int nb = (m + L - 1 + kTape - 1) / kTape;
-        GRow rw;
+        const int nb = (m + L - 1 + K - 1) / K;
+        GRow<T> rw;
         load_grow(a, o, n, v, g, rw);
         if (s + 1 < S) {
           if (grad)
-            forward_strip<true, false>(a, l, rw, lane, s, v, m, L, nb, true, wscr);
+            forward_strip<T, true, false>(a, l, rw, lane, s, m, L, nb, true, wscr);
           else
-            forward_strip<false, false>(a, l, rw, lane, s, v, m, L, nb, true, wscr);
+            forward_strip<T, false, false>(a, l, rw, lane, s, m, L, nb, true, wscr);
         } else {
-          const double acc =
-              grad ? forward_strip<true, true>(a, l, rw, lane, s, v, m, L, nb, false, wscr)
-                   : forward_strip<false, true>(a, l, rw, lane, s, v, m, L, nb, false, wscr);
-          total = readlane(acc, n - 1 - 64 * s);
+          const T acc = grad ? forward_strip<T, true, true>(a, l, rw, lane, s, m, L, nb, false, wscr)
+                             : forward_strip<T, false, true>(a, l, rw, lane, s, m, L, nb, false, wscr);
+          total = double(readlane(acc, n - 1 - 64 * s));
         }
         wave_sync_global();
       }
-      if (a.log10_out) a.log10_out[out_base + h] = log10(total) - kLog10Initial;  // every lane
-      if (!grad || !(total > 0.0) || !(total <= 1.7976931348623157e308)) continue;
+      // FP32: a pair below the rescue threshold (or NaN) is left to the FP64 pass.
+      const bool lost = kFp32 && !(total >= GT<T>::kMinAccepted);
+      if (kFp32 && lane == 0) a.lost[out_base + h] = lost ? 1 : 0;
+      any_lost |= lost;
+      if (a.log10_out && lane == 0)
+        a.log10_out[out_base + h] = lost ? __builtin_nan("") : log10(total) - GT<T>::kLog10Init;
+      if (lost || !grad || !(total > 0.0) || !(total <= 1.7976931348623157e308)) continue;
       // Pass B: the bottom strip has nothing below it.
       {
         const int abnd_cols = a.bnd_cols + 1;
-        Adj2* z = reinterpret_cast<Adj2*>(wscr + a.abnd_off) + size_t(S & 1) * abnd_cols;
+        Adj2<T>* z = reinterpret_cast<Adj2<T>*>(wscr + a.abnd_off) + size_t(S & 1) * abnd_cols;
         for (int c0 = 0; c0 <= m + 1; c0 += 64) {
           const int c = c0 + lane;
-          *(c <= m + 1 ? &z[c] : reinterpret_cast<Adj2*>(wscr + a.dummy_off + lane * 4)) =
-              Adj2{0.0, 0.0};
+          *(c <= m + 1 ? &z[c] : reinterpret_cast<Adj2<T>*>(gslot)) = Adj2<T>{T(0), T(0)};
         }
         wave_sync_global();
       }
-      const double seed = w / total;  // every adjoint is w d ln L / d cell
+      // Every adjoint is w d ln L / d cell: seeded with w / L at the last row.
+      const T seed = T(w / total);
       for (int s = S - 1; s >= 0; --s) {
         const int v = 64 * s + lane + 1;
         const int L = min(64, n - 64 * s);
-        const int nb = (m + L - 1 + kTape - 1) / kTape;
-        GRow rw;
+        const int nb = (m + L - 1 + K - 1) / K;
+        GRow<T> rw;
         load_grow(a, o, n, v, g, rw);
-        backward_strip(a, l, rw, lane, s, v, n, m, L, nb, v == n ? seed : 0.0, o, wscr, dgl);
+        backward_strip<T>(a, l, rw, lane, s, v, n, m, L, nb, v == n ? seed : T(0), o, wscr, dgl);
         wave_sync_global();
       }
+    }
+    if (kFp32 && any_lost && lane == 0) {
+      const int idx = atomicAdd(a.lost_count, 1);
+      a.lost_items[idx] = r;
     }
     if constexpr (BWD) {
       // Sum over lanes in lane order through LDS; every lane stores the same value.
@@ -470,7 +542,10 @@ __global__ void __launch_bounds__(256) grad_kernel(GradArgs a) {
       double tot = 0.0;
 #pragma unroll 4
       for (int i = 0; i < 64; ++i) tot += red[i];
-      a.dg_part[r] = tot * kInvLn10;
+      if (a.rescue)
+        a.dg_part[r] += tot * kInvLn10;
+      else
+        a.dg_part[r] = tot * kInvLn10;
     }
   }
 }
@@ -526,12 +601,16 @@ struct GradBatch::Impl {
   int64_t* hap_off = nullptr;
   uint8_t* hap_bases = nullptr;
   int32_t* items = nullptr;
+  // Work-queue heads and counts: 0 forward, 1 backward, 2 reads with lost pairs, 3 the rescue.
   int32_t* counters = nullptr;
   double* dg_part = nullptr;
-  double* scratch = nullptr;
+  uint8_t* lost = nullptr;        // per pair (FP32 forward)
+  int32_t* lost_items = nullptr;  // reads with lost pairs
+  char* scratch = nullptr;
   int64_t scratch_waves = 0;
-  GradArgs proto{};
-  int occ[2] = {0, 0};
+  GradArgs proto[2] = {};         // [fp64]: precision-specific scratch and LDS layout
+  int occ[2][2] = {};             // resident blocks per CU [fp64][bwd]
+  int fp32 = 0;                   // precision of the forward, the tape and the adjoint
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = {};
   double *d_err = nullptr, *d_ins = nullptr, *d_del = nullptr, *d_gcp = nullptr;
@@ -582,28 +661,73 @@ struct GradBatch::Impl {
     if (count) GRAD_TRY(hipMemcpy(*p, src, sizeof(P) * count, hipMemcpyHostToDevice));
     return kOk;
   }
-  int launch(bool bwd, GradArgs a, hipStream_t stream) {
-    hipLaunchKernelGGL(grad_fill, dim3(1), dim3(64), 0, stream, counters, 2, 0);
+  template <typename T>
+  int launch_one(bool bwd, GradArgs a, int64_t max_items) {
+    if (max_items <= 0) return kOk;
+    const int ti = std::is_same<T, double>::value ? 1 : 0;
+    const int64_t want = (max_items + kWaves - 1) / kWaves;
+    const int64_t resident = int64_t(occ[ti][bwd ? 1 : 0]) * num_cus;
+    const int grid = int(std::min({want, resident, scratch_waves / kWaves}));
+    if (bwd)
+      hipLaunchKernelGGL((grad_kernel<T, true>), dim3(grid), dim3(kBlock),
+                         size_t(a.lds_bytes) * kWaves, stream, a);
+    else
+      hipLaunchKernelGGL((grad_kernel<T, false>), dim3(grid), dim3(kBlock),
+                         size_t(a.lds_bytes) * kWaves, stream, a);
     GRAD_TRY(hipGetLastError());
-    if (a.n_items > 0) {
-      const int64_t want = (int64_t(a.n_items) + kWaves - 1) / kWaves;
-      const int64_t resident = int64_t(occ[bwd ? 1 : 0]) * num_cus;
-      const int grid = int(std::min({want, resident, scratch_waves / kWaves}));
-      a.counter = counters + (bwd ? 1 : 0);
-      if (bwd)
-        hipLaunchKernelGGL(grad_kernel<true>, dim3(grid), dim3(kBlock), size_t(a.lds_bytes) * kWaves,
-                           stream, a);
-      else
-        hipLaunchKernelGGL(grad_kernel<false>, dim3(grid), dim3(kBlock),
-                           size_t(a.lds_bytes) * kWaves, stream, a);
-      GRAD_TRY(hipGetLastError());
-    }
     return kOk;
+  }
+  // FP64: one pass over every read.  FP32: one pass, then the FP64 pass over the reads with
+  // pairs below the threshold, which reruns only those pairs and adds their terms.
+  int launch(bool bwd, const GradArgs& base) {
+    hipLaunchKernelGGL(grad_fill, dim3(1), dim3(64), 0, stream, counters, 4, 0);
+    GRAD_TRY(hipGetLastError());
+    if (base.n_items <= 0) return kOk;
+    if (!fp32) {
+      GradArgs a = base;
+      a.counter = counters + (bwd ? 1 : 0);
+      return launch_one<double>(bwd, a, a.n_items);
+    }
+    GradArgs a = base;
+    const GradArgs& lf = proto[0];
+    a.ck_off = lf.ck_off; a.bnd_off = lf.bnd_off; a.abnd_off = lf.abnd_off;
+    a.dummy_off = lf.dummy_off; a.nb_max = lf.nb_max; a.lds_bytes = lf.lds_bytes;
+    a.counter = counters + (bwd ? 1 : 0);
+    a.lost = lost;
+    a.lost_items = lost_items;
+    a.lost_count = counters + 2;
+    if (int rc = launch_one<float>(bwd, a, a.n_items)) return rc;
+    GradArgs r = base;  // the FP64 layout
+    r.items = lost_items;
+    r.n_items = 0;
+    r.n_items_dev = counters + 2;
+    r.counter = counters + 3;
+    r.lost = lost;
+    r.rescue = 1;
+    return launch_one<double>(bwd, r, n_reads);
   }
 };
 
 GradBatch::GradBatch() : impl_(new Impl) {}
 GradBatch::~GradBatch() { delete impl_; }
+
+namespace {
+// Per-wave scratch of one precision, in bytes: checkpoints, forward boundary rows of every strip
+// (row 0 and each strip's last row), the two adjoint boundary rows, the dummy slots.
+template <typename T>
+void layout(GradArgs& a, int S_max, int max_m) {
+  constexpr int K = GT<T>::kTape;
+  a.nb_max = (max_m + 63 + K - 1) / K;
+  a.bnd_cols = max_m + 1;
+  a.lds_bytes = (lds_fixed<T>() + 2 * kHapPad + max_m + 64 + K + 31) & ~31;
+  const int64_t sz = int64_t(sizeof(T));
+  a.ck_off = 0;
+  a.bnd_off = (int64_t(S_max) * a.nb_max * 6 * 64 * sz + 255) & ~int64_t(255);
+  a.abnd_off = (a.bnd_off + int64_t(S_max + 1) * a.bnd_cols * 4 * sz + 255) & ~int64_t(255);
+  a.dummy_off = (a.abnd_off + 2 * int64_t(a.bnd_cols + 1) * 2 * sz + 255) & ~int64_t(255);
+  a.wave_stride = (a.dummy_off + 64 * kSlot + 255) & ~int64_t(255);
+}
+}  // namespace
 
 int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
   if (in.n_regions < 0 || (in.n_regions > 0 && (!in.region_read || !in.region_hap))) {
@@ -700,7 +824,8 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
       (rc = d.alloc(&d.d_del, nrb)) || (rc = d.alloc(&d.d_gcp, 1)) ||
       (rc = d.alloc(&d.d_w, npair)) || (rc = d.alloc(&d.d_log10, npair)) ||
       (rc = d.alloc(&d.d_ge, nrb)) || (rc = d.alloc(&d.d_gi, nrb)) ||
-      (rc = d.alloc(&d.d_gd, nrb)) || (rc = d.alloc(&d.d_gg, 1)))
+      (rc = d.alloc(&d.d_gd, nrb)) || (rc = d.alloc(&d.d_gg, 1)) ||
+      (rc = d.alloc(&d.lost, npair)) || (rc = d.alloc(&d.lost_items, nr)))
     return rc;
   if ((rc = d.upload(&d.bases, in.bases, size_t(d.n_read_bases))) ||
       (rc = d.upload(&d.read_off, in.read_off, size_t(NR + 1))) ||
@@ -712,57 +837,68 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
       (rc = d.upload(&d.hap_off, in.hap_off, size_t(NH + 1))) ||
       (rc = d.upload(&d.hap_bases, in.hap_bases, size_t(nb))) ||
       (rc = d.upload(&d.items, items.data(), size_t(NR))) ||
-      (rc = d.alloc(&d.counters, 2)) || (rc = d.alloc(&d.dg_part, size_t(NR))))
+      (rc = d.alloc(&d.counters, 4)) || (rc = d.alloc(&d.dg_part, size_t(NR))))
     return rc;
-  // Per-wave LDS and scratch.
-  const int lds = (kLdsFixed + 2 * kHapPad + max_m + 64 + kTape + 31) & ~31;
-  if (lds * kWaves > 64 * 1024) {
-    set_error(kInvalidArgument, "haplotypes are too long for the gradient engine's LDS staging");
+  // Per-wave LDS and scratch, per precision; both passes share the larger per-wave stride.
+  const int S_max = (max_n + 64) / 64;
+  layout<float>(d.proto[0], S_max, max_m);
+  layout<double>(d.proto[1], S_max, max_m);
+  for (const GradArgs& a : d.proto)
+    if (a.lds_bytes * kWaves > 64 * 1024) {
+      set_error(kInvalidArgument, "haplotypes are too long for the gradient engine's LDS staging");
+      return kInvalidArgument;
+    }
+  const int64_t stride = std::max(d.proto[0].wave_stride, d.proto[1].wave_stride);
+  for (int t = 0; t < 2; ++t)
+    for (int k = 0; k < 2; ++k) {
+      int nbk = 0;
+      const void* fn = t ? (k ? reinterpret_cast<const void*>(grad_kernel<double, true>)
+                              : reinterpret_cast<const void*>(grad_kernel<double, false>))
+                         : (k ? reinterpret_cast<const void*>(grad_kernel<float, true>)
+                              : reinterpret_cast<const void*>(grad_kernel<float, false>));
+      GRAD_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, kBlock,
+                                                           size_t(d.proto[t].lds_bytes) * kWaves));
+      d.occ[t][k] = std::max(nbk, 1);
+    }
+  int occ_max = 0;
+  for (auto& row : d.occ)
+    for (int o : row) occ_max = std::max(occ_max, o);
+  const int64_t want_waves = std::max<int64_t>((NR + kWaves - 1) / kWaves * kWaves, kWaves);
+  d.scratch_waves = std::min<int64_t>(int64_t(occ_max) * d.num_cus * kWaves, want_waves);
+  if ((rc = d.alloc(&d.scratch, size_t(d.scratch_waves * stride)))) return rc;
+  for (GradArgs& a : d.proto) {
+    a.wave_stride = stride;
+    a.bases = d.bases;
+    a.read_off = d.read_off;
+    a.read_region = d.read_region;
+    a.read_local = d.read_local;
+    a.region_out = d.region_out;
+    a.region_hap0 = d.region_hap0;
+    a.region_nhap = d.region_nhap;
+    a.hap_off = d.hap_off;
+    a.hap_bases = d.hap_bases;
+    a.items = d.items;
+    a.n_items = int32_t(NR);
+    a.dg_part = d.dg_part;
+    a.scratch = d.scratch;
+    a.err = d.d_err;
+    a.ins = d.d_ins;
+    a.del = d.d_del;
+    a.gcp = d.d_gcp;
+    a.g_err = d.d_ge;
+    a.g_ins = d.d_gi;
+    a.g_del = d.d_gd;
+  }
+  out = std::move(b);
+  return kOk;
+}
+
+int GradBatch::set_precision(int fp32) {
+  if (fp32 != 0 && fp32 != 1) {
+    set_error(kInvalidArgument, "precision must be 0 (FP64) or 1 (FP32 with FP64 rescue)");
     return kInvalidArgument;
   }
-  const int S_max = (max_n + 64) / 64;
-  const int nb_max = (max_m + 63 + kTape - 1) / kTape;
-  const int bnd_cols = max_m + 1;
-  GradArgs& a = d.proto;
-  a.nb_max = nb_max;
-  a.bnd_cols = bnd_cols;
-  a.lds_bytes = lds;
-  a.ck_off = 0;
-  a.bnd_off = int64_t(S_max) * nb_max * 6 * 64;
-  a.abnd_off = a.bnd_off + int64_t(S_max + 1) * bnd_cols * 4;
-  a.dummy_off = (a.abnd_off + 2 * int64_t(bnd_cols + 1) * 2 + 3) & ~int64_t(3);
-  a.wave_stride = (a.dummy_off + 64 * 4 + 63) & ~int64_t(63);
-  for (int k = 0; k < 2; ++k) {
-    int nbk = 0;
-    GRAD_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nbk, k ? grad_kernel<true> : grad_kernel<false>, kBlock, size_t(lds) * kWaves));
-    d.occ[k] = std::max(nbk, 1);
-  }
-  const int64_t want_waves = std::max<int64_t>((NR + kWaves - 1) / kWaves * kWaves, kWaves);
-  d.scratch_waves = std::min<int64_t>(int64_t(std::max(d.occ[0], d.occ[1])) * d.num_cus * kWaves,
-                                      want_waves);
-  if ((rc = d.alloc(&d.scratch, size_t(d.scratch_waves * a.wave_stride)))) return rc;
-  a.bases = d.bases;
-  a.read_off = d.read_off;
-  a.read_region = d.read_region;
-  a.read_local = d.read_local;
-  a.region_out = d.region_out;
-  a.region_hap0 = d.region_hap0;
-  a.region_nhap = d.region_nhap;
-  a.hap_off = d.hap_off;
-  a.hap_bases = d.hap_bases;
-  a.items = d.items;
-  a.n_items = int32_t(NR);
-  a.dg_part = d.dg_part;
-  a.scratch = d.scratch;
-  a.err = d.d_err;
-  a.ins = d.d_ins;
-  a.del = d.d_del;
-  a.gcp = d.d_gcp;
-  a.g_err = d.d_ge;
-  a.g_ins = d.d_gi;
-  a.g_del = d.d_gd;
-  out = std::move(b);
+  impl_->fp32 = fp32;
   return kOk;
 }
 
@@ -775,9 +911,9 @@ int GradBatch::forward(const GradParams& p, double* log10_out, double* kernel_ms
   if (int rc = d.put_params(p)) return rc;
   GRAD_TRY(hipEventRecord(d.ev[0], d.stream));
   if (d.n_pairs) {
-    GradArgs a = d.proto;
+    GradArgs a = d.proto[1];
     a.log10_out = d.d_log10;
-    if (int rc = d.launch(false, a, d.stream)) return rc;
+    if (int rc = d.launch(false, a)) return rc;
   }
   GRAD_TRY(hipEventRecord(d.ev[1], d.stream));
   if (d.n_pairs)
@@ -795,7 +931,7 @@ int GradBatch::backward(const GradParams& p, const double* weights, double* log1
     return kInvalidArgument;
   }
   if (int rc = d.put_params(p)) return rc;
-  GradArgs a = d.proto;
+  GradArgs a = d.proto[1];
   if (weights && d.n_pairs) {
     GRAD_TRY(hipMemcpyAsync(d.d_w, weights, sizeof(double) * size_t(d.n_pairs),
                             hipMemcpyHostToDevice, d.stream));
@@ -804,7 +940,7 @@ int GradBatch::backward(const GradParams& p, const double* weights, double* log1
   a.log10_out = d.d_log10;
   GRAD_TRY(hipEventRecord(d.ev[0], d.stream));
   if (d.n_reads) {
-    if (int rc = d.launch(true, a, d.stream)) return rc;
+    if (int rc = d.launch(true, a)) return rc;
   }
   hipLaunchKernelGGL(grad_reduce, dim3(1), dim3(kBlock), 0, d.stream, d.dg_part, d.n_reads,
                      d.d_gg);
@@ -826,7 +962,7 @@ int GradBatch::backward(const GradParams& p, const double* weights, double* log1
 int64_t GradBatch::n_pairs() const { return impl_->n_pairs; }
 int64_t GradBatch::n_read_bases() const { return impl_->n_read_bases; }
 int64_t GradBatch::scratch_bytes() const {
-  return impl_->scratch_waves * impl_->proto.wave_stride * int64_t(sizeof(double));
+  return impl_->scratch_waves * impl_->proto[1].wave_stride;
 }
 
 int dpp_shl_selftest(int32_t* out64) {

@@ -532,6 +532,11 @@ int fgkl_grad_create(fgkl_grad** out, int64_t n_regions, const int64_t* region_r
   }
 }
 
+int fgkl_grad_set_precision(fgkl_grad* g, int fp32) {
+  if (!g) return fail(kInvalidArgument, "null gradient handle");
+  return g->grad->set_precision(fp32);
+}
+
 int64_t fgkl_grad_pairs(const fgkl_grad* g) { return g ? g->grad->n_pairs() : 0; }
 
 int64_t fgkl_grad_read_bases(const fgkl_grad* g) { return g ? g->grad->n_read_bases() : 0; }

@@ -28,6 +28,10 @@ class GradBatch {
  public:
   ~GradBatch();
   static int create(const GradInput& in, std::unique_ptr<GradBatch>& out);
+  // 0: FP64 forward, tape and adjoint for every pair (the default).  1: FP32 forward, tape and
+  // adjoint with FP64 accumulation of the gradient terms; pairs whose FP32 likelihood falls
+  // below 1e-28 are rerun in FP64.
+  int set_precision(int fp32);
   // log10 L of every pair, region after region, [r * H + h] inside each.  kernel_ms (may be
   // null) receives the device time of the kernels.
   int forward(const GradParams& p, double* log10_out, double* kernel_ms);

@@ -5,8 +5,10 @@ Phred-derived probabilities made continuous (crates/pairhmm/src/model.rs:66-84):
 an error probability e (priors 1 - e / e/3), gap-open probabilities d_ins and d_del
 (M->M = 1 - d_ins - d_del), and one global gap-continuation probability g.  The reference has no
 gradient; ``GradProblem.backward`` runs the HIP adjoint kernel (reverse anti-diagonal sweep over
-checkpointed forward blocks, FP64 throughout) and ``log10_likelihoods`` wraps it as a
-``torch.autograd.Function``.
+checkpointed forward blocks) and ``log10_likelihoods`` wraps it as a ``torch.autograd.Function``.
+``precision="double"`` (the default) runs the forward and the adjoint in FP64;
+``precision="float"`` runs them in FP32 with FP64 accumulation of the gradient terms and an FP64
+rerun of every pair whose FP32 likelihood falls below 1e-28.
 
 Arrays cross the boundary in host memory (NumPy, or CPU torch tensors in the Function): the
 engine links ROCm's HIP runtime, and torch's wheel carries a HIP runtime of its own; one process
@@ -45,7 +47,9 @@ class GradProblem:
     orders them; per-base arrays follow the reads' bases.
     """
 
-    def __init__(self, data):
+    def __init__(self, data, precision: str = "double"):
+        if precision not in ("double", "float"):
+            raise ValueError("precision must be 'double' or 'float'")
         h = C.c_void_p()
         d = data
         _check(lib().fgkl_grad_create(
@@ -54,6 +58,9 @@ class GradProblem:
             _ptr(d.bases, C.c_uint8), _ptr(d.hap_offsets, C.c_int64),
             _ptr(d.hap_bases, C.c_uint8)))
         self._h = h
+        self.precision = precision
+        if precision == "float":
+            _check(lib().fgkl_grad_set_precision(h, 1))
         self.n_pairs = int(lib().fgkl_grad_pairs(h))
         self.n_read_bases = int(lib().fgkl_grad_read_bases(h))
 

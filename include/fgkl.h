@@ -308,11 +308,18 @@ int fgkl_selftest_dpp_shl(int32_t* out64);
  * (w_p = 1 when weights is NULL) and log10 L when log10_likelihoods is not NULL.  A pair with
  * w_p = 0, or whose FP64 likelihood underflows to 0, contributes nothing.  Calls on one handle
  * must not overlap (they share its device buffers).
+ *
+ * fgkl_grad_set_precision selects the arithmetic of later calls: 0 (the default) runs the
+ * forward, the checkpoints and the adjoint in FP64; 1 runs them in FP32 (likelihoods scaled by
+ * 2^120), adds each block's gradient terms into FP64 accumulators, and reruns in FP64 every pair
+ * whose FP32 likelihood falls below 1e-28 (the threshold the reference's FP32 kernel rescues
+ * at, crates/pairhmm/src/kernel.rs:897-914).  Other values return FGKL_INVALID_ARGUMENT.
  */
 typedef struct fgkl_grad fgkl_grad;
 int fgkl_grad_create(fgkl_grad** out, int64_t n_regions, const int64_t* region_reads,
                      const int64_t* region_haps, const int64_t* read_offsets,
                      const uint8_t* bases, const int64_t* hap_offsets, const uint8_t* hap_bases);
+int fgkl_grad_set_precision(fgkl_grad* g, int fp32);
 int64_t fgkl_grad_pairs(const fgkl_grad* g);
 int64_t fgkl_grad_read_bases(const fgkl_grad* g);
 int fgkl_grad_forward(fgkl_grad* g, const double* err, const double* ins, const double* del,

@@ -7,7 +7,10 @@
   adjoint workload (reads U(50,250) bp, up to 200 per region, up to 8 haplotypes);
 * the continuous-parameter forward of the oracle (log10 L per pair);
 * linearity in the pair weights, bitwise determinism, and the torch.autograd.Function (CPU
-  tensors: torch's wheel carries its own HIP runtime, so it stays off the device here).
+  tensors: torch's wheel carries its own HIP runtime, so it stays off the device here);
+* the FP32 mode (forward, tape and adjoint in FP32, FP64 accumulation, FP64 rerun of pairs below
+  1e-28) against the same oracle with FP32 bounds, and its rescue: reads lost against every
+  haplotype get the FP64 mode's bits, reads lost against some get the FP64 terms added.
 
 The reference computes no gradient; its model is the FP64 recurrence of reference.rs:20-64.
 """
@@ -58,11 +61,11 @@ def test_without_a_gpu_the_gradient_engine_fails_loudly():
 
 
 # ---- GPU --------------------------------------------------------------------------------------
-def _gpu_grad(data, weights=None, g=None):
+def _gpu_grad(data, weights=None, g=None, precision="double"):
     G = _grad_module()
     e, di, dd, g0 = G.probabilities(data)
     g = g0 if g is None else g
-    P = G.GradProblem(data)
+    P = G.GradProblem(data, precision=precision)
     ll, ge, gi, gd, gg = P.backward(e, di, dd, g, weights=weights)
     P.close()
     return (ll, ge, gi, gd, np.array([gg])), (e, di, dd, g)
@@ -87,11 +90,22 @@ def _mutated(rng: Rng, hap: bytes, n: int) -> bytes:
     return bytes(s)
 
 
-def _check_against_oracle(data, got, params, weights=None, rtol=1e-9):
+# FP32 mode bounds.  log10 L: the engine's FP32 bound everywhere (1e-4 absolute).  Gradients:
+# every forward and adjoint value is a product along a path of at most n + m <= 700 FP32
+# roundings (2^-24 each), so a term's relative error is at most ~700 * 6e-8 = 4.2e-5, and the sum
+# of terms of both signs is held to that times the largest component of the read set.
+F32_LOG10_TOL = 1e-4
+F32_RTOL = 4.2e-5
+
+
+def _check_against_oracle(data, got, params, weights=None, rtol=1e-9, ll_tol=None):
     e, di, dd, g = params
     ll, ge, gi, gd, gg = got
     rl, re_, ri, rd, rg = O.grad_batch(data, e, di, dd, g, weights)
-    assert np.all(np.abs(ll - rl) <= 1e-12 * np.abs(rl) + 1e-12), np.max(np.abs(ll - rl))
+    if ll_tol is None:
+        assert np.all(np.abs(ll - rl) <= 1e-12 * np.abs(rl) + 1e-12), np.max(np.abs(ll - rl))
+    else:
+        assert np.all(np.abs(ll - rl) <= ll_tol), np.max(np.abs(ll - rl))
     for name, a, b in (("e", ge, re_), ("d_ins", gi, ri), ("d_del", gd, rd)):
         # FP64 against FP64 in another summation order; terms of both signs are summed, so the
         # bound is relative to the largest gradient component of the read set.
@@ -109,15 +123,21 @@ def test_dpp_wave_shl_moves_each_lane_down(gpu):
     assert list(out) == [1000 + i + 1 for i in range(63)] + [-1]
 
 
-@pytest.mark.gpu
-def test_gradient_matches_the_oracle_on_seeded_regions(gpu):
-    d = Region.generate(11, 12, 150, 4, 220)   # 3 strips per read, 30 blocks per strip
-    got, params = _gpu_grad(d)
-    _check_against_oracle(d, got, params)
+def _bounds(precision):
+    return {} if precision == "double" else {"rtol": F32_RTOL, "ll_tol": F32_LOG10_TOL}
 
 
 @pytest.mark.gpu
-def test_gradient_matches_the_oracle_across_strip_and_block_edges(gpu):
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_gradient_matches_the_oracle_on_seeded_regions(gpu, precision):
+    d = Region.generate(11, 12, 150, 4, 220)   # 3 strips per read, 30 (FP64) / 15 (FP32) blocks
+    got, params = _gpu_grad(d, precision=precision)
+    _check_against_oracle(d, got, params, **_bounds(precision))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_gradient_matches_the_oracle_across_strip_and_block_edges(gpu, precision):
     rng = Rng(77)
     haps = [bytes(b"ACGT"[rng.below(4)] for _ in range(m)) for m in (1, 2, 7, 8, 9, 64, 151)]
     reads = []
@@ -126,28 +146,30 @@ def test_gradient_matches_the_oracle_across_strip_and_block_edges(gpu):
             _mutated(rng, haps[-1], n), *(getattr(_read(rng, n), f)
                                           for f in ("quals", "ins_gop", "del_gop", "gcp"))))
     d = batch_from_regions([(reads, haps)])
-    got, params = _gpu_grad(d)
-    _check_against_oracle(d, got, params)
+    got, params = _gpu_grad(d, precision=precision)
+    _check_against_oracle(d, got, params, **_bounds(precision))
 
 
 @pytest.mark.gpu
-def test_gradient_with_n_bases_matches_the_oracle(gpu):
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_gradient_with_n_bases_matches_the_oracle(gpu, precision):
     rng = Rng(91)
     hap = bytes(b"ACGTN"[rng.below(5)] for _ in range(90))
     reads = [F.ReadRef(_mutated(rng, hap, n).replace(b"A", b"N", 2),
                        *(getattr(_read(rng, n), f) for f in ("quals", "ins_gop", "del_gop", "gcp")))
              for n in (40, 70, 100)]
     d = batch_from_regions([(reads, [hap, hap[5:], b"NNNN" + hap])])
-    got, params = _gpu_grad(d)
-    _check_against_oracle(d, got, params)
+    got, params = _gpu_grad(d, precision=precision)
+    _check_against_oracle(d, got, params, **_bounds(precision))
 
 
 @pytest.mark.gpu
-def test_forward_matches_the_continuous_oracle(gpu):
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_forward_matches_the_continuous_oracle(gpu, precision):
     G = _grad_module()
     d = Region.generate(21, 9, 120, 5, 260)
     e, di, dd, g = G.probabilities(d)
-    P = G.GradProblem(d)
+    P = G.GradProblem(d, precision=precision)
     ll = P.forward(e, di, dd, g)
     base = d.out_base()
     H = d.n_haps
@@ -155,7 +177,8 @@ def test_forward_matches_the_continuous_oracle(gpu):
         lo, hi = int(d.read_offsets[r]), int(d.read_offsets[r + 1])
         for k in range(H):
             ref = O.cont_pair(d.hap(k), bytes(d.bases[lo:hi]), e[lo:hi], di[lo:hi], dd[lo:hi], g)
-            assert abs(ll[int(base[0]) + r * H + k] - ref) <= 1e-12 * abs(ref), (r, k)
+            tol = 1e-12 * abs(ref) if precision == "double" else F32_LOG10_TOL
+            assert abs(ll[int(base[0]) + r * H + k] - ref) <= tol, (r, k)
 
 
 def _fd_pairs(n_pairs=64, seed=3):
@@ -239,12 +262,57 @@ def test_weights_scale_the_gradient_and_zero_weights_remove_pairs(gpu):
 
 
 @pytest.mark.gpu
-def test_backward_is_bitwise_deterministic(gpu):
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_backward_is_bitwise_deterministic(gpu, precision):
     d = workload(4, seed=9, n_regions=6)
-    a, _ = _gpu_grad(d)
-    b, _ = _gpu_grad(d)
+    a, _ = _gpu_grad(d, precision=precision)
+    b, _ = _gpu_grad(d, precision=precision)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def _riddled(rng: Rng, hap: bytes, start: int, n: int, every: int) -> F.ReadRef:
+    """A window of `hap` with a substitution every `every` bases at Phred 35: log10 L falls by
+    about 4 per substitution, below the FP32 threshold (scaled L < 1e-28, log10 L < -64.1) for
+    n / every > 20, while staying far inside FP64's range."""
+    b = bytearray(hap[start:start + n])
+    for j in range(0, n, every):
+        b[j] = ord("A") if b[j] != ord("A") else ord("C")
+    return F.ReadRef(bytes(b), [35] * n, [40] * n, [40] * n, [10] * n)
+
+
+@pytest.mark.gpu
+def test_float_mode_reruns_lost_pairs_in_double(gpu):
+    """Reads lost against every haplotype: the FP32 pass leaves their gradients at zero and the
+    FP64 rerun adds every term in the FP64 mode's order, so their gradients, log10 L and the
+    g term are the FP64 mode's bits.  A read lost against one haplotype only gets that pair's
+    FP64 terms added to its FP32 ones and matches the oracle at the FP32 bounds."""
+    rng = Rng(41)
+    hap = bytes(b"ACGT"[rng.below(4)] for _ in range(420))
+    alt = bytearray(hap)
+    for j in range(100, 300, 9):  # alt differs from hap every 9 bases over the span 100..300
+        alt[j] = ord("G") if alt[j] != ord("G") else ord("T")
+    alt = bytes(alt)
+    ok = [F.ReadRef(_mutated(rng, hap, n), *(getattr(_read(rng, n), f)
+                                             for f in ("quals", "ins_gop", "del_gop", "gcp")))
+          for n in (90, 150)]
+    lost_both = [_riddled(rng, hap, 20, 200, 8), _riddled(rng, hap, 150, 180, 7)]
+    half = F.ReadRef(hap[110:290], [35] * 180, [40] * 180, [40] * 180, [10] * 180)
+    reads = ok + lost_both + [half]
+    d = batch_from_regions([(reads, [hap, alt])])
+    (l64, e64, i64, d64, g64), params = _gpu_grad(d)
+    (l32, e32, i32, d32, g32), _ = _gpu_grad(d, precision="float")
+    assert np.all(np.isfinite(l64))
+    H = 2
+    assert np.all(l64[2 * H:4 * H] < -70), l64  # both riddled reads, both haplotypes
+    assert l64[4 * H] > -10 and l64[4 * H + 1] < -70, l64[4 * H:]  # half: lost against alt only
+    assert np.array_equal(l32[2 * H:5 * H][[0, 1, 2, 3, 5]], l64[2 * H:5 * H][[0, 1, 2, 3, 5]])
+    for r in (2, 3):
+        lo, hi = int(d.read_offsets[r]), int(d.read_offsets[r + 1])
+        for a, b in ((e32, e64), (i32, i64), (d32, d64)):
+            assert np.array_equal(a[lo:hi], b[lo:hi]), r
+    _check_against_oracle(d, (l32, e32, i32, d32, g32), params, rtol=F32_RTOL,
+                          ll_tol=F32_LOG10_TOL)
 
 
 @pytest.mark.gpu
