@@ -49,12 +49,14 @@ template <> struct GT<double> {
   static constexpr double kLog10Init = 1020.0 * 0.301029995663981195213738894724493027;
   static constexpr double kMinAccepted = 0.0;
   static constexpr int kTape = 8;
+  static constexpr bool kPrefetch = false;  // its registers would spill
 };
 template <> struct GT<float> {
   static constexpr double kInit = 0x1p120;
   static constexpr double kLog10Init = 120.0 * 0.301029995663981195213738894724493027;
   static constexpr double kMinAccepted = 1e-28;
   static constexpr int kTape = 16;
+  static constexpr bool kPrefetch = true;   // the next block's inputs load during this block
 };
 
 constexpr double kInvLn10 = 0.434294481903251827651128918916605082;
@@ -224,19 +226,21 @@ __device__ __forceinline__ void fblock(St<T>& st, St<T>& P, T& acc, St<T>* tp, c
   if constexpr (TAPE) tp[K] = st;
 }
 
-// Stages columns t0 .. t0+K of a stored boundary row (clamped to 0..m) for lane 0; lanes past K
-// write their copy into the dummy area.
+// Column t0 + j (clamped to 0..m) of a stored boundary row, the value lane j <= K stages for
+// lane 0 (lanes past K load column t0 + K and stage it into the dummy area).  Loaded one block
+// ahead of its use, so the global load's latency overlaps a block of compute.
 template <typename T>
-__device__ __forceinline__ void stage_above(const Lds<T>& l, const T* bnd, int t0, int m,
-                                            int lane) {
+__device__ __forceinline__ St4<T> load_above(const T* bnd, int t0, int m, int lane) {
   constexpr int K = GT<T>::kTape;
-  wave_sync_lds();
-  const int j = min(lane, K);
-  const int col = min(t0 + j, m);
-  const St4<T> w = *reinterpret_cast<const St4<T>*>(bnd + size_t(col) * 4);
-  St4<T>* dst = lane <= K ? &l.stage[j] : reinterpret_cast<St4<T>*>(l.dummy + lane * kSlot);
+  const int col = min(t0 + min(lane, K), m);
+  return *reinterpret_cast<const St4<T>*>(bnd + size_t(col) * 4);
+}
+
+template <typename T>
+__device__ __forceinline__ void put_above(const Lds<T>& l, const St4<T>& w, int lane) {
+  constexpr int K = GT<T>::kTape;
+  St4<T>* dst = lane <= K ? &l.stage[lane] : reinterpret_cast<St4<T>*>(l.dummy + lane * kSlot);
   *dst = St4<T>{w.m, w.x, w.y, T(0)};
-  wave_sync_lds();
 }
 
 // Pass F over strip s: returns the lane's sum of M + X over columns 1..m when ACC (row n's lane
@@ -254,9 +258,15 @@ __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw
   T acc = T(0);
   St4<T>* gdummy = reinterpret_cast<St4<T>*>(wscr + a.dummy_off + lane * kSlot);
   if (has_next) *reinterpret_cast<St4<T>*>(bnd_out) = St4<T>{T(0), T(0), T(0), T(0)};
+  St4<T> above = load_above<T>(bnd_in, 0, m, lane);
   for (int b = 0; b < nb; ++b) {
     const int t0 = b * K;
-    stage_above(l, bnd_in, t0, m, lane);
+    if constexpr (!GT<T>::kPrefetch) above = load_above<T>(bnd_in, t0, m, lane);
+    wave_sync_lds();
+    put_above(l, above, lane);
+    if constexpr (GT<T>::kPrefetch)
+      above = load_above<T>(bnd_in, t0 + K, m, lane);  // the next block's (clamped: in bounds)
+    wave_sync_lds();
     if (b == 0) P = shr_st(l.stage[0], st);  // the diagonal of step 0: column 0 of the row above
     if constexpr (CK) {
       T* c = ck + size_t(b) * 6 * 64 + lane;
@@ -362,21 +372,44 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
   ab_out[0] = Adj2<T>{T(0), T(0)};
   ab_out[m + 1] = Adj2<T>{T(0), T(0)};
   T AX = T(0), AY = T(0), Q = T(0), Bq = T(0);  // columns beyond m: zero adjoints
-  double ge = 0.0, gmm = 0.0, gim = 0.0, gmi = 0.0, gmd = 0.0, gg = 0.0;
+  // d/d e, d/d d_ins (M->I minus M->M), d/d d_del (M->D minus M->M), d/d g (I->I, D->D minus
+  // I->M, D->M): each block's terms combine in the working precision, then add in FP64.
+  double ge = 0.0, gi = 0.0, gd = 0.0, gq = 0.0;
+  // A block's inputs, loaded one block ahead: the row above (columns t0..t0+K, for lane 0), the
+  // adjoint row below (columns t0-62..t0-63+K, for lane 63) and the lane's checkpoint.
+  struct In {
+    St4<T> above;
+    Adj2<T> below;
+    T c[6];
+  };
+  auto load = [&](int b) {
+    const int t0 = b * K;
+    In in;
+    in.above = load_above<T>(bnd_in, t0, m, lane);
+    in.below = ab_in[min(max(t0 + lane - 62, 0), m + 1)];
+    const T* c = ck + size_t(b) * 6 * 64 + lane;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) in.c[i] = c[i * 64];
+    return in;
+  };
+  In next = load(nb - 1);
   for (int b = nb - 1; b >= 0; --b) {
     const int t0 = b * K;
-    {
-      const Adj2<T> w = ab_in[min(max(t0 + lane - 62, 0), m + 1)];
-      Adj2<T>* dst = lane < K ? &l.astage[lane] : reinterpret_cast<Adj2<T>*>(l.dummy + lane * kSlot);
-      wave_sync_lds();
-      *dst = w;
+    In cur;
+    if constexpr (GT<T>::kPrefetch) {
+      cur = next;
+      next = load(max(b - 1, 0));
+    } else {
+      cur = load(b);
     }
-    stage_above(l, bnd_in, t0, m, lane);
+    wave_sync_lds();
+    *(lane < K ? &l.astage[lane] : reinterpret_cast<Adj2<T>*>(l.dummy + lane * kSlot)) = cur.below;
+    put_above(l, cur.above, lane);
+    wave_sync_lds();
     // Recompute the block from its checkpoint into the tape: tp[k] is the state before step k.
-    const T* c = ck + size_t(b) * 6 * 64 + lane;
     St<T> tp[K + 1];
-    St<T> st{c[0 * 64], c[1 * 64], c[2 * 64]};
-    const St<T> P0{c[3 * 64], c[4 * 64], c[5 * 64]};
+    St<T> st{cur.c[0], cur.c[1], cur.c[2]};
+    const St<T> P0{cur.c[3], cur.c[4], cur.c[5]};
     St<T> P = P0;
     T unused = T(0);
     if (t0 < L - 1)
@@ -389,11 +422,9 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
     else
       rblock<T, false>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
     ge += double(gs.ge);
-    gmm += double(gs.gmm);
-    gim += double(gs.gim);
-    gmi += double(gs.gmi);
-    gmd += double(gs.gmd);
-    gg += double(gs.gg);
+    gi += double(gs.gmi - gs.gmm);
+    gd += double(gs.gmd - gs.gmm);
+    gq += double(gs.gg - gs.gim);
     wave_sync_lds();
     {
       const int col = t0 + lane + 1;  // lane 0's column at step t0 + lane
@@ -409,13 +440,16 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
   double* pi = real ? a.g_ins + i : reinterpret_cast<double*>(gdummy) + 1;
   double* pd = real ? a.g_del + i : reinterpret_cast<double*>(gdummy) + 2;
   *pe += ge * kInvLn10;
-  *pi += (gmi - gmm) * kInvLn10;  // M->M = 1 - d_ins - d_del
-  *pd += (gmd - gmm) * kInvLn10;
-  dgl += real ? gg - gim : 0.0;   // I->M = D->M = 1 - g
+  *pi += gi * kInvLn10;  // M->M = 1 - d_ins - d_del
+  *pd += gd * kInvLn10;
+  dgl += real ? gq : 0.0;  // I->M = D->M = 1 - g
 }
 
+// FP32: three waves per SIMD (at most 168 VGPRs) measured faster than two without spills.
 template <typename T, bool BWD>
-__global__ void __launch_bounds__(256) grad_kernel(GradArgs a) {
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 3 : 1)))
+grad_kernel(GradArgs a) {
   constexpr int K = GT<T>::kTape;
   constexpr bool kFp32 = std::is_same<T, float>::value;
   extern __shared__ __attribute__((aligned(32))) uint8_t glds[];

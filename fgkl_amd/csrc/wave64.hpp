@@ -27,6 +27,9 @@ __device__ __forceinline__ double shr1(double old, double src) {
   const uint32_t hi = uint32_t(shr1i(int(uint32_t(o >> 32)), int(uint32_t(s >> 32))));
   return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
+__device__ __forceinline__ float shl1(float old, float src) {
+  return __builtin_bit_cast(float, shl1i(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src)));
+}
 __device__ __forceinline__ double shl1(double old, double src) {
   const uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
   const uint32_t lo = uint32_t(shl1i(int(uint32_t(o)), int(uint32_t(s))));
