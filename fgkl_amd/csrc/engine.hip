@@ -47,8 +47,11 @@ template <typename T> struct Kernels;
 template <> struct Kernels<float> {
   static KernelF get(bool pd, bool boost, int rows) { return kSweepF32[pd][boost][rows - 1]; }
 };
+// FP64 has no occupancy-raised build; its flag selects the suffix-sharing build (plain only).
 template <> struct Kernels<double> {
-  static KernelD get(bool pd, bool, int rows) { return kSweepF64[pd][rows - 1]; }
+  static KernelD get(bool pd, bool suffix, int rows) {
+    return suffix && !pd ? kSweepF64Suffix[rows - 1] : kSweepF64[pd][rows - 1];
+  }
 };
 template <typename T> struct LongKernel;
 template <> struct LongKernel<float> {
@@ -399,15 +402,16 @@ struct DeviceBatch::Impl {
     return kOk;
   }
 
-  // Launches bucket R's sweep with configuration `c` (FP64 launches ignore `boost`).  The
-  // scratch stride follows the kernel's rows per lane.
+  // Launches bucket R's sweep with configuration `c` (FP64 launches ignore `boost` and take the
+  // suffix-sharing build when `a.suffix` is set).  The scratch stride follows the kernel's rows
+  // per lane.
   template <typename T>
   int launch(int R, SweepArgs a, int64_t max_items, LaunchCfg c = LaunchCfg{},
              hipStream_t on = nullptr, size_t scratch_off = 0) {
     if (max_items <= 0) return kOk;
     if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
-    if (!std::is_same<T, float>::value) c.boost = false;
+    if (!std::is_same<T, float>::value) c.boost = a.suffix != 0;  // FP64: the suffix build
     int grid = 0;
     if (int rc = grid_waves<T>(c, rows, max_items, &grid)) return rc;
     if (a.scratch) {
@@ -437,7 +441,7 @@ struct DeviceBatch::Impl {
     if (grid_waves<T>(c, rows, max_items < 0 ? bucket_n[R - 1] : max_items, &grid)) return -1;
     return int64_t(grid) * (c.block / 64) * scratch_stride<T>(rows) * int64_t(sizeof(T));
   }
-  // Per wave: the snapshot slots, then (FP32 plain) the suffix-sharing node records.
+  // Per wave: the snapshot slots, then (plain) the suffix-sharing node records.
   template <typename T>
   int64_t snap_elems(int rows) const {
     const int nv = ((pd ? 6 * rows + 7 : 3 * rows + 4) + 3) & ~3;  // snap_stride
@@ -445,7 +449,7 @@ struct DeviceBatch::Impl {
   }
   template <typename T>
   int64_t scratch_stride(int rows) const {
-    const bool nodes = std::is_same<T, float>::value && !pd;
+    const bool nodes = !pd;
     const int ns = (2 * rows + 1 + 3) & ~3;  // node_stride
     return snap_elems<T>(rows) + (nodes ? int64_t(plan.max_nodes) * ns * 64 : 0);
   }
@@ -529,19 +533,23 @@ struct DeviceBatch::Impl {
     return rows >= R && rows <= max_rows(R) && block_index(c.block) >= 0;
   }
 
+  // Whether a main pass with prefix sharing `share` also shares suffixes (plain batches with a
+  // suffix plan, both precisions).
+  bool suffix_run(int share) const { return share && !pd && !plan.bsweeps.empty(); }
+
   SweepArgs primary_args(int R, int share, bool fp32) const {
     SweepArgs a = base_args();
     a.items = bucket_items[R - 1];
     a.n_items = bucket_n[R - 1];
     a.counter = counters + (R - 1);
     a.share = share;
-    a.suffix = fp32 && share && !pd && !plan.bsweeps.empty();
-    if (fp32) {
+    a.suffix = suffix_run(share);
+    if (fp32 || a.suffix) {  // FP64: joins below 1e24 are queued for an unshared rerun
       a.rescue = 1;
       for (int q = 0; q < kMaxRowsPerLane; ++q) a.rescue_pairs[q] = rescue_pairs[q];
       a.rescue_count = counters + 16;
       a.rescue_by_rows = 1;
-      if (!pd) {  // PD batches rescue pair by pair (see enqueue)
+      if (fp32 && !pd) {  // PD batches rescue pair by pair (see enqueue)
         a.dense = dense_flags;
         for (int q = 0; q < kMaxRowsPerLane; ++q) a.dense_items[q] = dense_items[q];
         a.dense_count = counters + 24;
@@ -609,12 +617,15 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
     }
     if (timed) HIP_TRY(hipEventRecord(ev[3], stream));
   }
-  if (fp32) {
+  // FP64 joins below 1e24 are recomputed without suffix sharing (lib.rs:467-487), as are the
+  // pairs an FP32 pass lost that the dense pass did not take.
+  const bool joins64 = !fp32 && suffix_run(share);
+  if (fp32 || joins64) {
     // Lost pairs by their read's own rows per lane, a wave each.
-    for (int R = 1; R <= kMaxRowsPerLane && mode == kModeFp32Rescue; ++R) {
+    for (int R = 1; R <= kMaxRowsPerLane && (mode == kModeFp32Rescue || joins64); ++R) {
       if (!rows_pairs[R - 1]) continue;
       SweepArgs a = base_args();
-      a.dense = pd ? nullptr : dense_flags;  // pairs of dense reads are done
+      a.dense = pd || joins64 ? nullptr : dense_flags;  // pairs of dense reads are done
       a.items = rescue_pairs[R - 1];
       a.n_items = 0;
       a.n_items_dev = counters + 16 + (R - 1);
@@ -795,6 +806,7 @@ int DeviceBatch::assign(const BatchInput& in) {
       int64_t n = 0;
       for (const LaunchCfg& c : d.candidates(R, true)) n = std::max(n, d.scratch_need<float>(R, c));
       n = std::max(n, d.scratch_need<double>(R, LaunchCfg{}));
+      if (d.suffix_run(1)) n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, true}));
       need = std::max(need, n);  // tuning runs every bucket on the batch's stream
       if (d.side[R - 1]) side_need = std::max(side_need, n);
     }
@@ -878,6 +890,8 @@ int DeviceBatch::run(int mode, int share, int use_graph, RunStats* stats) {
     if (mode == kModeFp32Rescue || mode == kModeFp32NoRescue) {
       for (int R = 0; R < kMaxRowsPerLane; ++R) rescued += counts[16 + R];
       rescued += counts[42];
+    } else if (d.suffix_run(share)) {  // FP64 joins rerun unshared
+      for (int R = 0; R < kMaxRowsPerLane; ++R) rescued += counts[16 + R];
     }
     stats->rescued_pairs = rescued;
     for (int R = 0; R < kMaxRowsPerLane; ++R) {

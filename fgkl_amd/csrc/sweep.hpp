@@ -36,15 +36,22 @@ namespace fgkl {
 // ---------------------------------------------------------------------------------------------
 // Numeric constants (simd.rs:102-141).
 template <typename T> struct Num;
+// kBwdScale: the weight of a path that ends, in a backward sweep (simd.rs:106-136);
+// kMinAcceptedJoined: the smallest accepted scaled likelihood of a joined pair.  FP64 joins below
+// 1e24 (a likelihood of about 1e-283) are rerun without suffix sharing (lib.rs:467-487).
 template <> struct Num<float> {
   static constexpr double kScale = 0x1p120;                       // 2^120
   static constexpr double kLog10Scale = 120.0 * 0.301029995663981195213738894724493027;
   static constexpr double kMinAccepted = 1e-28;
+  static constexpr double kBwdScale = 0x1p120;
+  static constexpr double kMinAcceptedJoined = 1e-28;
 };
 template <> struct Num<double> {
   static constexpr double kScale = 0x1p1020;                      // 2^1020
   static constexpr double kLog10Scale = 1020.0 * 0.301029995663981195213738894724493027;
-  static constexpr double kMinAccepted = 0.0;                     // never rescued
+  static constexpr double kMinAccepted = 0.0;                     // never rescued unjoined
+  static constexpr double kBwdScale = 1.0;
+  static constexpr double kMinAcceptedJoined = 1e24;
 };
 
 template <typename T> struct DevTab {
@@ -92,7 +99,7 @@ struct SweepArgs {
   int32_t* dense_count;                   // kMaxRowsPerLane counts
   int32_t nan_only;
   int32_t lds_words;           // LDS words per wave for the staged haplotype
-  // Suffix sharing (FP32 plain main pass only, when `suffix` is set): the backward sweeps of
+  // Suffix sharing (plain main pass only, when `suffix` is set): the backward sweeps of
   // region g are bsweeps[region_bw[g] .. region_bw[g + 1]) with node writes in bwrites; node
   // records sit `node_off` elements into the wave's scratch, after its snapshot slots.
   const BwdSweep* bsweeps;
@@ -489,9 +496,9 @@ __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Suffix sharing (FP32 plain): backward sweeps and the join (kernel.rs:579-602, 744-891,
-// 1029-1126).  B_M, B_X, B_Y of a cell are the scaled mass of the paths from that cell to the end
-// of the read (2^120 per path that ends, simd.rs:109).  The backward recurrence mirrors the
+// Suffix sharing (plain, FP32 and FP64): backward sweeps and the join (kernel.rs:579-602,
+// 744-891, 1029-1126).  B_M, B_X, B_Y of a cell are the scaled mass of the paths from that cell to
+// the end of the read (2^120 per path that ends in FP32, 1 in FP64: simd.rs:109, 134).  The backward recurrence mirrors the
 // forward one: row i at column j reads row i+1 at column j+1 (match, through the prior and
 // transitions of read position i+1), row i+1 at column j (insertion) and row i at column j+1
 // (deletion).  In the lane layout rows run bottom-up inside a lane and the row below a lane's
@@ -506,8 +513,8 @@ template <int R> constexpr int node_stride() { return (2 * R + 1 + 3) & ~3; }
 
 // Parameters of the row below a lane's last row (lane l+1's first row), zero for a read's last
 // lane (nothing lies below its last row).
-struct EdgeRow {
-  float pm, px, mm, im, mi, gc;
+template <typename T> struct EdgeRow {
+  T pm, px, mm, im, mi, gc;
   uint32_t rb;
 };
 
@@ -543,9 +550,9 @@ __device__ __forceinline__ void stage_bwd(const SweepArgs& a, const BwdSweep& b,
 // The state a lane carries through a backward sweep: its rows' B_M, B_X, B_Y at the column it
 // computed last, B_M of the row below its last row one column further right, and the word of
 // the column it is on.
-template <int R> struct BLane {
-  float bm[R], bx[R], by[R];
-  float P;
+template <typename T, int R> struct BLane {
+  T bm[R], bx[R], by[R];
+  T P;
   uint32_t w;
 };
 
@@ -553,30 +560,30 @@ template <int R> struct BLane {
 // new values) and one column to the right (match, the old value), and itself one column to the
 // right (deletion); the row below the last row is lane l+1's first row (`qm`, `qx` this column,
 // P the previous one), with parameters `eg`.  Records a node when the word carries one.
-template <int R>
-__device__ __forceinline__ void bcells(BLane<R>& s, const Rows<float, R, false>& rw,
-                                       const EdgeRow& eg, float qm, float qx, float sink,
-                                       float* nodes, int lane) {
+template <typename T, int R>
+__device__ __forceinline__ void bcells(BLane<T, R>& s, const Rows<T, R, false>& rw,
+                                       const EdgeRow<T>& eg, T qm, T qx, T sink, T* nodes,
+                                       int lane) {
   const uint32_t hb = s.w & 0x1FFu;
   const bool hN = hb == uint32_t('N');
-  float below_x = qx, diag = s.P;
+  T below_x = qx, diag = s.P;
 #pragma unroll
   for (int k = R - 1; k >= 0; --k) {
     const bool edge = k == R - 1;
     const int kb = edge ? k : k + 1;
-    const float pmb = edge ? eg.pm : rw.pm[kb];
-    const float pxb = edge ? eg.px : rw.px[kb];
+    const T pmb = edge ? eg.pm : rw.pm[kb];
+    const T pxb = edge ? eg.px : rw.px[kb];
     const uint32_t rbb = edge ? eg.rb : rw.rb[kb];
-    const float mmb = edge ? eg.mm : rw.mm[kb];
-    const float imb = edge ? eg.im : rw.im[kb];
-    const float mib = edge ? eg.mi : rw.mi[kb];
-    const float iib = edge ? eg.gc : rw.gc[kb];
+    const T mmb = edge ? eg.mm : rw.mm[kb];
+    const T imb = edge ? eg.im : rw.im[kb];
+    const T mib = edge ? eg.mi : rw.mi[kb];
+    const T iib = edge ? eg.gc : rw.gc[kb];
     const bool match = hN | (hb == rbb);
-    const float into = (match ? pmb : pxb) * diag;  // kernel.rs:1071-1082
-    const float ind = into * imb;
-    float nbx = fmaf(below_x, iib, ind);
-    const float nby = fmaf(s.by[k], rw.gc[k], ind);
-    float nbm = fmaf(s.by[k], rw.md[k], fmaf(below_x, mib, into * mmb));
+    const T into = (match ? pmb : pxb) * diag;  // kernel.rs:1071-1082
+    const T ind = into * imb;
+    T nbx = fma(below_x, iib, ind);
+    const T nby = fma(s.by[k], rw.gc[k], ind);
+    T nbm = fma(s.by[k], rw.md[k], fma(below_x, mib, into * mmb));
     if (edge) {
       nbm += sink;
       nbx += sink;
@@ -590,7 +597,7 @@ __device__ __forceinline__ void bcells(BLane<R>& s, const Rows<float, R, false>&
   s.P = qm;
   const uint32_t node = s.w >> kSlotShift;
   if (node) {
-    float* r = nodes + (size_t(node - 1) * 64 + lane) * node_stride<R>();
+    T* r = nodes + (size_t(node - 1) * 64 + lane) * node_stride<R>();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       r[k] = s.bm[k];
@@ -611,32 +618,32 @@ __device__ __forceinline__ void bcells(BLane<R>& s, const Rows<float, R, false>&
 // lane L's values into lane L-1, where a stale NaN times the zero parameters below a read's last
 // row would still be NaN.  `full` is the step from which every lane of the item is inside the
 // sweep (the largest delay).
-template <int R>
-__device__ void bsweep(const Rows<float, R, false>& rw, const EdgeRow& eg, const uint32_t* lw,
+template <typename T, int R>
+__device__ void bsweep(const Rows<T, R, false>& rw, const EdgeRow<T>& eg, const uint32_t* lw,
                        int Wb, int steps, int full, int lane, int L, int delay, uint32_t from_lds,
-                       float sink, const float* rec_in, float* nodes) {
+                       T sink, const T* rec_in, T* nodes) {
   constexpr int NS = node_stride<R>();
-  BLane<R> s;
-  s.P = 0.f;
+  BLane<T, R> s;
+  s.P = T(0);
   s.w = 0;
   if (rec_in) {
-    const float* r = rec_in + lane * NS;
+    const T* r = rec_in + lane * NS;
     const bool live = lane < L;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      s.bm[k] = live ? r[k] : 0.f;
-      s.by[k] = live ? r[R + k] : 0.f;
-      s.bx[k] = 0.f;
+      s.bm[k] = live ? r[k] : T(0);
+      s.by[k] = live ? r[R + k] : T(0);
+      s.bx[k] = T(0);
     }
-    s.P = live ? r[2 * R] : 0.f;
+    s.P = live ? r[2 * R] : T(0);
   } else {
 #pragma unroll
-    for (int k = 0; k < R; ++k) s.bm[k] = s.bx[k] = s.by[k] = 0.f;
+    for (int k = 0; k < R; ++k) s.bm[k] = s.bx[k] = s.by[k] = T(0);
   }
   auto step = [&](uint32_t word, bool live) {
     s.w = (shl1z(s.w) & ~from_lds) | (word & from_lds);
-    const float qm = shl1z(s.bm[0]), qx = shl1z(s.bx[0]);  // the row below, this column
-    if (live) bcells<R>(s, rw, eg, qm, qx, sink, nodes, lane);
+    const T qm = shl1z(s.bm[0]), qx = shl1z(s.bx[0]);  // the row below, this column
+    if (live) bcells<T, R>(s, rw, eg, qm, qx, sink, nodes, lane);
   };
   // Ramp: lanes start one after the other; rounded up to four steps so the steady loop reads
   // aligned groups of four words.
@@ -664,8 +671,10 @@ constexpr int boost_waves(int R, bool PD) {
             : (R <= 1 ? 8 : R <= 2 ? 7 : R <= 3 ? 6 : R <= 5 ? 5 : 4);
 }
 
-// BOOST selects the occupancy-raised variant (FP32 only).
-template <typename T, int R, bool PD, bool BOOST>
+// BOOST selects the occupancy-raised variant (FP32 only).  SFX compiles the suffix-sharing path
+// (backward sweeps and joins); FP64 has a build without it, whose registers are not raised by
+// that path, for the runs that share no suffixes (the rescue of FP32 pairs among them).
+template <typename T, int R, bool PD, bool BOOST, bool SFX>
 __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     sweep_kernel(SweepArgs a, DevTab<T> tab) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds_all[];
@@ -734,18 +743,18 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     const int span = two ? max(La, Lb) : La;
 
     // Suffix sharing: the region's backward sweeps, once per item, before its forward sweeps.
-    constexpr bool kSuffix = std::is_same<T, float>::value && !PD;
+    constexpr bool kSuffix = SFX && !PD;
     const bool suffix = kSuffix && a.suffix && !a.pair_mode;
     T* nodes = scratch ? scratch + a.node_off : nullptr;
     if constexpr (kSuffix) {
       if (suffix) {
         const bool bottom = lane == La - 1 || (two && lane == L - 1);
         const int delay = lane < La ? La - 1 - lane : (lane < L ? L - 1 - lane : 1 << 20);
-        const float sink = bottom ? float(Num<float>::kScale) : 0.f;
+        const T sink = bottom ? T(Num<T>::kBwdScale) : T(0);
         // All ones in the read's last lanes; opaque (see shift_up and bsweep).
         uint32_t from_lds = bottom ? ~0u : 0u;
         asm volatile("" : "+v"(from_lds));
-        EdgeRow eg;  // lane l+1's first row, zero below a read's last lane
+        EdgeRow<T> eg;  // lane l+1's first row, zero below a read's last lane
         eg.pm = and_bits(shl1z(rw.pm[0]), ~from_lds);
         eg.px = and_bits(shl1z(rw.px[0]), ~from_lds);
         eg.mm = and_bits(shl1z(rw.mm[0]), ~from_lds);
@@ -767,7 +776,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
           h.len = __builtin_amdgcn_readfirstlane(h.len);
           stage_bwd(a, b, h, L, lane, lw);
           const int Wb = b.right - b.left;
-          bsweep<R>(rw, eg, lw, Wb, Wb + steps_extra, steps_extra, lane, L, delay, from_lds, sink,
+          bsweep<T, R>(rw, eg, lw, Wb, Wb + steps_extra, steps_extra, lane, L, delay, from_lds, sink,
                     b.start_node >= 0 ? nodes + size_t(b.start_node) * 64 * node_stride<R>()
                                       : nullptr,
                     nodes);
@@ -814,8 +823,10 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
           sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
                                  scratch, row0_above, keep, st);
         // A read's last row is its last lane's last register (kernel.rs:897-914).
-        auto finish = [](double raw) {
-          if (Num<T>::kMinAccepted > 0.0 && !(raw >= Num<T>::kMinAccepted)) return __builtin_nan("");
+        // A joined pair is held to kMinAcceptedJoined (kernel.rs:261-275).
+        auto finish = [cut](double raw) {
+          const double lo = cut ? Num<T>::kMinAcceptedJoined : Num<T>::kMinAccepted;
+          if (lo > 0.0 && !(raw >= lo)) return __builtin_nan("");
           return log10(raw) - Num<T>::kLog10Scale;
         };
         double raw_a = double(readlane(st.acc, La - 1));
@@ -830,9 +841,9 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
             // B's), is summed in chunks of eight rows, and the chunks in order: the grouping
             // depends on row numbers only, not on how the rows sit in lanes, so a read gets the
             // same bits whatever it shares its wave with and at any rows per lane.
-            const float um = shr1z(st.m[R - 1]), ux = shr1z(st.x[R - 1]), uy = shr1z(st.y[R - 1]);
+            const T um = shr1z(st.m[R - 1]), ux = shr1z(st.x[R - 1]), uy = shr1z(st.y[R - 1]);
             const uint32_t hc = uint32_t(a.hap_bytes[p.off + p.cut - 1]);
-            const float* rec = nodes + (size_t(p.cut_node) * 64 + lane) * node_stride<R>();
+            const T* rec = nodes + (size_t(p.cut_node) * 64 + lane) * node_stride<R>();
             double* rows = reinterpret_cast<double*>(lw);   // [na + 1 + nb + 1]
             double* chunks = rows + 2 * kMaxRowsPerLane * 64 / 2;  // after 512 rows
             const int base = in_b ? na + 1 : 0;
@@ -840,13 +851,13 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
             wave_sync_lds();
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-              const float dm = k ? st.m[k - 1] : um;
-              const float dx = k ? st.x[k - 1] : ux;
-              const float dy = k ? st.y[k - 1] : uy;
+              const T dm = k ? st.m[k - 1] : um;
+              const T dx = k ? st.x[k - 1] : ux;
+              const T dy = k ? st.y[k - 1] : uy;
               const bool match = (hc == uint32_t('N')) | (hc == rw.rb[k]);
-              const float jm = (match ? rw.pm[k] : rw.px[k]) *
-                               fmaf(dy, rw.im[k], fmaf(dx, rw.im[k], dm * rw.mm[k]));
-              const float jy = fmaf(st.y[k], rw.gc[k], st.m[k] * rw.md[k]);
+              const T jm = (match ? rw.pm[k] : rw.px[k]) *
+                           fma(dy, rw.im[k], fma(dx, rw.im[k], dm * rw.mm[k]));
+              const T jy = fma(st.y[k], rw.gc[k], st.m[k] * rw.md[k]);
               const double term = fma(double(jm), double(rec[k]), double(jy) * double(rec[R + k]));
               const int row = first + k + 1;  // DP row; rows above DP row 0 hold nothing
               if (lane < L && row >= 0) rows[base + row] = term;
@@ -865,8 +876,8 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
             for (int c = 0; c < ca; ++c) sa += chunks[c];
             for (int c = ca; c < ca + cb; ++c) sb += chunks[c];
             wave_sync_lds();
-            raw_a += sa * (1.0 / Num<float>::kScale);
-            raw_b += sb * (1.0 / Num<float>::kScale);
+            raw_a += sa * (1.0 / Num<T>::kBwdScale);
+            raw_b += sb * (1.0 / Num<T>::kBwdScale);
           }
         }
         res_a = finish(raw_a);
@@ -1092,9 +1103,10 @@ using KernelF = void (*)(SweepArgs, DevTab<float>);
 using KernelD = void (*)(SweepArgs, DevTab<double>);
 
 // Every instantiation the host launches: [PD][BOOST][R-1] for FP32 (sweep_f32.hip), [PD][R-1]
-// for FP64 (sweep_f64.hip).
+// for FP64 without suffix sharing and [R-1] for plain FP64 with it (sweep_f64.hip).
 extern const KernelF kSweepF32[2][2][kMaxRowsPerLane];
 extern const KernelD kSweepF64[2][kMaxRowsPerLane];
+extern const KernelD kSweepF64Suffix[kMaxRowsPerLane];
 // long_kernel by [PD].
 extern const KernelF kLongF32[2];
 extern const KernelD kLongF64[2];

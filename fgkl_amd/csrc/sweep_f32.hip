@@ -5,10 +5,10 @@
 namespace fgkl {
 
 #define FGKL_F32_ROW(PD, B)                                                                 \
-  {sweep_kernel<float, 1, PD, B>, sweep_kernel<float, 2, PD, B>,                            \
-   sweep_kernel<float, 3, PD, B>, sweep_kernel<float, 4, PD, B>,                            \
-   sweep_kernel<float, 5, PD, B>, sweep_kernel<float, 6, PD, B>,                            \
-   sweep_kernel<float, 7, PD, B>, sweep_kernel<float, 8, PD, B>}
+  {sweep_kernel<float, 1, PD, B, !PD>, sweep_kernel<float, 2, PD, B, !PD>,                  \
+   sweep_kernel<float, 3, PD, B, !PD>, sweep_kernel<float, 4, PD, B, !PD>,                  \
+   sweep_kernel<float, 5, PD, B, !PD>, sweep_kernel<float, 6, PD, B, !PD>,                  \
+   sweep_kernel<float, 7, PD, B, !PD>, sweep_kernel<float, 8, PD, B, !PD>}
 
 const KernelF kSweepF32[2][2][kMaxRowsPerLane] = {
     {FGKL_F32_ROW(false, false), FGKL_F32_ROW(false, true)},

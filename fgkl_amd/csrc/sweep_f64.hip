@@ -1,16 +1,19 @@
-// FP64 instantiations of the sweep kernel, plain and PD: the FP64 mode, the rescue of FP32 pairs
-// and the per-region baseline (sweep.hpp).
+// FP64 instantiations of the sweep kernel: plain and PD without suffix sharing (the FP64 mode
+// without a suffix plan, the rescue of FP32 pairs, the per-region baseline) and plain with it
+// (FP64 main passes of batches with a suffix plan) (sweep.hpp).
 #include "sweep.hpp"
 
 namespace fgkl {
 
-#define FGKL_F64_ROW(PD)                                                                    \
-  {sweep_kernel<double, 1, PD, false>, sweep_kernel<double, 2, PD, false>,                  \
-   sweep_kernel<double, 3, PD, false>, sweep_kernel<double, 4, PD, false>,                  \
-   sweep_kernel<double, 5, PD, false>, sweep_kernel<double, 6, PD, false>,                  \
-   sweep_kernel<double, 7, PD, false>, sweep_kernel<double, 8, PD, false>}
+#define FGKL_F64_ROW(PD, S)                                                                 \
+  {sweep_kernel<double, 1, PD, false, S>, sweep_kernel<double, 2, PD, false, S>,            \
+   sweep_kernel<double, 3, PD, false, S>, sweep_kernel<double, 4, PD, false, S>,            \
+   sweep_kernel<double, 5, PD, false, S>, sweep_kernel<double, 6, PD, false, S>,            \
+   sweep_kernel<double, 7, PD, false, S>, sweep_kernel<double, 8, PD, false, S>}
 
-const KernelD kSweepF64[2][kMaxRowsPerLane] = {FGKL_F64_ROW(false), FGKL_F64_ROW(true)};
+const KernelD kSweepF64[2][kMaxRowsPerLane] = {FGKL_F64_ROW(false, false),
+                                               FGKL_F64_ROW(true, false)};
+const KernelD kSweepF64Suffix[kMaxRowsPerLane] = FGKL_F64_ROW(false, true);
 
 #undef FGKL_F64_ROW
 

@@ -57,6 +57,11 @@ __device__ __forceinline__ uint32_t shl1z(uint32_t src) {
 __device__ __forceinline__ float shl1z(float src) {
   return __builtin_bit_cast(float, int(shl1z(__builtin_bit_cast(uint32_t, src))));
 }
+__device__ __forceinline__ double shl1z(double src) {
+  const uint64_t s = __builtin_bit_cast(uint64_t, src);
+  const uint64_t lo = shl1z(uint32_t(s)), hi = shl1z(uint32_t(s >> 32));
+  return __builtin_bit_cast(double, (hi << 32) | lo);
+}
 
 __device__ __forceinline__ float readlane(float v, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));

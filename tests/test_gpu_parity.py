@@ -977,8 +977,13 @@ def test_suffix_sharing_matches_the_oracle(gpu, wl, regions):
     shared = b.results()
     b.run(F.MODE_FP32_RESCUE, share=False)
     assert_close(b.results(), shared, FLOAT_TOL, "prefix-only vs suffix sharing")
-    b.run(F.MODE_FP64, share=True)  # FP64 mode shares prefixes only
-    assert_rel(b.results(), expected, what="fp64 with a suffix plan")
+    b.run(F.MODE_FP64, share=True)  # FP64 shares prefixes and suffixes
+    fp64_shared = b.results()
+    assert_rel(fp64_shared, expected, what="fp64 with suffix sharing")
+    b.run(F.MODE_FP64, share=False)
+    assert_rel(fp64_shared, b.results(), what="fp64 suffix sharing vs unshared")
+    b.run(F.MODE_FP64, share=True, graph=True)
+    assert np.array_equal(b.results(), fp64_shared)
     # The scratch now holds FP64 records: the FP32 pass must read none it did not write.
     b.run(F.MODE_FP32_RESCUE, share=True)
     assert np.array_equal(b.results(), shared)
@@ -1033,24 +1038,63 @@ def test_suffix_sharing_on_edited_haplotype_sets(gpu):
     nores = b.results()
     ok = np.isfinite(nores)
     assert np.array_equal(nores[ok], got[ok])
+    b.run(F.MODE_FP64, share=True)
+    assert_rel(b.results(), expected, what="edited sets, fp64 suffix sharing")
     b.close()
 
 
-def test_suffix_shared_bits_do_not_depend_on_the_other_regions(gpu):
+@pytest.mark.parametrize("mode", [F.MODE_FP32_RESCUE, F.MODE_FP64])
+def test_suffix_shared_bits_do_not_depend_on_the_other_regions(gpu, mode):
     regions = _edited_sets(3141, 12)
     whole = F.Batch(batch_from_regions(regions))
-    whole.run(F.MODE_FP32_RESCUE, share=True)
+    whole.run(mode, share=True)
     all_bits = whole.results()
     whole.close()
-    base = 0
     for g in (0, 5, 11):
         alone = F.Batch(batch_from_regions([regions[g]]))
-        alone.run(F.MODE_FP32_RESCUE, share=True)
+        alone.run(mode, share=True)
         bits = alone.results()
         alone.close()
         start = sum(len(r) * len(h) for r, h in regions[:g])
         assert np.array_equal(bits, all_bits[start:start + len(bits)])
-        base += len(bits)
+
+
+def test_fp64_joins_below_1e24_are_rerun_without_suffix_sharing(gpu):
+    """simd.rs:136 and lib.rs:467-487: FP64 backward values are unscaled, so a joined FP64 result
+    below 1e24 (a likelihood below about 1e-283 once the 2^1020 scale is divided out) may have
+    lost precision and is recomputed without suffix sharing.  Reads with a substitution every
+    third base at Phred 40 and gap continuation Phred 20 (so that one long insertion costs about
+    as much: log10 L near -360 and -390) against haplotypes that share a suffix are rerun
+    (the run counts them) and match the oracle at the FP64 bound, as do the region's ordinary
+    reads, whose joins are accepted."""
+    rng = Rng(61)
+    base = bytes(b"ACGT"[rng.below(4)] for _ in range(320))
+    haps = [base]
+    for j in (8, 16, 24, 32):  # distinct prefixes, one suffix from column 33 on
+        h = bytearray(base)
+        h[j] = ord("A") if h[j] != ord("A") else ord("C")
+        haps.append(bytes(h))
+    reads = [F.ReadRef(base[s:s + 150], [30] * 150, [45] * 150, [45] * 150, [10] * 150)
+             for s in (0, 120, 160)]
+    for s, n in ((20, 240), (40, 260)):
+        b = bytearray(base[s:s + n])
+        for j in range(0, n, 3):
+            b[j] = ord("A") if b[j] != ord("A") else ord("G")
+        reads.append(F.ReadRef(bytes(b), [40] * n, [45] * n, [45] * n, [20] * n))
+    data = batch_from_regions([(reads, haps)])
+    expected = batch_reference(data)
+    H = len(haps)
+    assert np.all(expected[3 * H:] < -300) and np.all(expected[:3 * H] > -20)
+    bt = F.Batch(data)
+    assert bt.work()["backward"] > 0
+    st = bt.run(F.MODE_FP64, share=True)
+    got = bt.results()
+    assert_rel(got, expected, what="fp64 joins with reruns")
+    assert st.rescued_pairs > 0
+    bt.run(F.MODE_FP64, share=False)
+    unshared = bt.results()
+    assert np.array_equal(got[3 * H:], unshared[3 * H:])  # the reruns are the unshared sweeps
+    bt.close()
 
 
 def test_batch_graph_replays_match_eager_runs(gpu):
