@@ -306,8 +306,9 @@ template <typename T> struct GSum {
   T ge, gmm, gim, gmi, gmd, gg;
 };
 
-// The reverse sweep of one block over the tape.  BM zeroes the adjoint outside columns 1..m.
-template <typename T, bool BM>
+// The reverse sweep of one block over the tape.  BM zeroes the adjoint outside columns 1..m;
+// SEED adds `seedv` (non-zero only in the read's last row), which only the bottom strip holds.
+template <typename T, bool BM, bool SEED>
 __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const GRow<T>& rw,
                                        const Lds<T>& l, Adj2<T>* aout, int t0, int lane, int m,
                                        T seedv, T& AX, T& AY, T& Q, T& Bq, GSum<T>& gs) {
@@ -330,8 +331,12 @@ __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const G
     const T pr = Qd * rw.ig;
     T ay = fma(AY, rw.g, pr);
     const T amp = fma(Qd, rw.mmB, AY * rw.md);
-    T ax = fma(AXb, rw.g, pr) + seedv;
-    T am = fma(AXb, rw.miB, amp) + seedv;
+    T ax = fma(AXb, rw.g, pr);
+    T am = fma(AXb, rw.miB, amp);
+    if constexpr (SEED) {
+      ax += seedv;
+      am += seedv;
+    }
     if constexpr (BM) {
       const bool live = col >= 1 && col <= m;
       ay = live ? ay : T(0);
@@ -356,7 +361,7 @@ __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const G
 }
 
 // Pass B over strip s.  `dgl` collects the lane's gap-continuation terms.
-template <typename T>
+template <typename T, bool SEED>
 __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw, int lane,
                                int s, int v, int n, int m, int L, int nb, T seedv, int64_t o,
                                char* wscr, double& dgl) {
@@ -418,9 +423,9 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
       fblock<T, false, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
     GSum<T> gs{T(0), T(0), T(0), T(0), T(0), T(0)};
     if (t0 < L - 1 || t0 + K > m)
-      rblock<T, true>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
+      rblock<T, true, SEED>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
     else
-      rblock<T, false>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
+      rblock<T, false, SEED>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
     ge += double(gs.ge);
     gi += double(gs.gmi - gs.gmm);
     gd += double(gs.gmd - gs.gmm);
@@ -559,7 +564,11 @@ grad_kernel(GradArgs a) {
         const int nb = (m + L - 1 + K - 1) / K;
         GRow<T> rw;
         load_grow(a, o, n, v, g, rw);
-        backward_strip<T>(a, l, rw, lane, s, v, n, m, L, nb, v == n ? seed : T(0), o, wscr, dgl);
+        if (s == S - 1)  // the read's last row lies in its bottom strip
+          backward_strip<T, true>(a, l, rw, lane, s, v, n, m, L, nb, v == n ? seed : T(0), o,
+                                  wscr, dgl);
+        else
+          backward_strip<T, false>(a, l, rw, lane, s, v, n, m, L, nb, T(0), o, wscr, dgl);
         wave_sync_global();
       }
     }
