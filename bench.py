@@ -716,8 +716,8 @@ def main() -> int:
     fp64_batched = batch.results()
     result["fp64_batched"] = {
         "gcups": round(cells / (st64.total_ms * 1e-3) / 1e9, 2), "ms": round(st64.total_ms, 3),
-        "definition": "FP64 engine, every pair, prefix sharing, one launch per rows-per-lane "
-                      "bucket (HIP events)",
+        "definition": "FP64 engine, every pair, prefix and suffix sharing (joins below 1e24 "
+                      "rerun unshared), one launch per rows-per-lane bucket (HIP events)",
         "max_abs_dlog10L_fp32_rescue_vs_fp64_batched": float(np.abs(fp32_res - fp64_batched).max())}
     result["speedup_vs_fp64_batched"] = round(gcups / result["fp64_batched"]["gcups"], 2)
     log(f"fp64 batched: {st64.total_ms:.1f} ms, {result['fp64_batched']['gcups']:.1f} GCUPS")
