@@ -74,7 +74,9 @@ struct GradArgs {
   const int32_t* region_nhap;
   const int64_t* hap_off;
   const uint8_t* hap_bases;
-  const int32_t* items;      // reads, most work first (or, rescuing, the reads with lost pairs)
+  // Items of two read indices, most work first: (ra, -1) alone, or (ra, rb) sharing a strip
+  // sequence (rescuing: the reads with lost pairs, alone).
+  const int32_t* items;
   int32_t n_items;
   const int32_t* n_items_dev;  // when set, the item count is read from device memory
   int32_t* counter;
@@ -148,6 +150,33 @@ __device__ __forceinline__ St<T> shr_st(const St4<T>& old, const St<T>& src) {
   return St<T>{shr1(old.m, src.m), shr1(old.x, src.x), shr1(old.y, src.y)};
 }
 
+// Two reads in one strip sequence: read B's rows follow read A's, B's first row in the lane below
+// A's last row (lane la + 1 of A's last strip).  That lane takes B's DP row 0, (0, 0, C/m), as
+// the row above, where the DPP move brings A's last row (`keep_f` is zero there, all ones
+// elsewhere), and A's last row takes zero adjoints from below (`keep_r` zero in lane la).  The
+// masks are applied by bitwise ops after the move, so the move runs in every lane.
+__device__ __forceinline__ float and_bits(float v, uint32_t k) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & k);
+}
+__device__ __forceinline__ double and_bits(double v, uint32_t k) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & ((uint64_t(k) << 32) | k));
+}
+__device__ __forceinline__ float sel_bits(uint32_t k, float a, float b) {
+  const uint32_t x = __builtin_bit_cast(uint32_t, a), y = __builtin_bit_cast(uint32_t, b);
+  return __builtin_bit_cast(float, (x & k) | (y & ~k));
+}
+__device__ __forceinline__ double sel_bits(uint32_t k, double a, double b) {
+  const uint64_t m = (uint64_t(k) << 32) | k;
+  const uint64_t x = __builtin_bit_cast(uint64_t, a), y = __builtin_bit_cast(uint64_t, b);
+  return __builtin_bit_cast(double, (x & m) | (y & ~m));
+}
+template <typename T>
+__device__ __forceinline__ void row0_above(St<T>& q, uint32_t keep_f, T init) {
+  q.m = and_bits(q.m, keep_f);
+  q.x = and_bits(q.x, keep_f);
+  q.y = sel_bits(keep_f, q.y, init);
+}
+
 // A wave's LDS: the staged boundary rows and haplotype, and the per-block staging of the values
 // one lane hands to the neighbouring strip.  The lane that owns a boundary writes its staging
 // slot each step; every other lane writes a private dummy slot, so the writes need no branch.
@@ -193,15 +222,18 @@ __device__ __forceinline__ T prior_of(const Lds<T>& l, int c, const GRow<T>& rw)
 // K forward steps from `st` (the state before step t0) with diagonal `P`.  FM freezes lanes
 // that have not reached column 1; ACC adds M + X of columns 1..m; OUT stages lane 63's state.
 // tp[k] receives the state before step k (TAPE).
-template <typename T, bool FM, bool ACC, bool OUT, bool TAPE>
+// MID: the strip holds the first row of a second read (see row0_above).
+template <typename T, bool FM, bool ACC, bool OUT, bool TAPE, bool MID>
 __device__ __forceinline__ void fblock(St<T>& st, St<T>& P, T& acc, St<T>* tp, const GRow<T>& rw,
-                                       const Lds<T>& l, St4<T>* out, int t0, int lane, int m) {
+                                       const Lds<T>& l, St4<T>* out, int t0, int lane, int m,
+                                       uint32_t keep_f, T init) {
   constexpr int K = GT<T>::kTape;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if constexpr (TAPE) tp[k] = st;
     const int c = t0 + k - lane + 1;
-    const St<T> Q = shr_st(l.stage[k + 1], st);
+    St<T> Q = shr_st(l.stage[k + 1], st);
+    if constexpr (MID) row0_above(Q, keep_f, init);
     const T prior = prior_of(l, c, rw);
     const T Y = fma(st.y, rw.g, st.m * rw.md);
     const T M = prior * fma(P.y, rw.ig, fma(P.x, rw.ig, P.m * rw.mm));
@@ -245,9 +277,10 @@ __device__ __forceinline__ void put_above(const Lds<T>& l, const St4<T>& w, int 
 
 // Pass F over strip s: returns the lane's sum of M + X over columns 1..m when ACC (row n's lane
 // is the one that matters).
-template <typename T, bool CK, bool ACC>
+template <typename T, bool CK, bool ACC, bool MID>
 __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw, int lane, int s,
-                           int m, int L, int nb, bool has_next, char* wscr) {
+                           int m, int L, int nb, bool has_next, char* wscr, uint32_t keep_f,
+                           T init) {
   constexpr int K = GT<T>::kTape;
   const T* bnd_in = reinterpret_cast<const T*>(wscr + a.bnd_off) + size_t(s) * a.bnd_cols * 4;
   T* bnd_out = reinterpret_cast<T*>(wscr + a.bnd_off) + size_t(s + 1) * a.bnd_cols * 4;
@@ -267,7 +300,10 @@ __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw
     if constexpr (GT<T>::kPrefetch)
       above = load_above<T>(bnd_in, t0 + K, m, lane);  // the next block's (clamped: in bounds)
     wave_sync_lds();
-    if (b == 0) P = shr_st(l.stage[0], st);  // the diagonal of step 0: column 0 of the row above
+    if (b == 0) {  // the diagonal of step 0: column 0 of the row above
+      P = shr_st(l.stage[0], st);
+      if constexpr (MID) row0_above(P, keep_f, init);
+    }
     if constexpr (CK) {
       T* c = ck + size_t(b) * 6 * 64 + lane;
       c[0 * 64] = st.m;
@@ -280,9 +316,11 @@ __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw
     const bool fm = t0 < L - 1;
     if (has_next) {
       if (fm)
-        fblock<T, true, ACC, true, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, true, ACC, true, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
+                                               keep_f, init);
       else
-        fblock<T, false, ACC, true, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, false, ACC, true, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
+                                                keep_f, init);
       wave_sync_lds();
       // Lanes 0..K-1 store lane 63's columns t0+lane-62 that lie in 1..m.
       const int c = t0 + lane - 62;
@@ -292,9 +330,11 @@ __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw
       *dst = St4<T>{w.m, w.x, w.y, T(0)};
     } else {
       if (fm)
-        fblock<T, true, ACC, false, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, true, ACC, false, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
+                                                keep_f, init);
       else
-        fblock<T, false, ACC, false, false>(st, P, acc, nullptr, rw, l, out, t0, lane, m);
+        fblock<T, false, ACC, false, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
+                                                 keep_f, init);
     }
   }
   return acc;
@@ -308,12 +348,14 @@ template <typename T> struct GSum {
 
 // The reverse sweep of one block over the tape.  BM zeroes the adjoint outside columns 1..m;
 // SEED adds `seedv` (non-zero only in the read's last row), which only the bottom strip holds.
-template <typename T, bool BM, bool SEED>
+template <typename T, bool BM, bool SEED, bool MID>
 __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const GRow<T>& rw,
                                        const Lds<T>& l, Adj2<T>* aout, int t0, int lane, int m,
-                                       T seedv, T& AX, T& AY, T& Q, T& Bq, GSum<T>& gs) {
+                                       T seedv, T& AX, T& AY, T& Q, T& Bq, GSum<T>& gs,
+                                       uint32_t keep_f, uint32_t keep_r, T init) {
   constexpr int K = GT<T>::kTape;
   St<T> U = shr_st(l.stage[K], tp[K - 1]);  // up values of the last step
+  if constexpr (MID) row0_above(U, keep_f, init);
 #pragma unroll
   for (int k = K - 1; k >= 0; --k) {
     const int col = t0 + k - lane + 1;
@@ -322,11 +364,18 @@ __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const G
     const T ds = mt ? T(-1) : T(1.0 / 3.0);  // d prior / d e
     // The row below: its insertion adjoint in this column and Q one column to the right.
     const Adj2<T> below = l.astage[k];
-    const T AXb = shl1(below.ax, AX);
-    const T Qn = shl1(below.q, Q);
+    T AXb = shl1(below.ax, AX);
+    T Qn = shl1(below.q, Q);
+    if constexpr (MID) {
+      AXb = and_bits(AXb, keep_r);
+      Qn = and_bits(Qn, keep_r);
+    }
     const T Qd = Bq;
     Bq = Qn;
-    const St<T> D = k ? shr_st(l.stage[k], tp[k - 1]) : P0;
+    St<T> D = k ? shr_st(l.stage[k], tp[k - 1]) : P0;
+    if constexpr (MID) {
+      if (k) row0_above(D, keep_f, init);
+    }
     const St<T>& Lf = tp[k];
     const T pr = Qd * rw.ig;
     T ay = fma(AY, rw.g, pr);
@@ -360,11 +409,13 @@ __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const G
   }
 }
 
-// Pass B over strip s.  `dgl` collects the lane's gap-continuation terms.
-template <typename T, bool SEED>
+// Pass B over strip s.  The lane's row has gradient index `gidx` (-1: no row); `dgl_a` and
+// `dgl_b` collect its gap-continuation terms for the first and the second read of the item.
+template <typename T, bool SEED, bool MID>
 __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw, int lane,
-                               int s, int v, int n, int m, int L, int nb, T seedv, int64_t o,
-                               char* wscr, double& dgl) {
+                               int s, int m, int L, int nb, T seedv, int64_t gidx, bool in_b,
+                               char* wscr, double& dgl_a, double& dgl_b, uint32_t keep_f,
+                               uint32_t keep_r, T init) {
   constexpr int K = GT<T>::kTape;
   const T* bnd_in = reinterpret_cast<const T*>(wscr + a.bnd_off) + size_t(s) * a.bnd_cols * 4;
   const T* ck = reinterpret_cast<const T*>(wscr + a.ck_off) + size_t(s) * a.nb_max * 6 * 64;
@@ -418,14 +469,18 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
     St<T> P = P0;
     T unused = T(0);
     if (t0 < L - 1)
-      fblock<T, true, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
+      fblock<T, true, false, false, true, MID>(st, P, unused, tp, rw, l, nullptr, t0, lane, m,
+                                               keep_f, init);
     else
-      fblock<T, false, false, false, true>(st, P, unused, tp, rw, l, nullptr, t0, lane, m);
+      fblock<T, false, false, false, true, MID>(st, P, unused, tp, rw, l, nullptr, t0, lane, m,
+                                                keep_f, init);
     GSum<T> gs{T(0), T(0), T(0), T(0), T(0), T(0)};
     if (t0 < L - 1 || t0 + K > m)
-      rblock<T, true, SEED>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
+      rblock<T, true, SEED, MID>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs,
+                                 keep_f, keep_r, init);
     else
-      rblock<T, false, SEED>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs);
+      rblock<T, false, SEED, MID>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs,
+                                  keep_f, keep_r, init);
     ge += double(gs.ge);
     gi += double(gs.gmi - gs.gmm);
     gd += double(gs.gmd - gs.gmm);
@@ -439,15 +494,16 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
       *dst = w;
     }
   }
-  const bool real = v <= n;
-  const int64_t i = o + (real ? v : n) - 1;
+  const bool real = gidx >= 0;
+  const int64_t i = real ? gidx : 0;
   double* pe = real ? a.g_err + i : reinterpret_cast<double*>(gdummy);
   double* pi = real ? a.g_ins + i : reinterpret_cast<double*>(gdummy) + 1;
   double* pd = real ? a.g_del + i : reinterpret_cast<double*>(gdummy) + 2;
   *pe += ge * kInvLn10;
   *pi += gi * kInvLn10;  // M->M = 1 - d_ins - d_del
   *pd += gd * kInvLn10;
-  dgl += real ? gq : 0.0;  // I->M = D->M = 1 - g
+  dgl_a += real && !in_b ? gq : 0.0;  // I->M = D->M = 1 - g
+  dgl_b += real && in_b ? gq : 0.0;
 }
 
 // FP32: three waves per SIMD (at most 168 VGPRs) measured faster than two without spills.
@@ -472,28 +528,55 @@ grad_kernel(GradArgs a) {
     // in [64 i, 64 i + 63]; no lane-dependent branch reaches the readfirstlane.
     const int item = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, 1)) >> 6;
     if (item >= n_items) break;
-    const int r = __builtin_amdgcn_readfirstlane(a.items[item]);
-    const int64_t o = a.read_off[r];
-    const int n = __builtin_amdgcn_readfirstlane(int(a.read_off[r + 1] - o));
-    const int gi = __builtin_amdgcn_readfirstlane(a.read_region[r]);
+    // An item is one read, or two reads of one region (ra, then rb) in one sequence of strips:
+    // rb's rows follow ra's, its first row in the lane below ra's last row, inside ra's last strip.
+    // FP64 runs single reads only: its registers have no room for the shared strip's code.
+    const int ra = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
+    const int rb = kFp32 ? __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]) : -1;
+    const bool two = kFp32 && rb >= 0;
+    const int64_t oa = a.read_off[ra];
+    const int na = __builtin_amdgcn_readfirstlane(int(a.read_off[ra + 1] - oa));
+    const int64_t ob = two ? a.read_off[rb] : 0;
+    const int nr_b = two ? __builtin_amdgcn_readfirstlane(int(a.read_off[rb + 1] - ob)) : 0;
+    const int N = na + nr_b;  // rows of the item
+    const int gi = __builtin_amdgcn_readfirstlane(a.read_region[ra]);
     const int hap0 = __builtin_amdgcn_readfirstlane(a.region_hap0[gi]);
     const int H = __builtin_amdgcn_readfirstlane(a.region_nhap[gi]);
-    const int64_t out_base = a.region_out[gi] + int64_t(a.read_local[r]) * H;
-    const int S = (n + 63) / 64;  // strips of 64 read rows
+    const int64_t base_a = a.region_out[gi] + int64_t(a.read_local[ra]) * H;
+    const int64_t base_b = two ? a.region_out[gi] + int64_t(a.read_local[rb]) * H : 0;
+    const int S = (N + 63) / 64;   // strips of 64 rows
+    const int sa = (na - 1) / 64;  // the strip of ra's last row (and of rb's first)
+    const int la = (na - 1) % 64;  // ra's last row's lane there
+    const int lb = (N - 1) % 64;   // rb's last row's lane in strip S - 1
+    uint32_t keep_f = two && lane == la + 1 ? 0u : ~0u;
+    uint32_t keep_r = two && lane == la ? 0u : ~0u;
+    asm volatile("" : "+v"(keep_f), "+v"(keep_r));
+    // The lane's row in strip s: its read's offset, length and row number (1-based).
+    struct RowOf {
+      int64_t o;
+      int n, v;
+      bool in_b;
+    };
+    auto row_of = [&](int s) {
+      const int v = 64 * s + lane + 1;
+      const bool in_b = two && v > na;
+      return RowOf{in_b ? ob : oa, in_b ? nr_b : na, in_b ? v - na : v, in_b};
+    };
     if (BWD && !a.rescue) {  // a rescue adds to what the FP32 pass wrote
       for (int s = 0; s < S; ++s) {
-        const int v = 64 * s + lane + 1;
-        const bool real = v <= n;
-        const int64_t i = o + (real ? v : n) - 1;
+        const RowOf w = row_of(s);
+        const bool real = w.v <= w.n;
+        const int64_t i = w.o + (real ? w.v : w.n) - 1;
         *(real ? a.g_err + i : reinterpret_cast<double*>(gslot)) = 0.0;
         *(real ? a.g_ins + i : reinterpret_cast<double*>(gslot) + 1) = 0.0;
         *(real ? a.g_del + i : reinterpret_cast<double*>(gslot) + 2) = 0.0;
       }
     }
-    double dgl = 0.0;
-    int any_lost = 0;
+    double dgl_a = 0.0, dgl_b = 0.0;
+    int any_lost_a = 0, any_lost_b = 0;
     for (int h = 0; h < H; ++h) {
-      if (a.rescue && !__builtin_amdgcn_readfirstlane(int(a.lost[out_base + h]))) continue;
+      // Rescue items are single reads.
+      if (a.rescue && !__builtin_amdgcn_readfirstlane(int(a.lost[base_a + h]))) continue;
       const int64_t hoff = a.hap_off[hap0 + h];
       const int m = __builtin_amdgcn_readfirstlane(int(a.hap_off[hap0 + h + 1] - hoff));
       wave_sync_lds();
@@ -506,7 +589,7 @@ grad_kernel(GradArgs a) {
         *(i < hap_total ? &l.hap[i] : l.dummy + lane) = in ? hb : uint8_t(0);
       }
       // The row above strip 0 is DP row 0: no match or insertion mass, deletions C / m in every
-      // column (reference.rs:27-28).
+      // column (reference.rs:27-28).  rb's DP row 0 is the same row, given by row0_above.
       const T init = T(GT<T>::kInit / double(m));
       T* bnd0 = reinterpret_cast<T*>(wscr + a.bnd_off);
       for (int c0 = 0; c0 <= m; c0 += 64) {
@@ -517,35 +600,59 @@ grad_kernel(GradArgs a) {
       }
       wave_sync_lds();
       wave_sync_global();
-      const double w = BWD ? (a.weights ? a.weights[out_base + h] : 1.0) : 0.0;
-      const bool grad = BWD && w != 0.0;
-      // Pass F.
-      double total = 0.0;
+      const double wa = BWD ? (a.weights ? a.weights[base_a + h] : 1.0) : 0.0;
+      const double wb = BWD && two ? (a.weights ? a.weights[base_b + h] : 1.0) : 0.0;
+      const bool grad = BWD && (wa != 0.0 || wb != 0.0);
+      // Pass F.  The strips holding a read's last row sum its M + X (ACC).
+      double total_a = 0.0, total_b = 0.0;
       for (int s = 0; s < S; ++s) {
-        const int v = 64 * s + lane + 1;
-        const int L = min(64, n - 64 * s);
+        const RowOf w = row_of(s);
+        const int L = min(64, N - 64 * s);
         const int nb = (m + L - 1 + K - 1) / K;
         GRow<T> rw;
-        load_grow(a, o, n, v, g, rw);
-        if (s + 1 < S) {
-          if (grad)
-            forward_strip<T, true, false>(a, l, rw, lane, s, m, L, nb, true, wscr);
-          else
-            forward_strip<T, false, false>(a, l, rw, lane, s, m, L, nb, true, wscr);
+        load_grow(a, w.o, w.n, w.v, g, rw);
+        const bool next = s + 1 < S;
+        T acc = T(0);
+        if (two && s == sa) {  // the strip where rb begins; it also ends ra
+          acc = grad ? forward_strip<T, true, true, true>(a, l, rw, lane, s, m, L, nb, next, wscr,
+                                                         keep_f, init)
+                     : forward_strip<T, false, true, true>(a, l, rw, lane, s, m, L, nb, next,
+                                                          wscr, keep_f, init);
+        } else if (s == sa || s == S - 1) {
+          acc = grad ? forward_strip<T, true, true, false>(a, l, rw, lane, s, m, L, nb, next,
+                                                          wscr, keep_f, init)
+                     : forward_strip<T, false, true, false>(a, l, rw, lane, s, m, L, nb, next,
+                                                           wscr, keep_f, init);
+        } else if (grad) {
+          forward_strip<T, true, false, false>(a, l, rw, lane, s, m, L, nb, next, wscr, keep_f,
+                                               init);
         } else {
-          const T acc = grad ? forward_strip<T, true, true>(a, l, rw, lane, s, m, L, nb, false, wscr)
-                             : forward_strip<T, false, true>(a, l, rw, lane, s, m, L, nb, false, wscr);
-          total = double(readlane(acc, n - 1 - 64 * s));
+          forward_strip<T, false, false, false>(a, l, rw, lane, s, m, L, nb, next, wscr, keep_f,
+                                                init);
         }
+        if (s == sa) total_a = double(readlane(acc, la));
+        if (two && s == S - 1) total_b = double(readlane(acc, lb));
         wave_sync_global();
       }
       // FP32: a pair below the rescue threshold (or NaN) is left to the FP64 pass.
-      const bool lost = kFp32 && !(total >= GT<T>::kMinAccepted);
-      if (kFp32 && lane == 0) a.lost[out_base + h] = lost ? 1 : 0;
-      any_lost |= lost;
-      if (a.log10_out && lane == 0)
-        a.log10_out[out_base + h] = lost ? __builtin_nan("") : log10(total) - GT<T>::kLog10Init;
-      if (lost || !grad || !(total > 0.0) || !(total <= 1.7976931348623157e308)) continue;
+      const bool lost_a = kFp32 && !(total_a >= GT<T>::kMinAccepted);
+      const bool lost_b = kFp32 && two && !(total_b >= GT<T>::kMinAccepted);
+      if (kFp32 && lane == 0) {
+        a.lost[base_a + h] = lost_a ? 1 : 0;
+        if (two) a.lost[base_b + h] = lost_b ? 1 : 0;
+      }
+      any_lost_a |= lost_a;
+      any_lost_b |= lost_b;
+      if (a.log10_out && lane == 0) {
+        a.log10_out[base_a + h] = lost_a ? __builtin_nan("") : log10(total_a) - GT<T>::kLog10Init;
+        if (two)
+          a.log10_out[base_b + h] =
+              lost_b ? __builtin_nan("") : log10(total_b) - GT<T>::kLog10Init;
+      }
+      auto usable = [](double t) { return t > 0.0 && t <= 1.7976931348623157e308; };
+      const bool ga = grad && wa != 0.0 && !lost_a && usable(total_a);
+      const bool gb = grad && two && wb != 0.0 && !lost_b && usable(total_b);
+      if (!ga && !gb) continue;
       // Pass B: the bottom strip has nothing below it.
       {
         const int abnd_cols = a.bnd_cols + 1;
@@ -556,39 +663,58 @@ grad_kernel(GradArgs a) {
         }
         wave_sync_global();
       }
-      // Every adjoint is w d ln L / d cell: seeded with w / L at the last row.
-      const T seed = T(w / total);
+      // Every adjoint is w d ln L / d cell: seeded with w / L at each read's last row (zero for
+      // a read that contributes nothing, whose rows then take zero adjoints).
+      const T seed_a = ga ? T(wa / total_a) : T(0);
+      const T seed_b = gb ? T(wb / total_b) : T(0);
       for (int s = S - 1; s >= 0; --s) {
-        const int v = 64 * s + lane + 1;
-        const int L = min(64, n - 64 * s);
+        const RowOf w = row_of(s);
+        const int L = min(64, N - 64 * s);
         const int nb = (m + L - 1 + K - 1) / K;
         GRow<T> rw;
-        load_grow(a, o, n, v, g, rw);
-        if (s == S - 1)  // the read's last row lies in its bottom strip
-          backward_strip<T, true>(a, l, rw, lane, s, v, n, m, L, nb, v == n ? seed : T(0), o,
-                                  wscr, dgl);
+        load_grow(a, w.o, w.n, w.v, g, rw);
+        const int64_t gidx = w.v <= w.n ? w.o + w.v - 1 : -1;
+        const T seedv = !w.in_b && w.v == na ? seed_a : (w.in_b && w.v == nr_b ? seed_b : T(0));
+        if (two && s == sa)
+          backward_strip<T, true, true>(a, l, rw, lane, s, m, L, nb, seedv, gidx, w.in_b, wscr,
+                                        dgl_a, dgl_b, keep_f, keep_r, init);
+        else if (s == sa || s == S - 1)  // a read's last row lies in this strip
+          backward_strip<T, true, false>(a, l, rw, lane, s, m, L, nb, seedv, gidx, w.in_b, wscr,
+                                         dgl_a, dgl_b, keep_f, keep_r, init);
         else
-          backward_strip<T, false>(a, l, rw, lane, s, v, n, m, L, nb, T(0), o, wscr, dgl);
+          backward_strip<T, false, false>(a, l, rw, lane, s, m, L, nb, T(0), gidx, w.in_b,
+                                          wscr, dgl_a, dgl_b, keep_f, keep_r, init);
         wave_sync_global();
       }
     }
-    if (kFp32 && any_lost && lane == 0) {
-      const int idx = atomicAdd(a.lost_count, 1);
-      a.lost_items[idx] = r;
+    if (kFp32 && lane == 0) {
+      if (any_lost_a) {
+        const int idx = atomicAdd(a.lost_count, 1);
+        a.lost_items[2 * idx] = ra;
+        a.lost_items[2 * idx + 1] = -1;
+      }
+      if (any_lost_b) {
+        const int idx = atomicAdd(a.lost_count, 1);
+        a.lost_items[2 * idx] = rb;
+        a.lost_items[2 * idx + 1] = -1;
+      }
     }
     if constexpr (BWD) {
       // Sum over lanes in lane order through LDS; every lane stores the same value.
       double* red = reinterpret_cast<double*>(l.dummy);
-      wave_sync_lds();
-      red[lane] = dgl;
-      wave_sync_lds();
-      double tot = 0.0;
+      for (int q = 0; q < (two ? 2 : 1); ++q) {
+        wave_sync_lds();
+        red[lane] = q ? dgl_b : dgl_a;
+        wave_sync_lds();
+        double tot = 0.0;
 #pragma unroll 4
-      for (int i = 0; i < 64; ++i) tot += red[i];
-      if (a.rescue)
-        a.dg_part[r] += tot * kInvLn10;
-      else
-        a.dg_part[r] = tot * kInvLn10;
+        for (int i = 0; i < 64; ++i) tot += red[i];
+        const int r = q ? rb : ra;
+        if (a.rescue)
+          a.dg_part[r] += tot * kInvLn10;
+        else
+          a.dg_part[r] = tot * kInvLn10;
+      }
     }
   }
 }
@@ -643,12 +769,14 @@ struct GradBatch::Impl {
   int32_t *region_hap0 = nullptr, *region_nhap = nullptr;
   int64_t* hap_off = nullptr;
   uint8_t* hap_bases = nullptr;
-  int32_t* items = nullptr;
+  int32_t* items = nullptr;         // FP32: reads alone or in pairs
+  int32_t* items_single = nullptr;  // FP64: reads alone
+  int32_t n_single = 0;
   // Work-queue heads and counts: 0 forward, 1 backward, 2 reads with lost pairs, 3 the rescue.
   int32_t* counters = nullptr;
   double* dg_part = nullptr;
   uint8_t* lost = nullptr;        // per pair (FP32 forward)
-  int32_t* lost_items = nullptr;  // reads with lost pairs
+  int32_t* lost_items = nullptr;  // reads with lost pairs, as items (read, -1)
   char* scratch = nullptr;
   int64_t scratch_waves = 0;
   GradArgs proto[2] = {};         // [fp64]: precision-specific scratch and LDS layout
@@ -728,6 +856,8 @@ struct GradBatch::Impl {
     if (base.n_items <= 0) return kOk;
     if (!fp32) {
       GradArgs a = base;
+      a.items = items_single;
+      a.n_items = n_single;
       a.counter = counters + (bwd ? 1 : 0);
       return launch_one<double>(bwd, a, a.n_items);
     }
@@ -836,28 +966,77 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
   const size_t nr = static_cast<size_t>(NR), ng = static_cast<size_t>(G);
   std::vector<int32_t> read_region(nr), read_local(nr), hap0(ng), nhap(ng);
   std::vector<int64_t> region_out(ng + 1, 0);
-  std::vector<int64_t> cost(nr);
-  int max_n = 0;
+  // Work items: a read alone, or two reads of a region whose last, partial strips fit in one
+  // (both lengths not multiples of 64, their remainders summing to at most 64), which then run
+  // one strip fewer.  Per region the reads go by remainder, largest first, each with the unpaired
+  // read of the largest remainder that fits.  Costliest items first.
+  std::vector<int32_t> items, singles;
+  std::vector<int64_t> item_cost, single_cost;
+  items.reserve(2 * nr);
+  singles.reserve(2 * nr);
+  int S_max = 1;
   for (int64_t g = 0; g < G; ++g) {
     const int64_t R = in.region_read[g + 1] - in.region_read[g];
     const int64_t H = in.region_hap[g + 1] - in.region_hap[g];
     hap0[size_t(g)] = int32_t(in.region_hap[g]);
     nhap[size_t(g)] = int32_t(H);
     region_out[size_t(g + 1)] = region_out[size_t(g)] + R * H;
-    const int64_t hap_sum = in.hap_off[in.region_hap[g + 1]] - in.hap_off[in.region_hap[g]];
+    const int64_t hap_steps =
+        in.hap_off[in.region_hap[g + 1]] - in.hap_off[in.region_hap[g]] + 63 * H;
+    auto len = [&](int64_t r) { return int(in.read_off[r + 1] - in.read_off[r]); };
+    std::vector<std::pair<int, int32_t>> by_rest;  // (length mod 64, read)
     for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r) {
       read_region[size_t(r)] = int32_t(g);
       read_local[size_t(r)] = int32_t(r - in.region_read[g]);
-      const int64_t n = in.read_off[r + 1] - in.read_off[r];
-      max_n = std::max(max_n, int(n));
-      cost[size_t(r)] = ((n + 64) / 64) * (hap_sum + 64 * H);
+      by_rest.emplace_back(len(r) % 64, int32_t(r));
+      const int S = (len(r) + 63) / 64;
+      S_max = std::max(S_max, S);
+      singles.push_back(int32_t(r));
+      singles.push_back(-1);
+      single_cost.push_back(int64_t(S) * hap_steps);
+    }
+    std::stable_sort(by_rest.begin(), by_rest.end(),
+                     [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) {
+                       return x.first > y.first;
+                     });
+    std::vector<char> used(by_rest.size(), 0);
+    for (size_t i = 0; i < by_rest.size(); ++i) {
+      if (used[i]) continue;
+      used[i] = 1;
+      const int32_t ra = by_rest[i].second;
+      int32_t rb = -1;
+      if (by_rest[i].first > 0)
+        for (size_t j = i + 1; j < by_rest.size(); ++j)
+          if (!used[j] && by_rest[j].first > 0 && by_rest[i].first + by_rest[j].first <= 64) {
+            used[j] = 1;
+            rb = by_rest[j].second;
+            break;
+          }
+      const int rows = len(ra) + (rb >= 0 ? len(rb) : 0);
+      const int S = (rows + 63) / 64;
+      S_max = std::max(S_max, S);
+      items.push_back(ra);
+      items.push_back(rb);
+      item_cost.push_back(int64_t(S) * hap_steps);
     }
   }
   d.n_pairs = region_out[size_t(G)];
-  std::vector<int32_t> items(nr);
-  for (int64_t r = 0; r < NR; ++r) items[size_t(r)] = int32_t(r);
-  std::stable_sort(items.begin(), items.end(),
-                   [&](int32_t x, int32_t y) { return cost[size_t(x)] > cost[size_t(y)]; });
+  auto by_cost = [](std::vector<int32_t>& list, const std::vector<int64_t>& cost) {
+    const size_t n = cost.size();
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<int32_t> sorted(list.size());
+    for (size_t i = 0; i < n; ++i) {
+      sorted[2 * i] = list[2 * order[i]];
+      sorted[2 * i + 1] = list[2 * order[i] + 1];
+    }
+    list.swap(sorted);
+  };
+  by_cost(items, item_cost);
+  by_cost(singles, single_cost);
+  const int64_t n_items = int64_t(item_cost.size());
   const int64_t nb = NR ? in.hap_off[NH] : 0;
   int rc = 0;
   GRAD_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
@@ -868,7 +1047,7 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
       (rc = d.alloc(&d.d_w, npair)) || (rc = d.alloc(&d.d_log10, npair)) ||
       (rc = d.alloc(&d.d_ge, nrb)) || (rc = d.alloc(&d.d_gi, nrb)) ||
       (rc = d.alloc(&d.d_gd, nrb)) || (rc = d.alloc(&d.d_gg, 1)) ||
-      (rc = d.alloc(&d.lost, npair)) || (rc = d.alloc(&d.lost_items, nr)))
+      (rc = d.alloc(&d.lost, npair)) || (rc = d.alloc(&d.lost_items, 2 * nr)))
     return rc;
   if ((rc = d.upload(&d.bases, in.bases, size_t(d.n_read_bases))) ||
       (rc = d.upload(&d.read_off, in.read_off, size_t(NR + 1))) ||
@@ -879,11 +1058,11 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
       (rc = d.upload(&d.region_nhap, nhap.data(), size_t(G))) ||
       (rc = d.upload(&d.hap_off, in.hap_off, size_t(NH + 1))) ||
       (rc = d.upload(&d.hap_bases, in.hap_bases, size_t(nb))) ||
-      (rc = d.upload(&d.items, items.data(), size_t(NR))) ||
+      (rc = d.upload(&d.items, items.data(), items.size())) ||
+      (rc = d.upload(&d.items_single, singles.data(), singles.size())) ||
       (rc = d.alloc(&d.counters, 4)) || (rc = d.alloc(&d.dg_part, size_t(NR))))
     return rc;
   // Per-wave LDS and scratch, per precision; both passes share the larger per-wave stride.
-  const int S_max = (max_n + 64) / 64;
   layout<float>(d.proto[0], S_max, max_m);
   layout<double>(d.proto[1], S_max, max_m);
   for (const GradArgs& a : d.proto)
@@ -906,6 +1085,7 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
   int occ_max = 0;
   for (auto& row : d.occ)
     for (int o : row) occ_max = std::max(occ_max, o);
+  d.n_single = int32_t(NR);
   const int64_t want_waves = std::max<int64_t>((NR + kWaves - 1) / kWaves * kWaves, kWaves);
   d.scratch_waves = std::min<int64_t>(int64_t(occ_max) * d.num_cus * kWaves, want_waves);
   if ((rc = d.alloc(&d.scratch, size_t(d.scratch_waves * stride)))) return rc;
@@ -921,7 +1101,7 @@ int GradBatch::create(const GradInput& in, std::unique_ptr<GradBatch>& out) {
     a.hap_off = d.hap_off;
     a.hap_bases = d.hap_bases;
     a.items = d.items;
-    a.n_items = int32_t(NR);
+    a.n_items = int32_t(n_items);
     a.dg_part = d.dg_part;
     a.scratch = d.scratch;
     a.err = d.d_err;
