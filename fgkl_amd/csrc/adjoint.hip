@@ -20,7 +20,7 @@
 //    0).  Gradient terms are summed per block in the working precision and accumulated in FP64
 //    registers per row, written once per strip.
 //  * Two precisions.  FP64: C = 2^1020, K = 8, every pair.  FP32 (the forward, the tape and the
-//    adjoint): C = 2^120, K = 16, the adjoint seeded with w / L so that it is d ln L / d cell
+//    adjoint): C = 2^120, K = 20, the adjoint seeded with w / L so that it is d ln L / d cell
 //    (its values then stay within FP32's normal range from the last row to the first); a pair
 //    whose FP32 likelihood falls below 1e-28 (kernel.rs:897-914's rescue threshold) is flagged,
 //    and the FP64 kernel reruns the flagged pairs of each read and adds their terms.
@@ -55,7 +55,7 @@ template <> struct GT<float> {
   static constexpr double kInit = 0x1p120;
   static constexpr double kLog10Init = 120.0 * 0.301029995663981195213738894724493027;
   static constexpr double kMinAccepted = 1e-28;
-  static constexpr int kTape = 16;
+  static constexpr int kTape = 20;  // about sqrt(m) for haplotypes of a few hundred bases
   static constexpr bool kPrefetch = true;   // the next block's inputs load during this block
 };
 
