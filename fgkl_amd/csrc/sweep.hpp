@@ -245,7 +245,10 @@ __device__ __forceinline__ double max_nonneg(double a, double b) { return fmax(a
 // The R cells of one column.  Rows are updated in place: match, deletion and branch values
 // bottom-up (each row still reads the old column of the row above), then insertion values
 // top-down (each row reads the new column of the row above), so no value is copied.
-template <typename T, int R, bool PD>
+// UNI (PD): every row of the lane reads equal after-deletion bits, or holds zero branch values
+// (identity rows and lanes past the read), so a row's left maxima are the diagonal maxima the
+// row below it has just formed from the same old values: they are reused, not recomputed.
+template <typename T, int R, bool PD, bool UNI = false>
 __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
                                       const Cell<T>& d, const Cell<T>& u) {
   const uint32_t w = s.w;
@@ -257,6 +260,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
   // included).  All values are non-negative, so max(v, b & after) is v outside AFTER_DEL and
   // max(v, b) inside it, and the states need no compares or selects.
   const uint32_t de = PD ? bit_mask(w, kDelEndBit) : 0u;
+  T nlme = T(0), nlxe = T(0), nlye = T(0);  // UNI: the next row up's left maxima
 #pragma unroll
   for (int k = R - 1; k >= 0; --k) {
     const T pdm = k ? s.m[k - 1] : d.m;
@@ -282,9 +286,19 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       const T dme = max_nonneg(pdm, and_bits(pdbm, A));
       const T dxe = max_nonneg(pdx, and_bits(pdbx, A));
       const T dye = max_nonneg(pdy, and_bits(pdby, A));
-      const T lme = max_nonneg(s.m[k], and_bits(s.bm[k], A));
-      const T lxe = max_nonneg(s.x[k], and_bits(s.bx[k], A));
-      const T lye = max_nonneg(s.y[k], and_bits(s.by[k], A));
+      T lme, lxe, lye;
+      if (!UNI || k == R - 1) {
+        lme = max_nonneg(s.m[k], and_bits(s.bm[k], A));
+        lxe = max_nonneg(s.x[k], and_bits(s.bx[k], A));
+        lye = max_nonneg(s.y[k], and_bits(s.by[k], A));
+      } else {
+        lme = nlme;
+        lxe = nlxe;
+        lye = nlye;
+      }
+      nlme = dme;  // row k-1's left maxima (its old values under equal bits)
+      nlxe = dxe;
+      nlye = dye;
       const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
       // NORMAL copies the row's values (lme is m there), AFTER_DEL takes the maxima, INSIDE_DEL
       // holds the branch.
@@ -368,7 +382,7 @@ __device__ __forceinline__ uint32_t next_word(uint32_t from_lds, uint32_t w, uin
 // once it has computed column W, so it ends holding its rows of column W (a cut haplotype joins
 // from there) and s.acc of a read's last lane is the raw scaled sum over the read's last row.
 // `span` is the lane count of the wave's longest read.
-template <typename T, int R, bool PD, bool SNAP>
+template <typename T, int R, bool PD, bool SNAP, bool UNI = false>
 __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int span, int L,
                       int pos, int lane, uint32_t head, T init, const T* snap_in, T scale,
                       T* scratch, bool row0_above, uint32_t keep, Lane<T, R, PD>& s) {
@@ -434,7 +448,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells(s, rw, P, Q);
+      cells<T, R, PD, UNI>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -447,26 +461,26 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
     s.w = next_word(cq.x, s.w, head);
     shift_up(s, Q, keep);
-    cells(s, rw, P, Q);
+    cells<T, R, PD, UNI>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.y, s.w, head);
     shift_up(s, P, keep);
-    cells(s, rw, Q, P);
+    cells<T, R, PD, UNI>(s, rw, Q, P);
     record(P);
     s.w = next_word(cq.z, s.w, head);
     shift_up(s, Q, keep);
-    cells(s, rw, P, Q);
+    cells<T, R, PD, UNI>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.w, s.w, head);
     shift_up(s, P, keep);
-    cells(s, rw, Q, P);
+    cells<T, R, PD, UNI>(s, rw, Q, P);
     record(P);
   }
   for (; t < steps; ++t) {
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells(s, rw, P, Q);
+      cells<T, R, PD, UNI>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -816,12 +830,23 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         const int W = (cut ? p.cut - 1 : p.len) - start;
         stage_hap<PD>(a, p, start, W, L, lane, lw, nsnap);
         Lane<T, R, PD> st;
-        if (nsnap)
-          sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                scratch, row0_above, keep, st);
-        else
-          sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                 scratch, row0_above, keep, st);
+        // PD haplotypes whose rows read equal state bits take the sweep that shares maxima.
+        const bool uni = PD && __builtin_amdgcn_readfirstlane(p.uni) != 0;
+        if (nsnap) {
+          if (uni)
+            sweep<T, R, PD, true, PD>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                      scratch, row0_above, keep, st);
+          else
+            sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                  scratch, row0_above, keep, st);
+        } else {
+          if (uni)
+            sweep<T, R, PD, false, PD>(rw, lw, W, span, L, pos, lane, head, init, snap_in,
+                                       scale, scratch, row0_above, keep, st);
+          else
+            sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                   scratch, row0_above, keep, st);
+        }
         // A read's last row is its last lane's last register (kernel.rs:897-914).
         // A joined pair is held to kMinAcceptedJoined (kernel.rs:261-275).
         auto finish = [cut](double raw) {

@@ -425,6 +425,53 @@ def test_pd_carried_deletion_state_keeps_later_rows_from_skipping(gpu):
     assert_close(out, expected, FLOAT_TOL, "pd carry float")
 
 
+@pytest.mark.parametrize("n", [70, 130, 200, 333, 480])
+def test_pd_rows_sharing_maxima_match_the_reference(gpu, n):
+    """A PD haplotype whose deletion state carried out of a row is NORMAL (read row 1 reads the
+    same state bits as the rows below it, pd_reference.rs:46) takes the sweep in which each row
+    reuses, as its left maxima, the diagonal maxima the row below formed from the same values;
+    a haplotype ending inside or just after a deletion takes the full sweep.  Reads of 2 to 8 rows
+    per lane over both kinds of haplotype, batched (prefix sharing, two reads to a wave) and per
+    call, in both precisions, against the PD oracle."""
+    rng = Rng(300 + n)
+    m = n + 120
+    base = bytes(rng.base() for _ in range(m))
+
+    def flags(kind):
+        f = [0] * m
+        for j in range(7, m - 14, 23):  # alternates and deletion runs inside the haplotype
+            f[j] = F.SNP | F.ALT_G
+            f[j + 4] = F.DEL_START
+            f[j + 5 + rng.below(5)] |= F.DEL_END
+        if kind == 1:
+            f[m - 3] = F.DEL_START
+            f[m - 1] = F.DEL_END    # AFTER_DEL carried into the next row
+        elif kind == 2:
+            f[m - 2] = F.DEL_START  # INSIDE_DEL carried into the next row
+        return f
+
+    haps = [F.PdHaplotype(base, flags(k)) for k in (0, 1, 2, 0)]
+    reads = []
+    for length in (n, n - 3, max(1, n // 2), max(1, n // 3)):
+        start = rng.below(m - length + 1)
+        b = bytearray(base[start:start + length])
+        for _ in range(max(1, length // 40)):
+            b[rng.below(length)] = b"ACGT"[rng.below(4)]
+        reads.append(F.ReadRef(bytes(b), [20 + rng.below(21) for _ in range(length)],
+                               [40] * length, [40] * length, [10] * length))
+    expected = pd_reference_all(reads, haps)
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what=f"pd shared maxima per call double n={n}")
+    out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, f"pd shared maxima per call float n={n}")
+    b = F.Batch(batch_from_regions([(reads, haps)], pd=True))
+    b.run(F.MODE_FP64, share=True)
+    assert_rel(b.results(), expected, what=f"pd shared maxima batch double n={n}")
+    b.run(F.MODE_FP32_RESCUE, share=True)
+    assert_close(b.results(), expected, FLOAT_TOL, f"pd shared maxima batch float n={n}")
+    b.close()
+
+
 def test_pd_adjacent_and_single_base_deletions_match_the_reference(gpu):
     hap = b"ACGTTGCAAGGCTTAGGCTTACGATCGATTACA"
     n = len(hap)
