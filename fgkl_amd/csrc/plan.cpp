@@ -549,9 +549,10 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     if (in.pairs_only) return;
 
     // Work items.  Two reads of a region can share a wave: read A's rows fill lanes [0, L_A),
-    // read B's the lanes after, both swept against the same haplotype columns (B lags L_A
-    // steps).  A step costs about kStepFixed + kStepPerRow * R VALU instructions (wave-wide
-    // moves, the column word, then R cells), and a region's sweeps take sum(W) + H' (L - 1) steps
+    // read B's the lanes after, both swept against the same haplotype columns, each on its own
+    // anti-diagonal.  A step costs about kStepFixed + kStepPerRow * R VALU instructions
+    // (wave-wide moves, the column word, then R cells), and a region's sweeps take
+    // sum(W) + H' (L - 1) steps with L = max(L_A, L_B)
     // (H' sweeps, forward over the columns between the shared prefix and the cut, and backward
     // over shared suffixes).  Reads are paired greedily, longest first, each with the free read
     // that lowers the modelled cost most; the candidates are, for every R, the longest free read
@@ -596,10 +597,12 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         const int R = rows_per_lane(n);
         return item_cost(R, lanes.of(n, R));
       };
+      // Two reads sweep side by side, each on its own anti-diagonal: the ramp is the larger
+      // read's lanes.
       auto pair_cost = [&](int64_t na, int64_t nb) {
         for (int R = 1; R <= kMaxRowsPerLane; ++R) {  // rows_per_lane2 through the table
-          const int64_t L = lanes.of(na, R) + lanes.of(nb, R);
-          if (L <= 64) return item_cost(R, L);
+          const int64_t La = lanes.of(na, R), Lb = lanes.of(nb, R);
+          if (La + Lb <= 64) return item_cost(R, std::max(La, Lb));
         }
         return INT64_MAX;
       };
@@ -627,7 +630,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         }
         const int64_t b = nb > 0 ? free_reads.take(nb) : -1;
         const int R = b >= 0 ? rows_per_lane2(na, nb) : rows_per_lane(na);
-        const int64_t L = lanes_for(na, R) + (b >= 0 ? lanes_for(nb, R) : 0);
+        const int64_t L = std::max(lanes_for(na, R), b >= 0 ? lanes_for(nb, R) : int64_t(0));
         auto& bucket = q.bucket_items[R - 1];
         bucket.push_back(int32_t(a));
         bucket.push_back(int32_t(b));
