@@ -184,6 +184,8 @@ template <typename T> struct Cell {
 template <typename T, int R, bool PD> struct Lane {
   T m[R], x[R], y[R];
   T bm[PD ? R : 1], bx[PD ? R : 1], by[PD ? R : 1];
+  // PD: each row's diagonal maxima for its next column (see cells).
+  T dm[PD ? R : 1], dx[PD ? R : 1], dy[PD ? R : 1];
   T acc;
   uint32_t w;
 };
@@ -248,7 +250,12 @@ __device__ __forceinline__ double max_nonneg(double a, double b) { return fmax(a
 // UNI (PD): every row of the lane reads equal after-deletion bits, or holds zero branch values
 // (identity rows and lanes past the read), so a row's left maxima are the diagonal maxima the
 // row below it has just formed from the same old values: they are reused, not recomputed.
-template <typename T, int R, bool PD, bool UNI = false>
+// CARRY (PD): a row's diagonal maxima in column j+1 are its up maxima in column j.  The state
+// after a DEL_END column is AFTER_DEL for every read row, so column j+1's after-deletion mask is
+// column j's DEL_END mask; identity rows and lanes past the read differ there only in rows whose
+// branch values are zero or whose prior is zero.  So the up maxima (Y included) are kept as the
+// next step's diagonal maxima, which the sweep forms afresh only for its first column.
+template <typename T, int R, bool PD, bool UNI = false, bool CARRY = false>
 __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
                                       const Cell<T>& d, const Cell<T>& u) {
   const uint32_t w = s.w;
@@ -283,9 +290,16 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       const T pdbm = k ? s.bm[k - 1] : d.bm;
       const T pdbx = k ? s.bx[k - 1] : d.bx;
       const T pdby = k ? s.by[k - 1] : d.by;
-      const T dme = max_nonneg(pdm, and_bits(pdbm, A));
-      const T dxe = max_nonneg(pdx, and_bits(pdbx, A));
-      const T dye = max_nonneg(pdy, and_bits(pdby, A));
+      T dme, dxe, dye;
+      if constexpr (CARRY) {
+        dme = s.dm[k];
+        dxe = s.dx[k];
+        dye = s.dy[k];
+      } else {
+        dme = max_nonneg(pdm, and_bits(pdbm, A));
+        dxe = max_nonneg(pdx, and_bits(pdbx, A));
+        dye = max_nonneg(pdy, and_bits(pdby, A));
+      }
       T lme, lxe, lye;
       if (!UNI || k == R - 1) {
         lme = max_nonneg(s.m[k], and_bits(s.bm[k], A));
@@ -328,6 +342,13 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       const T ume = max_nonneg(pum, and_bits(pubm, de));
       const T uxe = max_nonneg(pux, and_bits(pubx, de));
       s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[k]);
+      if constexpr (CARRY) {
+        const T puy = k ? s.y[k - 1] : u.y;
+        const T puby = k ? s.by[k - 1] : u.by;
+        s.dm[k] = ume;
+        s.dx[k] = uxe;
+        s.dy[k] = max_nonneg(puy, and_bits(puby, de));
+      }
     }
   }
   s.acc += s.m[R - 1] + s.x[R - 1];
@@ -430,6 +451,20 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     if (row0_above) P.y = init;
   }
   s.w = 0;
+  if constexpr (PD) {
+    // The diagonal maxima of the first column, under that column's after-deletion masks.
+    const uint32_t w0 = lw[0];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const uint32_t A = bit_mask(w0, rw.sh[k] + 1);
+      const T pm = k ? s.m[k - 1] : P.m, px = k ? s.x[k - 1] : P.x, py = k ? s.y[k - 1] : P.y;
+      const T pbm = k ? s.bm[k - 1] : P.bm, pbx = k ? s.bx[k - 1] : P.bx;
+      const T pby = k ? s.by[k - 1] : P.by;
+      s.dm[k] = max_nonneg(pm, and_bits(pbm, A));
+      s.dx[k] = max_nonneg(px, and_bits(pbx, A));
+      s.dy[k] = max_nonneg(py, and_bits(pby, A));
+    }
+  }
 
   auto record = [&](const Cell<T>& u) {
     if constexpr (SNAP) {
@@ -448,7 +483,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells<T, R, PD, UNI>(s, rw, P, Q);
+      cells<T, R, PD, UNI, PD>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -461,26 +496,26 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
     s.w = next_word(cq.x, s.w, head);
     shift_up(s, Q, keep);
-    cells<T, R, PD, UNI>(s, rw, P, Q);
+    cells<T, R, PD, UNI, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.y, s.w, head);
     shift_up(s, P, keep);
-    cells<T, R, PD, UNI>(s, rw, Q, P);
+    cells<T, R, PD, UNI, PD>(s, rw, Q, P);
     record(P);
     s.w = next_word(cq.z, s.w, head);
     shift_up(s, Q, keep);
-    cells<T, R, PD, UNI>(s, rw, P, Q);
+    cells<T, R, PD, UNI, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.w, s.w, head);
     shift_up(s, P, keep);
-    cells<T, R, PD, UNI>(s, rw, Q, P);
+    cells<T, R, PD, UNI, PD>(s, rw, Q, P);
     record(P);
   }
   for (; t < steps; ++t) {
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells<T, R, PD, UNI>(s, rw, P, Q);
+      cells<T, R, PD, UNI, PD>(s, rw, P, Q);
       P = Q;
       record(P);
     }
