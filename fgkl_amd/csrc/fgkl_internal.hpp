@@ -68,11 +68,8 @@ struct HapPlan {        // one per haplotype, region-contiguous, in sorted order
   int32_t nsnap;        // columns this haplotype must snapshot
   int32_t cut;          // suffix sharing: forward sweep stops at column cut - 1 (0: no cut)
   int32_t cut_node;     // backward node the cut joins (record index)
-  // PD: the deletion state carried out of a row is NORMAL, so read row 1 and the rows below it
-  // read equal state bits in every column (0 for plain haplotypes).
-  int32_t uni;
 };
-static_assert(sizeof(HapPlan) == 56, "HapPlan layout");
+static_assert(sizeof(HapPlan) == 48, "HapPlan layout");
 
 // One backward sweep of a region's suffix plan (plan.rs:14-31): over columns right-1 down to
 // left of sorted haplotype `hap` (a global plan index), starting from node `start_node`'s record

@@ -170,7 +170,6 @@ __global__ void prep_kernel(const int32_t* ro32, const int32_t* ho32, int64_t n_
       h.nsnap = 0;
       h.cut = 0;
       h.cut_node = 0;
-      h.uni = 0;
       plan[i] = h;
     }
     if (i < pairs) {

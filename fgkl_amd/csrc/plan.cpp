@@ -464,7 +464,6 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         hp.nsnap = 0;
         hp.cut = 0;
         hp.cut_node = 0;
-        hp.uni = in.pd && (*rh.end_state)[size_t(h)] == kNormal ? 1 : 0;
         if (is_long) {
           hp.lcp = 0;
           hp.dup = 0;

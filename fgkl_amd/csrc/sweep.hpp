@@ -247,15 +247,15 @@ __device__ __forceinline__ double max_nonneg(double a, double b) { return fmax(a
 // The R cells of one column.  Rows are updated in place: match, deletion and branch values
 // bottom-up (each row still reads the old column of the row above), then insertion values
 // top-down (each row reads the new column of the row above), so no value is copied.
-// UNI (PD): every row of the lane reads equal after-deletion bits, or holds zero branch values
-// (identity rows and lanes past the read), so a row's left maxima are the diagonal maxima the
-// row below it has just formed from the same old values: they are reused, not recomputed.
-// CARRY (PD): a row's diagonal maxima in column j+1 are its up maxima in column j.  The state
-// after a DEL_END column is AFTER_DEL for every read row, so column j+1's after-deletion mask is
-// column j's DEL_END mask; identity rows and lanes past the read differ there only in rows whose
-// branch values are zero or whose prior is zero.  So the up maxima (Y included) are kept as the
-// next step's diagonal maxima, which the sweep forms afresh only for its first column.
-template <typename T, int R, bool PD, bool UNI = false, bool CARRY = false>
+// CARRY (PD): the state after a DEL_END column is AFTER_DEL for every read row, so column j+1's
+// after-deletion mask is column j's DEL_END mask for every row.  A row's diagonal maxima in
+// column j+1 are then the up maxima it forms in column j, and the left maxima of the row above
+// it are the same values: both are kept (s.dm, with Y included) from one step to the next, and
+// only the lane's last row forms its left maxima afresh.  Identity rows and lanes past the read
+// read other mask bits but hold zero branch values (or have a zero prior), where the masks do
+// not matter; so are all values of column 0, where a sweep from the first column starts.  The
+// sweep forms the diagonal maxima of its first column afresh.
+template <typename T, int R, bool PD, bool CARRY = false>
 __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
                                       const Cell<T>& d, const Cell<T>& u) {
   const uint32_t w = s.w;
@@ -267,7 +267,6 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
   // included).  All values are non-negative, so max(v, b & after) is v outside AFTER_DEL and
   // max(v, b) inside it, and the states need no compares or selects.
   const uint32_t de = PD ? bit_mask(w, kDelEndBit) : 0u;
-  T nlme = T(0), nlxe = T(0), nlye = T(0);  // UNI: the next row up's left maxima
 #pragma unroll
   for (int k = R - 1; k >= 0; --k) {
     const T pdm = k ? s.m[k - 1] : d.m;
@@ -301,18 +300,15 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
         dye = max_nonneg(pdy, and_bits(pdby, A));
       }
       T lme, lxe, lye;
-      if (!UNI || k == R - 1) {
+      if (!CARRY || k == R - 1) {
         lme = max_nonneg(s.m[k], and_bits(s.bm[k], A));
         lxe = max_nonneg(s.x[k], and_bits(s.bx[k], A));
         lye = max_nonneg(s.y[k], and_bits(s.by[k], A));
-      } else {
-        lme = nlme;
-        lxe = nlxe;
-        lye = nlye;
+      } else {  // the row below's diagonal maxima, formed from the same values last step
+        lme = s.dm[k + 1];
+        lxe = s.dx[k + 1];
+        lye = s.dy[k + 1];
       }
-      nlme = dme;  // row k-1's left maxima (its old values under equal bits)
-      nlxe = dxe;
-      nlye = dye;
       const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
       // NORMAL copies the row's values (lme is m there), AFTER_DEL takes the maxima, INSIDE_DEL
       // holds the branch.
@@ -403,7 +399,7 @@ __device__ __forceinline__ uint32_t next_word(uint32_t from_lds, uint32_t w, uin
 // once it has computed column W, so it ends holding its rows of column W (a cut haplotype joins
 // from there) and s.acc of a read's last lane is the raw scaled sum over the read's last row.
 // `span` is the lane count of the wave's longest read.
-template <typename T, int R, bool PD, bool SNAP, bool UNI = false>
+template <typename T, int R, bool PD, bool SNAP>
 __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int span, int L,
                       int pos, int lane, uint32_t head, T init, const T* snap_in, T scale,
                       T* scratch, bool row0_above, uint32_t keep, Lane<T, R, PD>& s) {
@@ -483,7 +479,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells<T, R, PD, UNI, PD>(s, rw, P, Q);
+      cells<T, R, PD, PD>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -496,26 +492,26 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
     s.w = next_word(cq.x, s.w, head);
     shift_up(s, Q, keep);
-    cells<T, R, PD, UNI, PD>(s, rw, P, Q);
+    cells<T, R, PD, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.y, s.w, head);
     shift_up(s, P, keep);
-    cells<T, R, PD, UNI, PD>(s, rw, Q, P);
+    cells<T, R, PD, PD>(s, rw, Q, P);
     record(P);
     s.w = next_word(cq.z, s.w, head);
     shift_up(s, Q, keep);
-    cells<T, R, PD, UNI, PD>(s, rw, P, Q);
+    cells<T, R, PD, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.w, s.w, head);
     shift_up(s, P, keep);
-    cells<T, R, PD, UNI, PD>(s, rw, Q, P);
+    cells<T, R, PD, PD>(s, rw, Q, P);
     record(P);
   }
   for (; t < steps; ++t) {
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells<T, R, PD, UNI, PD>(s, rw, P, Q);
+      cells<T, R, PD, PD>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -865,23 +861,12 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         const int W = (cut ? p.cut - 1 : p.len) - start;
         stage_hap<PD>(a, p, start, W, L, lane, lw, nsnap);
         Lane<T, R, PD> st;
-        // PD haplotypes whose rows read equal state bits take the sweep that shares maxima.
-        const bool uni = PD && __builtin_amdgcn_readfirstlane(p.uni) != 0;
-        if (nsnap) {
-          if (uni)
-            sweep<T, R, PD, true, PD>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                      scratch, row0_above, keep, st);
-          else
-            sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                  scratch, row0_above, keep, st);
-        } else {
-          if (uni)
-            sweep<T, R, PD, false, PD>(rw, lw, W, span, L, pos, lane, head, init, snap_in,
-                                       scale, scratch, row0_above, keep, st);
-          else
-            sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                   scratch, row0_above, keep, st);
-        }
+        if (nsnap)
+          sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                scratch, row0_above, keep, st);
+        else
+          sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
+                                 scratch, row0_above, keep, st);
         // A read's last row is its last lane's last register (kernel.rs:897-914).
         // A joined pair is held to kMinAcceptedJoined (kernel.rs:261-275).
         auto finish = [cut](double raw) {
