@@ -344,6 +344,20 @@ def selftest_pool(threads: int, rounds: int) -> dict:
     return dict(zip(keys, (int(v) for v in out)))
 
 
+def plan_snapshots(data: "BatchData", pd: bool | None = None) -> dict:
+    """Prefix-sharing snapshot records of a batch's plan per FP32 pass, summed over reads, planned
+    on the host alone (fgkl_plan_snapshots): written to HBM slots, kept in a wave's LDS record,
+    and sweeps that resume from a record."""
+    d = data
+    out = (C.c_int64 * 3)()
+    _check(lib().fgkl_plan_snapshots(
+        int(d.pd if pd is None else pd), d.n_regions, _ptr(d.region_reads, C.c_int64),
+        _ptr(d.region_haps, C.c_int64), _ptr(d.read_offsets, C.c_int64),
+        _ptr(d.hap_offsets, C.c_int64), _ptr(d.hap_bases, C.c_uint8),
+        _ptr(d.hap_flags, C.c_uint8), out))
+    return dict(zip(("hbm", "lds", "restarts"), (int(x) for x in out)))
+
+
 # ---- batched engine --------------------------------------------------------------------------
 class Batch:
     """Many regions resident in device memory (include/fgkl.h, fgkl_batch_*)."""
@@ -378,6 +392,18 @@ class Batch:
         w = (C.c_int64 * 4)()
         _check(lib().fgkl_batch_work(self._h, w))
         return dict(zip(("nominal", "prefix", "forward", "backward"), (int(x) for x in w)))
+
+    def set_lds_records(self, enable: bool):
+        """Lets prefix-sharing launches keep a snapshot record in each wave's LDS (the default)
+        or sends every record to HBM (fgkl_batch_set_lds_records); the bits are the same."""
+        _check(lib().fgkl_batch_set_lds_records(self._h, int(bool(enable))))
+
+    def snapshots(self) -> dict:
+        """Snapshot records of the plan per FP32 pass (HBM, LDS, restarts) and the sweep launches
+        so far that kept a record in LDS (fgkl_batch_snapshots)."""
+        out = (C.c_int64 * 4)()
+        _check(lib().fgkl_batch_snapshots(self._h, out))
+        return dict(zip(("hbm", "lds", "restarts", "lds_launches"), (int(x) for x in out)))
 
     def tune(self, share: bool = True):
         """Times each FP32 bucket's launch configurations and keeps the fastest

@@ -338,6 +338,47 @@ int fgkl_batch_work(const fgkl_batch* batch, int64_t* cells4) {
   return kOk;
 }
 
+int fgkl_batch_set_lds_records(fgkl_batch* batch, int enable) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  if (enable != 0 && enable != 1) return fail(kInvalidArgument, "enable must be 0 or 1");
+  batch->batch->set_lds_records(enable != 0);
+  return kOk;
+}
+
+int fgkl_batch_snapshots(const fgkl_batch* batch, int64_t* out4) {
+  if (!batch || !out4) return fail(kInvalidArgument, "null argument");
+  batch->batch->snapshots(out4);
+  return kOk;
+}
+
+int fgkl_plan_snapshots(int pd, int64_t n_regions, const int64_t* region_reads,
+                        const int64_t* region_haps, const int64_t* read_offsets,
+                        const int64_t* hap_offsets, const uint8_t* hap_bases,
+                        const uint8_t* hap_flags, int64_t* out3) {
+  if (!region_reads || !region_haps || !read_offsets || !hap_offsets || !hap_bases || !out3)
+    return fail(kInvalidArgument, "null argument");
+  if (pd && !hap_flags) return fail(kInvalidArgument, "a PD batch needs haplotype flags");
+  try {
+    BatchInput in;
+    in.pd = pd != 0;
+    in.n_regions = n_regions;
+    in.region_read = region_reads;
+    in.region_hap = region_haps;
+    in.read_off = read_offsets;
+    in.hap_off = hap_offsets;
+    in.hap_bases = hap_bases;
+    in.hap_flags = pd ? hap_flags : nullptr;
+    HostPlan plan;
+    if (int rc = build_plan(in, plan)) return rc;
+    out3[0] = plan.snap_hbm;
+    out3[1] = plan.snap_lds;
+    out3[2] = plan.snap_restarts;
+    return kOk;
+  } catch (const std::exception& e) {
+    return fail(kInternal, e.what());
+  }
+}
+
 int fgkl_batch_tune(fgkl_batch* batch, int share_prefixes, fgkl_tune_report* report) {
   if (!batch) return fail(kInvalidArgument, "null batch");
   try {

@@ -4,9 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -78,8 +80,8 @@ struct DeviceState {
   DevTab<double> td{};
   // Resident blocks per CU of each kernel [fp64][pd][boost][rows-1][block], at the LDS size it
   // was queried with.
-  int occ[2][2][2][kMaxRowsPerLane][3] = {};
-  int occ_lds[2][2][2][kMaxRowsPerLane][3] = {};
+  // Resident blocks per CU of a sweep kernel at a block size and LDS bytes per block.
+  std::map<std::array<int, 6>, int> occ;
   // Launch configurations the autotuner chose for FP32 buckets [pd][R-1] (rows 0: none yet).
   LaunchCfg tuned[2][kMaxRowsPerLane];
   // Resident one-wave blocks per CU of long_kernel [fp64][pd] (0: not queried yet).
@@ -144,16 +146,13 @@ template <typename T>
 int resident_blocks(DeviceState* st, bool pd, bool boost, int rows, int block, int lds_bytes,
                     int* blocks) {
   const int ti = std::is_same<T, double>::value ? 1 : 0;
-  const int bi = block_index(block);
   std::lock_guard<std::mutex> lock(g_dev_mutex);
-  int& occ = st->occ[ti][pd][boost][rows - 1][bi];
-  int& occ_lds = st->occ_lds[ti][pd][boost][rows - 1][bi];
-  if (occ == 0 || occ_lds != lds_bytes) {
+  int& occ = st->occ[{ti, int(pd), int(boost), rows, block, lds_bytes}];
+  if (occ == 0) {
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, Kernels<T>::get(pd, boost, rows),
                                                          block, size_t(lds_bytes)));
     occ = std::max(nb, 1);
-    occ_lds = lds_bytes;
   }
   *blocks = occ;
   return kOk;
@@ -263,6 +262,8 @@ struct DeviceBatch::Impl {
   hipEvent_t bev[2 * kMaxRowsPerLane] = {};  // around each primary bucket launch
   hipGraphExec_t graph[8] = {};  // by mode * 2 + share
   int lds_words = 0;
+  bool lds_records = true;  // prefix-sharing launches may keep a snapshot record in LDS
+  int64_t lds_launches = 0;  // launches (eager or captured) that kept one
   LaunchCfg cfg[kMaxRowsPerLane];  // FP32 primary launch of each bucket
 
   void drop_graphs() {
@@ -390,15 +391,45 @@ struct DeviceBatch::Impl {
   }
 
   // Waves a launch of bucket R's `max_items` items runs with: the resident limit, or fewer when
-  // the bucket is small.
+  // the bucket is small.  `wave_words` is the launch's LDS per wave (default: the staging only).
   template <typename T>
-  int grid_waves(const LaunchCfg& c, int rows, int64_t max_items, int* grid) {
+  int grid_waves(const LaunchCfg& c, int rows, int64_t max_items, int* grid, int wave_words = 0) {
     const int wpb = c.block / 64;
     int per_cu = 0;
-    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, lds_words * 4 * wpb, &per_cu))
+    const int words = wave_words ? wave_words : lds_words;
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, words * 4 * wpb, &per_cu))
       return rc;
     const int64_t want = (max_items + wpb - 1) / wpb;
     *grid = int(std::min<int64_t>(int64_t(per_cu) * st->num_cus, want));
+    return kOk;
+  }
+
+  // Words of a wave's LDS snapshot record at `rows` rows per lane: 64 lanes of snap_stride
+  // values of T.
+  template <typename T>
+  int lds_record_words(int rows) const {
+    const int nv = ((pd ? 6 * rows + 1 : 3 * rows + 1) + 1) & ~1;  // snap_stride
+    return 64 * nv * int(sizeof(T) / 4);
+  }
+  // Gives a prefix-sharing launch the wave's LDS snapshot record when the plan puts records
+  // there and the record costs no resident blocks (and fits the 64 KiB a block may hold);
+  // otherwise its records all go to HBM slots.  Sets the LDS fields of `a`.
+  template <typename T>
+  int lds_layout(SweepArgs& a, const LaunchCfg& c, int rows) {
+    a.lds_words = lds_words;
+    a.lds_rec = lds_words;
+    a.lds_snap = 0;
+    if (!lds_records || !a.share || a.pair_mode || plan.snap_lds == 0) return kOk;
+    const int wpb = c.block / 64;
+    const int with = lds_words + lds_record_words<T>(rows);
+    if (int64_t(with) * 4 * wpb > 64 * 1024) return kOk;
+    int b0 = 0, b1 = 0;
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, lds_words * 4 * wpb, &b0)) return rc;
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, with * 4 * wpb, &b1)) return rc;
+    if (b1 < b0) return kOk;
+    a.lds_words = with;
+    a.lds_snap = 1;
+    ++lds_launches;
     return kOk;
   }
 
@@ -412,8 +443,9 @@ struct DeviceBatch::Impl {
     if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
     if (!std::is_same<T, float>::value) c.boost = a.suffix != 0;  // FP64: the suffix build
+    if (int rc = lds_layout<T>(a, c, rows)) return rc;
     int grid = 0;
-    if (int rc = grid_waves<T>(c, rows, max_items, &grid)) return rc;
+    if (int rc = grid_waves<T>(c, rows, max_items, &grid, a.lds_words)) return rc;
     if (a.scratch) {
       a.scratch_wave_stride = scratch_stride<T>(rows);
       a.node_off = snap_elems<T>(rows);
@@ -428,7 +460,7 @@ struct DeviceBatch::Impl {
       }
     }
     hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows), dim3(grid), dim3(c.block),
-                       lds_words * 4 * (c.block / 64), on, a, dev_tab<T>(st));
+                       a.lds_words * 4 * (c.block / 64), on, a, dev_tab<T>(st));
     HIP_TRY(hipGetLastError());
     return kOk;
   }
@@ -444,7 +476,7 @@ struct DeviceBatch::Impl {
   // Per wave: the snapshot slots, then (plain) the suffix-sharing node records.
   template <typename T>
   int64_t snap_elems(int rows) const {
-    const int nv = ((pd ? 6 * rows + 7 : 3 * rows + 4) + 3) & ~3;  // snap_stride
+    const int nv = ((pd ? 6 * rows + 1 : 3 * rows + 1) + 1) & ~1;  // snap_stride
     return int64_t(plan.max_slots) * nv * 64;
   }
   template <typename T>
@@ -1152,6 +1184,18 @@ int DeviceBatch::set_launch(int R, const LaunchCfg& cfg) {
   d.cfg[R - 1].rows = cfg.rows ? cfg.rows : R;
   d.drop_graphs();
   return kOk;
+}
+
+void DeviceBatch::set_lds_records(bool on) {
+  impl_->lds_records = on;
+  impl_->drop_graphs();
+}
+
+void DeviceBatch::snapshots(int64_t out[4]) const {
+  out[0] = impl_->plan.snap_hbm;
+  out[1] = impl_->plan.snap_lds;
+  out[2] = impl_->plan.snap_restarts;
+  out[3] = impl_->lds_launches;
 }
 
 int DeviceBatch::get_launch(int R, LaunchCfg* cfg) const {

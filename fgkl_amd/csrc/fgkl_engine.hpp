@@ -75,6 +75,12 @@ class DeviceBatch {
   // Sets (or reads) bucket R's configuration explicitly; kInvalidArgument outside the tuned
   // space.
   int set_launch(int R, const LaunchCfg& cfg);
+  // Whether prefix-sharing launches may keep a snapshot record in the wave's LDS (default on;
+  // off: every record goes to an HBM slot).  Results are the same bits either way.
+  void set_lds_records(bool on);
+  // Snapshot records of the plan per FP32 pass (HBM, LDS-eligible, restarts) and the sweep
+  // launches so far that kept a record in LDS.
+  void snapshots(int64_t out[4]) const;
   int get_launch(int R, LaunchCfg* cfg) const;
 
  private:

@@ -62,7 +62,7 @@ struct HapPlan {        // one per haplotype, region-contiguous, in sorted order
   int32_t lcp;          // common prefix with the previous sorted haplotype (0 for the first)
   int32_t src;          // caller's index of this haplotype within its region
   int32_t dup;          // equal to the previous sorted haplotype: copies its results
-  int32_t start_slot;   // snapshot slot holding column `lcp`, or -1 to start from column 0
+  int32_t start_slot;   // snapshot slot holding column `lcp` (kSnapLds: maybe in LDS), or -1
   int32_t start_src_len;  // length of the haplotype that wrote that snapshot
   int32_t snap0;        // first entry in the snapshot list
   int32_t nsnap;        // columns this haplotype must snapshot
@@ -87,8 +87,13 @@ struct BwdSweep {
 
 struct SnapEntry {
   int32_t col;   // 1-based haplotype column
-  int32_t slot;  // scratch slot
+  int32_t slot;  // scratch slot, with kSnapLds set when the wave's LDS record may hold it
 };
+
+// Set in a snapshot slot (SnapEntry::slot, HapPlan::start_slot) whose record the wave keeps in
+// its LDS record when the launch has one; the low bits still name the column's HBM slot, which
+// a launch without the LDS record uses.
+constexpr int32_t kSnapLds = int32_t(1) << 30;
 
 // A batch as the caller hands it over: regions of reads and haplotypes, concatenated.
 struct BatchInput {
@@ -141,6 +146,9 @@ struct HostPlan {
   // DP cells of the batch: nominal (every read base x haplotype base), computed with prefix
   // sharing alone, and with suffix sharing (forward, backward) in the FP32 pass.
   int64_t work_nominal = 0, work_prefix = 0, work_forward = 0, work_backward = 0;
+  // Snapshot records one FP32 pass writes (summed over reads): to HBM slots, to the wave's LDS
+  // record; and the sweeps that start from a record.
+  int64_t snap_hbm = 0, snap_lds = 0, snap_restarts = 0;
   // Regions with a read of more than kMaxReadLen bases or a haplotype of more than
   // kMaxStagedHap run every pair in long_kernel, unshared: pairs (read, sorted haplotype) in
   // region order, region g's from region_long[g] to region_long[g + 1]; their reads have

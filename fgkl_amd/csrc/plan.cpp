@@ -394,10 +394,104 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     int32_t long_max_hap = 0;
     int64_t long_cells = 0;
     int64_t work_nominal = 0, work_prefix = 0, work_forward = 0, work_backward = 0;
+    int64_t snap_hbm = 0, snap_lds = 0, snap_restarts = 0;
     std::vector<int32_t> bucket_items[kMaxRowsPerLane];
     int64_t bucket_pairs[kMaxRowsPerLane] = {}, bucket_cells[kMaxRowsPerLane] = {},
             bucket_work[kMaxRowsPerLane] = {};
     std::vector<int64_t> item_costs[kMaxRowsPerLane];
+  };
+
+  // Snapshot records of one region's sorted haplotypes hp[0..H) (kernel.rs:236-259, 556-562).
+  // A haplotype after the first starts at hp[k].lcp, the column it shares with the previous
+  // sorted one (0: from column 0), from the state the last earlier sweep to pass that column left
+  // there, rescaled by the two lengths (kernel.rs:641).  Only those states are recorded: sweep j
+  // records column c when it is the last sweep to pass c before a haplotype starts at c.  A sweep
+  // passes c when it starts before c and computes c, up to the column before its cut.  A sweep
+  // that starts before c and stops short of it ends the prefix earlier records of c hold, so a
+  // haplotype that would start at c with no record left starts from column 0 (sorted order never
+  // leads there: the cut lies past every column shared with a neighbour).
+  // A wave holds one record in LDS.  Records take it in order of their last reader, earliest
+  // first (the schedule that fits the most intervals in one slot); the others are written to the
+  // HBM slot of their column.  A record lives from the sweep that writes it to the start of its
+  // last reader, and a sweep restores its start before it records anything.
+  auto plan_snapshots = [](HapPlan* hp, int64_t H, const std::map<int64_t, int32_t>& slot_of,
+                           int64_t reads, Part& q) {
+    struct Rec {
+      int32_t writer, col, last;
+      bool lds;
+    };
+    std::vector<Rec> recs;
+    std::map<std::pair<int32_t, int32_t>, int32_t> rec_of;  // (writer, column) -> record
+    std::vector<int32_t> reads_rec(size_t(H), -1);
+    std::map<int64_t, int32_t> pending;    // column -> later haplotypes starting there
+    std::map<int64_t, int32_t> last_pass;  // column -> last sweep to pass it, or -1
+    for (int64_t k = 1; k < H; ++k)
+      if (!hp[k].dup && hp[k].lcp > 0) pending[hp[k].lcp] += 1;
+    for (int64_t k = 0; k < H; ++k) {
+      if (hp[k].dup) continue;
+      if (hp[k].lcp > 0) {
+        pending[hp[k].lcp] -= 1;
+        const auto it = last_pass.find(hp[k].lcp);
+        const int32_t w = it == last_pass.end() ? -1 : it->second;
+        if (w < 0) {
+          hp[k].lcp = 0;
+        } else {
+          const auto key = std::make_pair(w, hp[k].lcp);
+          auto r = rec_of.find(key);
+          if (r == rec_of.end()) {
+            r = rec_of.emplace(key, int32_t(recs.size())).first;
+            recs.push_back(Rec{w, hp[k].lcp, int32_t(k), false});
+          }
+          recs[size_t(r->second)].last = int32_t(k);
+          reads_rec[size_t(k)] = r->second;
+        }
+      }
+      const int64_t end = hp[k].cut > 0 ? hp[k].cut - 1 : hp[k].len;
+      for (auto it = pending.upper_bound(hp[k].lcp); it != pending.end(); ++it)
+        if (it->second > 0) last_pass[it->first] = it->first <= end ? int32_t(k) : -1;
+    }
+    std::vector<int32_t> by_last(recs.size());
+    std::iota(by_last.begin(), by_last.end(), 0);
+    std::sort(by_last.begin(), by_last.end(), [&](int32_t x, int32_t y) {
+      const Rec &a = recs[size_t(x)], &b = recs[size_t(y)];
+      return a.last != b.last ? a.last < b.last : a.writer < b.writer;
+    });
+    int32_t lds_free_from = -1;
+    for (int32_t i : by_last) {
+      Rec& r = recs[size_t(i)];
+      if (r.writer >= lds_free_from) {
+        r.lds = true;
+        lds_free_from = r.last;
+      }
+    }
+    std::vector<int32_t> by_writer(recs.size());
+    std::iota(by_writer.begin(), by_writer.end(), 0);
+    std::sort(by_writer.begin(), by_writer.end(), [&](int32_t x, int32_t y) {
+      const Rec &a = recs[size_t(x)], &b = recs[size_t(y)];
+      return a.writer != b.writer ? a.writer < b.writer : a.col < b.col;
+    });
+    size_t next = 0;
+    int64_t lds_recs = 0, restarts = 0;
+    for (int64_t k = 0; k < H; ++k) {
+      hp[k].snap0 = int32_t(q.snaps.size());
+      hp[k].nsnap = 0;
+      for (; next < by_writer.size() && recs[size_t(by_writer[next])].writer == k; ++next) {
+        const Rec& r = recs[size_t(by_writer[next])];
+        q.snaps.push_back(SnapEntry{r.col, slot_of.at(r.col) | (r.lds ? kSnapLds : 0)});
+        hp[k].nsnap += 1;
+        lds_recs += r.lds;
+      }
+      const int32_t ri = reads_rec[size_t(k)];
+      if (ri >= 0) {
+        const Rec& r = recs[size_t(ri)];
+        hp[k].start_slot = slot_of.at(r.col) | (r.lds ? kSnapLds : 0);
+        hp[k].start_src_len = hp[r.writer].len;
+        restarts += 1;
+      }
+    }
+    q.snap_hbm += reads * (int64_t(recs.size()) - lds_recs);
+    q.snap_lds += reads * lds_recs;
+    q.snap_restarts += reads * restarts;
   };
   auto plan_range = [&](int64_t g0, int64_t g1, Part& q) {
     std::vector<int64_t> order;
@@ -447,7 +541,6 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       // sweeps from column 0 (duplicates still copy).  A long region shares nothing.
       const bool share_prefix = !is_long && slot_of.size() <= size_t(kMaxRegionSlots);
       if (!share_prefix) slot_of.clear();
-      std::vector<int32_t> last_len(slot_of.size(), 0);
       q.max_slots = std::max<int32_t>(q.max_slots, int32_t(slot_of.size()));
 
       for (int64_t k = 0; k < H; ++k) {
@@ -472,21 +565,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
         }
         q.max_hap_len = std::max(q.max_hap_len, hp.len);
         if (hp.dup) continue;
-        const int64_t start = k == 0 || !share_prefix ? 0 : lcp[size_t(k)];
-        hp.lcp = int32_t(start);
-        if (k > 0) counts[start] -= 1;  // kernel.rs:248-250
-        if (start > 0) {
-          hp.start_slot = slot_of.at(start);
-          hp.start_src_len = last_len[size_t(hp.start_slot)];
-        }
-        // Columns in (start, len] some later haplotype starts from (kernel.rs:253-258).
-        for (auto it = slot_of.upper_bound(start); it != slot_of.end() && it->first <= hp.len; ++it) {
-          if (counts[it->first] > 0) {
-            q.snaps.push_back(SnapEntry{int32_t(it->first), it->second});
-            last_len[size_t(it->second)] = hp.len;
-            hp.nsnap += 1;
-          }
-        }
+        hp.lcp = int32_t(k == 0 || !share_prefix ? 0 : lcp[size_t(k)]);
       }
       if (is_long) {
         int64_t read_bases = 0, hap_bases = 0;
@@ -517,7 +596,8 @@ int build_plan(const BatchInput& in, HostPlan& p) {
           dp[size_t(k)] = char(hp.dup);
         }
         const SuffixPlan sp = plan_suffixes(hb, hl, lcp, dp);
-        if (sp.nodes - 1 > kMaxRegionSlots) continue;  // too many node records: no suffix sharing
+        // Beyond kMaxRegionSlots node records the region shares no suffixes.
+        if (sp.nodes - 1 <= kMaxRegionSlots) {
         for (int64_t k = 0; k < H; ++k) {
           const int32_t d = sp.depth[size_t(k)];
           if (d > 0) {
@@ -542,7 +622,9 @@ int build_plan(const BatchInput& in, HostPlan& p) {
           q.bsweeps.push_back(b);
         }
         q.max_nodes = std::max(q.max_nodes, sp.nodes - 1);
+        }
       }
+      if (share_prefix && H > 1) plan_snapshots(&p.plan[size_t(h0)], H, slot_of, r1 - r0, q);
     }
     q.region_bw.push_back(int32_t(q.bsweeps.size()));
     if (in.pairs_only) return;
@@ -670,6 +752,9 @@ int build_plan(const BatchInput& in, HostPlan& p) {
     p.work_prefix += q.work_prefix;
     p.work_forward += q.work_forward;
     p.work_backward += q.work_backward;
+    p.snap_hbm += q.snap_hbm;
+    p.snap_lds += q.snap_lds;
+    p.snap_restarts += q.snap_restarts;
     for (int R = 0; R < kMaxRowsPerLane; ++R) {
       p.bucket_items[R].insert(p.bucket_items[R].end(), q.bucket_items[R].begin(), q.bucket_items[R].end());
       item_costs[R].insert(item_costs[R].end(), q.item_costs[R].begin(), q.item_costs[R].end());

@@ -98,7 +98,11 @@ struct SweepArgs {
   int32_t* dense_items[kMaxRowsPerLane];  // by the read's own rows per lane
   int32_t* dense_count;                   // kMaxRowsPerLane counts
   int32_t nan_only;
-  int32_t lds_words;           // LDS words per wave for the staged haplotype
+  int32_t lds_words;           // LDS words per wave: the staged haplotype, then the record
+  // The wave's LDS snapshot record (when lds_snap is set): lds_rec words into its LDS, one
+  // snap_stride record per lane.  Snapshot entries and start slots marked kSnapLds use it.
+  int32_t lds_snap;
+  int32_t lds_rec;
   // Suffix sharing (plain main pass only, when `suffix` is set): the backward sweeps of
   // region g are bsweeps[region_bw[g] .. region_bw[g + 1]) with node writes in bwrites; node
   // records sit `node_off` elements into the wave's scratch, after its snapshot slots.
@@ -166,11 +170,13 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
   }
 }
 
-// Number of scratch values a lane keeps per snapshot slot, and the record stride: a slot is
-// 64 per-lane records of snap_stride values, so the one lane that records a column writes one
-// contiguous, 16-byte-aligned block.
-template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 7 : 3 * R + 4; }
-template <int R, bool PD> constexpr int snap_stride() { return (snap_values<R, PD>() + 3) & ~3; }
+// Number of values a lane keeps per snapshot record (its rows' M, X, Y, the PD branch values and
+// the running last-row sum), and the record stride: a record is 64 per-lane blocks of
+// snap_stride values, so the one lane that records a column writes one contiguous block.  The
+// row above a lane's first row is not kept: a restored lane takes it from the lane above with
+// the same DPP move the sweep makes, which yields the value that lane recorded.
+template <int R, bool PD> constexpr int snap_values() { return PD ? 6 * R + 1 : 3 * R + 1; }
+template <int R, bool PD> constexpr int snap_stride() { return (snap_values<R, PD>() + 1) & ~1; }
 
 // The values of one DP row in one column: match, insertion, deletion and (PD) branch values.
 template <typename T> struct Cell {
@@ -350,10 +356,9 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
   s.acc += s.m[R - 1] + s.x[R - 1];
 }
 
-// Writes a lane's state into its snapshot record (value by value); `u` is the row above
-// its first row in the column just computed.
+// Writes a lane's state into its snapshot record (value by value), in HBM or in LDS.
 template <typename T, int R, bool PD>
-__device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Cell<T>& u, T* d) {
+__device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, T* d) {
   constexpr int NV = snap_values<R, PD>();
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -366,23 +371,16 @@ __device__ __forceinline__ void store_snapshot(const Lane<T, R, PD>& s, const Ce
       d[(3 * R + 3 * k + 2)] = s.by[k];
     }
   }
-  constexpr int U = PD ? 6 * R : 3 * R;
-  d[(U + 0)] = u.m;
-  d[(U + 1)] = u.x;
-  d[(U + 2)] = u.y;
-  if constexpr (PD) {
-    d[(U + 3)] = u.bm;
-    d[(U + 4)] = u.bx;
-    d[(U + 5)] = u.by;
-  }
   d[(NV - 1)] = s.acc;
 }
 
-// A staged column word carries, in bits 20..31, one plus the scratch slot that must record the
-// column's state (0: none).  The word travels down the lanes with the column, so each lane sees
-// the mark exactly when it has just computed that column.
+// A staged column word carries, in bits 20..31, the record the column's state goes to: 0 none,
+// kLdsField the wave's LDS record, else one plus the HBM scratch slot.  The word travels down
+// the lanes with the column, so each lane sees the mark exactly when it has just computed that
+// column.
 constexpr uint32_t kSlotShift = 20;
-constexpr int kMaxSlots = 4095;
+constexpr uint32_t kLdsField = 4095;
+constexpr int kMaxSlots = 4094;
 
 // The column word a lane works on next: lane l-1's word moved down with DPP, or, in the first
 // lane of each read, the word from LDS (lane 0 gets it as the move's old operand; `head` is all
@@ -399,14 +397,17 @@ __device__ __forceinline__ uint32_t next_word(uint32_t from_lds, uint32_t w, uin
 // once it has computed column W, so it ends holding its rows of column W (a cut haplotype joins
 // from there) and s.acc of a read's last lane is the raw scaled sum over the read's last row.
 // `span` is the lane count of the wave's longest read.
+// `snap_in` (HBM) or `lds_in` (the wave's LDS record) holds the start column's state when the
+// sweep resumes from one; `lrec` is the wave's LDS record that columns marked kLdsField go to.
 template <typename T, int R, bool PD, bool SNAP>
 __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int span, int L,
-                      int pos, int lane, uint32_t head, T init, const T* snap_in, T scale,
-                      T* scratch, bool row0_above, uint32_t keep, Lane<T, R, PD>& s) {
+                      int pos, int lane, uint32_t head, T init, const T* snap_in,
+                      const T* lds_in, T scale, T* scratch, T* lrec, bool row0_above,
+                      uint32_t keep, Lane<T, R, PD>& s) {
   constexpr int NV = snap_values<R, PD>();
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
-  if (snap_in) {
-    const T* p = snap_in + lane * snap_stride<R, PD>();
+  if (snap_in || lds_in) {
+    const T* p = (lds_in ? lds_in : snap_in) + lane * snap_stride<R, PD>();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       s.m[k] = p[(3 * k + 0)] * scale;
@@ -418,16 +419,10 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
         s.by[k] = p[(3 * R + 3 * k + 2)] * scale;
       }
     }
-    constexpr int U = PD ? 6 * R : 3 * R;
-    P.m = p[(U + 0)] * scale;
-    P.x = p[(U + 1)] * scale;
-    P.y = p[(U + 2)] * scale;
-    if constexpr (PD) {
-      P.bm = p[(U + 3)] * scale;
-      P.bx = p[(U + 4)] * scale;
-      P.by = p[(U + 5)] * scale;
-    }
     s.acc = p[(NV - 1)] * scale;
+    // The row above the first row at the start column: the restored last row of the lane above,
+    // moved and masked as the sweep moves it, which is the value the lane saw when it recorded.
+    shift_up(s, P, keep);
   } else {
     // Column 0: zeros below row 0; identity rows carry row 0's deletion value C/m.
 #pragma unroll
@@ -462,11 +457,15 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     }
   }
 
-  auto record = [&](const Cell<T>& u) {
+  auto record = [&](const Cell<T>&) {
     if constexpr (SNAP) {
       const uint32_t sl = s.w >> kSlotShift;
-      if (sl)
-        store_snapshot<T, R, PD>(s, u, scratch + (size_t(sl - 1) * 64 + lane) * snap_stride<R, PD>());
+      if (sl) {
+        if (sl == kLdsField)
+          store_snapshot<T, R, PD>(s, lrec + lane * snap_stride<R, PD>());
+        else
+          store_snapshot<T, R, PD>(s, scratch + (size_t(sl - 1) * 64 + lane) * snap_stride<R, PD>());
+      }
     }
   };
   const int steps = W + span - 1;
@@ -519,7 +518,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
 }
 
 // Stages the words of haplotype columns start+1 .. m into the wave's LDS, zero-padded, and marks
-// the columns this sweep must record with their scratch slots.
+// the columns this sweep must record with their record (the wave's LDS record or an HBM slot).
 template <bool PD>
 __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, int start, int W,
                                           int L, int lane, uint32_t* lw, int nsnap) {
@@ -534,7 +533,10 @@ __device__ __forceinline__ void stage_hap(const SweepArgs& a, const HapPlan& p, 
     wave_sync_lds();
     for (int i = lane; i < nsnap; i += 64) {
       const SnapEntry e = a.snaps[p.snap0 + i];
-      if (e.col - start <= W) lw[e.col - start - 1] |= uint32_t(e.slot + 1) << kSlotShift;
+      const uint32_t field = (a.lds_snap && (e.slot & kSnapLds))
+                                 ? kLdsField
+                                 : uint32_t(e.slot & (kSnapLds - 1)) + 1;
+      if (e.col - start <= W) lw[e.col - start - 1] |= field << kSlotShift;
     }
   }
   wave_sync_lds();
@@ -727,6 +729,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
   const int wave_in_block = threadIdx.x >> 6;
   const int64_t wave_global = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave_in_block;
   uint32_t* lw = lds_all + wave_in_block * a.lds_words;
+  T* lrec = reinterpret_cast<T*>(lw + a.lds_rec);
   T* scratch = a.scratch ? static_cast<T*>(a.scratch) + wave_global * a.scratch_wave_stride : nullptr;
   const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
   constexpr int NV = snap_values<R, PD>();
@@ -850,9 +853,13 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         const int start = shared ? p.lcp : 0;
         const int nsnap = shared ? p.nsnap : 0;
         const T* snap_in = nullptr;
+        const T* lds_in = nullptr;
         T scale = T(1);
         if (shared && p.start_slot >= 0) {
-          snap_in = scratch + size_t(p.start_slot) * 64 * snap_stride<R, PD>();
+          if (a.lds_snap && (p.start_slot & kSnapLds))
+            lds_in = lrec;
+          else
+            snap_in = scratch + size_t(p.start_slot & (kSnapLds - 1)) * 64 * snap_stride<R, PD>();
           scale = T(double(p.start_src_len) / double(p.len));  // kernel.rs:641
         }
         const T init = T(Num<T>::kScale / double(p.len));      // kernel.rs:640
@@ -862,11 +869,11 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
         stage_hap<PD>(a, p, start, W, L, lane, lw, nsnap);
         Lane<T, R, PD> st;
         if (nsnap)
-          sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                scratch, row0_above, keep, st);
+          sweep<T, R, PD, true>(rw, lw, W, span, L, pos, lane, head, init, snap_in, lds_in, scale,
+                                scratch, lrec, row0_above, keep, st);
         else
-          sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, scale,
-                                 scratch, row0_above, keep, st);
+          sweep<T, R, PD, false>(rw, lw, W, span, L, pos, lane, head, init, snap_in, lds_in,
+                                 scale, scratch, lrec, row0_above, keep, st);
         // A read's last row is its last lane's last register (kernel.rs:897-914).
         // A joined pair is held to kMinAcceptedJoined (kernel.rs:261-275).
         auto finish = [cut](double raw) {

@@ -208,6 +208,31 @@ int64_t fgkl_batch_pairs(const fgkl_batch* batch);
  * prefix sharing; plain batches only, PD batches report 0 backward and forward = [1]).
  */
 int fgkl_batch_work(const fgkl_batch* batch, int64_t* cells4);
+/*
+ * Plans a batch on the host alone (no device; the same arrays as fgkl_batch_create, of which the
+ * read arrays are not read and may be null) and reports its prefix-sharing snapshot records per
+ * FP32 pass, summed over reads: out3[0] records written to HBM scratch slots, out3[1] records a
+ * wave keeps in its LDS record (launches that cannot hold one write these to HBM too), out3[2]
+ * sweeps that resume from a record.  No reference counterpart: the reference keeps its
+ * snapshot columns in the CPU workspace (crates/pairhmm/src/kernel.rs:366-396, 655-662).
+ */
+/*
+ * Whether the batch's prefix-sharing launches may keep a snapshot record in each wave's LDS
+ * (1, the default: when the plan marks records for it and the record costs the launch no
+ * resident workgroups) or write every record to an HBM scratch slot (0).  Results are the same
+ * bits either way.  No reference counterpart.
+ */
+int fgkl_batch_set_lds_records(fgkl_batch* batch, int enable);
+/*
+ * The batch's snapshot records per FP32 pass as fgkl_plan_snapshots reports them (out4[0..2])
+ * and out4[3], the sweep launches so far (eager or captured into a graph) that kept a record in
+ * each wave's LDS.
+ */
+int fgkl_batch_snapshots(const fgkl_batch* batch, int64_t* out4);
+int fgkl_plan_snapshots(int pd, int64_t n_regions, const int64_t* region_reads,
+                        const int64_t* region_haps, const int64_t* read_offsets,
+                        const int64_t* hap_offsets, const uint8_t* hap_bases,
+                        const uint8_t* hap_flags, int64_t* out3);
 
 /*
  * Launch autotuner (no reference counterpart: fgkl picks lanes per batch from the CPU's SIMD

@@ -1165,3 +1165,29 @@ def test_batch_prefix_sharing_only_moves_results_within_tolerance(gpu):
     unshared = b.results()
     assert_rel(shared, unshared, what="share vs no-share")
     b.close()
+
+
+@pytest.mark.parametrize("wl,regions", [(1, 6), (3, 4), (2, 2)])
+def test_snapshot_records_in_lds_give_the_bits_of_hbm_records(gpu, wl, regions):
+    """Prefix-sharing launches keep one snapshot record per wave in LDS where the plan marks one
+    and the record costs no resident workgroups.  The restored state, with the row above each
+    lane's first row moved down by DPP rather than stored, gives the bits of the HBM records in
+    both precisions (workload 2 reruns its dense reads in FP64 with prefix sharing)."""
+    data = workload(wl, seed=606 + wl, n_regions=regions)
+    b = F.Batch(data)
+    runs = {}
+    for lds in (True, False):
+        b.set_lds_records(lds)
+        before = b.snapshots()["lds_launches"]
+        for mode in (F.MODE_FP32_RESCUE, F.MODE_FP64):
+            b.run(mode, share=True)
+            runs[(lds, mode)] = b.results()
+        used = b.snapshots()["lds_launches"] - before
+        assert (used > 0) == lds, f"lds={lds}: {used} launches kept an LDS record"
+    assert b.snapshots()["lds"] > 0
+    for mode in (F.MODE_FP32_RESCUE, F.MODE_FP64):
+        assert np.array_equal(runs[(True, mode)], runs[(False, mode)]), f"mode {mode}"
+    expected = batch_reference(data)
+    assert_close(runs[(True, F.MODE_FP32_RESCUE)], expected, FLOAT_TOL, f"workload {wl} fp32")
+    assert_rel(runs[(True, F.MODE_FP64)], expected, what=f"workload {wl} fp64")
+    b.close()
