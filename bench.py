@@ -551,6 +551,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default launches")
     ap.add_argument("--launch", default="", metavar="R:ROWS:BLOCK:BOOST,...",
                     help="set these bucket launch configurations instead of tuning")
+    ap.add_argument("--no-lds-records", action="store_true",
+                    help="write every prefix-sharing snapshot record to HBM (same bits)")
     ap.add_argument("--no-side", action="store_true",
                     help="skip the rescue-stress and PD workloads")
     ap.add_argument("--no-sw", action="store_true", help="skip the Smith-Waterman workload")
@@ -607,6 +609,8 @@ def main() -> int:
     batch = F.Batch(data)
     prep_s = time.perf_counter() - t0
     log(f"planned + uploaded in {prep_s:.2f}s")
+    if args.no_lds_records:
+        batch.set_lds_records(False)
 
     # Launch autotuning per rows-per-lane bucket (untimed, like planning), or the configurations
     # given with --launch (so a profiling run can repeat a tuned run's kernels).
@@ -753,6 +757,14 @@ def main() -> int:
                                                / work["nominal"], 4),
                       "note": "cells computed per step: FP32 pass with prefix and suffix sharing "
                               "(forward + backward); value counts nominal cells"}
+    snaps = batch.snapshots()
+    result["snapshots"] = {
+        "hbm_records": snaps["hbm"], "lds_records": snaps["lds"], "restarts": snaps["restarts"],
+        "lds_record_launches": snaps["lds_launches"], "lds_records_enabled": not args.no_lds_records,
+        "note": "prefix-sharing snapshot records per FP32 pass (one per read and recorded "
+                "column): a sweep records a column only when it is the last to pass it before a "
+                "haplotype resumes there; one record per wave stays in LDS when its lifetime "
+                "allows and the launch holds it at the same occupancy, the rest go to HBM"}
     result["end_to_end"] = {"seconds": round(e2e_s, 4), "gcups": round(cells / e2e_s / 1e9, 2),
                             "includes": "plan on the host, upload, graph build + run, download"}
     log(f"end to end: {e2e_s * 1e3:.1f} ms, {cells / e2e_s / 1e9:.1f} GCUPS")

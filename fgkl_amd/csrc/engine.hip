@@ -261,7 +261,17 @@ struct DeviceBatch::Impl {
   hipEvent_t ev[4] = {};
   hipEvent_t bev[2 * kMaxRowsPerLane] = {};  // around each primary bucket launch
   hipGraphExec_t graph[8] = {};  // by mode * 2 + share
-  int lds_words = 0;
+  // LDS per wave, in words: the staged haplotype (the longest, plus the column a backward sweep
+  // adds past its end, the ramp of the widest read and a group of four read ahead), and with
+  // suffix sharing the join's FP64 row terms and chunk sums of up to 64 x rows rows, which reuse
+  // the staging area once the forward sweep is done.
+  int stage_hap_words = 0;
+  bool join_lds = false;
+  int stage_words(int rows) const {
+    int w = stage_hap_words;
+    if (join_lds) w = std::max(w, 2 * (64 * rows + 8 * rows + 2));
+    return (w + 3) & ~3;
+  }
   bool lds_records = true;  // prefix-sharing launches may keep a snapshot record in LDS
   int64_t lds_launches = 0;  // launches (eager or captured) that kept one
   LaunchCfg cfg[kMaxRowsPerLane];  // FP32 primary launch of each bucket
@@ -385,7 +395,7 @@ struct DeviceBatch::Impl {
     a.read_off = read_off; a.read_region = read_region; a.read_local = read_local;
     a.region_out = region_out; a.region_hap0 = region_hap0; a.region_nhap = region_nhap;
     a.plan = hplan; a.snaps = snaps; a.hap_bytes = hap_bytes; a.hap_words = hap_words;
-    a.scratch = scratch_bytes ? scratch : nullptr; a.out = out; a.lds_words = lds_words;
+    a.scratch = scratch_bytes ? scratch : nullptr; a.out = out;
     a.bsweeps = bsweeps; a.bwrites = bwrites; a.region_bw = region_bw;
     return a;
   }
@@ -396,7 +406,7 @@ struct DeviceBatch::Impl {
   int grid_waves(const LaunchCfg& c, int rows, int64_t max_items, int* grid, int wave_words = 0) {
     const int wpb = c.block / 64;
     int per_cu = 0;
-    const int words = wave_words ? wave_words : lds_words;
+    const int words = wave_words ? wave_words : stage_words(rows);
     if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, words * 4 * wpb, &per_cu))
       return rc;
     const int64_t want = (max_items + wpb - 1) / wpb;
@@ -416,15 +426,16 @@ struct DeviceBatch::Impl {
   // otherwise its records all go to HBM slots.  Sets the LDS fields of `a`.
   template <typename T>
   int lds_layout(SweepArgs& a, const LaunchCfg& c, int rows) {
-    a.lds_words = lds_words;
-    a.lds_rec = lds_words;
+    const int base = stage_words(rows);
+    a.lds_words = base;
+    a.lds_rec = base;
     a.lds_snap = 0;
     if (!lds_records || !a.share || a.pair_mode || plan.snap_lds == 0) return kOk;
     const int wpb = c.block / 64;
-    const int with = lds_words + lds_record_words<T>(rows);
+    const int with = base + lds_record_words<T>(rows);
     if (int64_t(with) * 4 * wpb > 64 * 1024) return kOk;
     int b0 = 0, b1 = 0;
-    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, lds_words * 4 * wpb, &b0)) return rc;
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, base * 4 * wpb, &b0)) return rc;
     if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, with * 4 * wpb, &b1)) return rc;
     if (b1 < b0) return kOk;
     a.lds_words = with;
@@ -798,12 +809,9 @@ int DeviceBatch::assign(const BatchInput& in) {
                                     " distinct haplotype prefix columns or shared suffixes");
     return kInvalidArgument;
   }
-  // LDS per wave: the longest haplotype (plus the column a backward sweep adds past its end) and
-  // the ramp of the widest read, in words.
-  // A join reduces up to 512 row terms and 66 chunk sums in FP64 there.
-  d.lds_words = ((p.max_hap_len + 1 + 64 + 8) + 3) & ~3;
-  if (!p.bsweeps.empty()) d.lds_words = std::max(d.lds_words, 2 * (512 + 72));
-  if (int64_t(d.lds_words) * 4 * (256 / 64) > 64 * 1024) {  // four waves per 256-thread block
+  d.stage_hap_words = p.max_hap_len + 1 + 64 + 8;
+  d.join_lds = !p.bsweeps.empty();
+  if (int64_t(d.stage_words(kMaxRowsPerLane)) * 4 * (256 / 64) > 64 * 1024) {  // 4 waves a block
     set_error(kInvalidArgument, "haplotype of " + std::to_string(p.max_hap_len) +
                                     " bases exceeds the engine's LDS staging capacity");
     return kInvalidArgument;

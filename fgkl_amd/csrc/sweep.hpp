@@ -732,7 +732,6 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
   T* lrec = reinterpret_cast<T*>(lw + a.lds_rec);
   T* scratch = a.scratch ? static_cast<T*>(a.scratch) + wave_global * a.scratch_wave_stride : nullptr;
   const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
-  constexpr int NV = snap_values<R, PD>();
 
   for (;;) {
     // Every lane adds one, so a claim advances the counter by 64 and every lane's old value lies
@@ -897,7 +896,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
             const uint32_t hc = uint32_t(a.hap_bytes[p.off + p.cut - 1]);
             const T* rec = nodes + (size_t(p.cut_node) * 64 + lane) * node_stride<R>();
             double* rows = reinterpret_cast<double*>(lw);   // [na + 1 + nb + 1]
-            double* chunks = rows + 2 * kMaxRowsPerLane * 64 / 2;  // after 512 rows
+            double* chunks = rows + 64 * R;  // after the 64 x R rows a wave holds
             const int base = in_b ? na + 1 : 0;
             const int first = R * (in_b ? lane - La : lane) - (in_b ? padb : pada);  // row - 1
             wave_sync_lds();
