@@ -215,6 +215,7 @@ struct DeviceBatch::Impl {
   SnapEntry* snaps = nullptr;
   uint8_t* hap_bytes = nullptr;
   uint32_t* hap_words = nullptr;
+  uint8_t* region_classes = nullptr;  // PD: read-base classes from 4 on, per region
   int32_t* bucket_items[kMaxRowsPerLane] = {};
   int32_t bucket_n[kMaxRowsPerLane] = {};
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
@@ -395,6 +396,7 @@ struct DeviceBatch::Impl {
     a.read_off = read_off; a.read_region = read_region; a.read_local = read_local;
     a.region_out = region_out; a.region_hap0 = region_hap0; a.region_nhap = region_nhap;
     a.plan = hplan; a.snaps = snaps; a.hap_bytes = hap_bytes; a.hap_words = hap_words;
+    a.region_classes = region_classes;
     a.scratch = scratch_bytes ? scratch : nullptr; a.out = out;
     a.bsweeps = bsweeps; a.bwrites = bwrites; a.region_bw = region_bw;
     return a;
@@ -740,9 +742,11 @@ int DeviceBatch::assign(const BatchInput& in) {
   d.place(&d.region_bw, p.region_bw.data(), p.region_bw.size());
   d.hap_words = nullptr;
   d.hap_bytes = nullptr;
-  if (in.pd)
+  d.region_classes = nullptr;
+  if (in.pd) {
     d.place(&d.hap_words, p.words.data(), p.words.size());
-  else
+    d.place(&d.region_classes, p.region_classes.data(), p.region_classes.size());
+  } else
     d.place(&d.hap_bytes, in.hap_bases, hb);
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
     const auto& items = p.bucket_items[R - 1];

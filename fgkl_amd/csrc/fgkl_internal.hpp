@@ -51,6 +51,40 @@ inline uint32_t pd_word(uint8_t base, uint8_t flag, uint32_t st1, uint32_t st2) 
          (uint32_t((flag & kDelEnd) != 0) << kDelEndBit) | (1u << kIdentShift);
 }
 
+// A column of a PD region the staged sweep runs carries, in bits 0-11 of its word instead of the
+// haplotype byte and alternates, the read-base classes it accepts as a match (MatchSet::accepts,
+// pd_kernel.rs:196-214): bit c for class c, where A, C, G, T are classes 0-3 and each other byte
+// the region's reads hold (lowercase bases and any other byte) takes one of kPdExtraClasses
+// classes from 4 on, in order of first appearance.  A read 'N' takes class 0: its match and
+// mismatch priors are equal.  A region whose reads hold more other bytes runs in long_kernel,
+// whose words keep the haplotype byte and alternates and whose cells compare bytes.
+constexpr int kPdExtraClasses = 8;
+inline uint32_t pd_alt_bit(uint8_t b) {  // pd.rs:99-110
+  switch (b) {
+    case 'A': case 'a': return 1u;
+    case 'C': case 'c': return 2u;
+    case 'G': case 'g': return 4u;
+    case 'T': case 't': return 8u;
+    default: return 0u;
+  }
+}
+inline bool pd_accepts(uint8_t read_base, uint8_t hap_base, uint8_t flag) {
+  const uint32_t alts = (flag & kSnp) ? ((flag >> 3) & 0xFu) : 0u;
+  return read_base == hap_base || read_base == 'N' || hap_base == 'N' ||
+         (alts & pd_alt_bit(read_base)) != 0u;
+}
+inline uint32_t pd_class_mask(uint8_t hap_base, uint8_t flag, const uint8_t* extra, int n_extra) {
+  static constexpr uint8_t kStd[4] = {'A', 'C', 'G', 'T'};
+  uint32_t m = 0;
+  for (int c = 0; c < 4; ++c) m |= uint32_t(pd_accepts(kStd[c], hap_base, flag)) << c;
+  for (int c = 0; c < n_extra; ++c) m |= uint32_t(pd_accepts(extra[c], hap_base, flag)) << (4 + c);
+  return m;
+}
+inline uint32_t pd_word_classes(uint32_t classes, uint8_t flag, uint32_t st1, uint32_t st2) {
+  return classes | (st1 << kRow1Shift) | (st2 << kRowShift) |
+         (uint32_t((flag & kDelEnd) != 0) << kDelEndBit) | (1u << kIdentShift);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Plan of one batch: haplotypes sorted per region for prefix sharing (kernel.rs:53-81,
 // pd_kernel.rs:61-136) and the snapshot columns each sorted haplotype must record
@@ -134,6 +168,10 @@ struct HostPlan {
   std::vector<HapPlan> plan;        // n_haps
   std::vector<SnapEntry> snaps;
   std::vector<uint32_t> words;      // PD column words (same offsets as the haplotype bytes)
+  // PD: per region, the bytes of read-base classes 4 on (kPdExtraClasses each, zero-padded), and
+  // whether the region runs in long_kernel (too many classes, or a read or haplotype too long).
+  std::vector<uint8_t> region_classes;
+  std::vector<char> region_pd_long;
   int32_t max_slots = 0;
   int32_t max_hap_len = 0;
   // Suffix sharing (plain batches only): the backward sweeps of region g are

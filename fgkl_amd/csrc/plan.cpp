@@ -347,19 +347,51 @@ int build_plan(const BatchInput& in, HostPlan& p) {
   std::vector<uint32_t> end_state(in.pd ? size_t(NH) : 0);
   if (in.pd) {
     p.words.resize(size_t(in.hap_off[NH]));
-    const int nt = host_threads(NH, 2048);
+    p.region_classes.assign(size_t(G) * kPdExtraClasses, 0);
+    p.region_pd_long.assign(size_t(G), 0);
+    const int nt = host_threads(G, 64);
     run_parallel(nt, [&](int t) {
       std::vector<uint32_t> st1, st2;
-      for (int64_t h = NH * t / nt; h < NH * (t + 1) / nt; ++h) {
-        const int64_t o = in.hap_off[h], m = in.hap_off[h + 1] - o;
-        st1.resize(size_t(m));
-        st2.resize(size_t(m));
-        const uint32_t e = column_states(in.hap_flags + o, m, kNormal, st1.data());
-        column_states(in.hap_flags + o, m, e, st2.data());
-        end_state[size_t(h)] = e;
-        for (int64_t j = 0; j < m; ++j)
-          p.words[size_t(o + j)] =
-              pd_word(in.hap_bases[o + j], in.hap_flags[o + j], st1[size_t(j)], st2[size_t(j)]);
+      for (int64_t g = G * t / nt; g < G * (t + 1) / nt; ++g) {
+        // The region's read-base classes beyond A, C, G, T (none when the read bytes are not
+        // given: a host-only plan needs no words).
+        uint8_t* extra = &p.region_classes[size_t(g) * kPdExtraClasses];
+        int n_extra = 0;
+        bool is_long = false;
+        bool seen[256] = {};
+        seen[uint8_t('A')] = seen[uint8_t('C')] = seen[uint8_t('G')] = seen[uint8_t('T')] =
+            seen[uint8_t('N')] = true;
+        for (int64_t r = in.region_read[g]; r < in.region_read[g + 1]; ++r) {
+          is_long |= in.read_off[r + 1] - in.read_off[r] > kMaxReadLen;
+          if (!in.bases) continue;
+          for (int64_t i = in.read_off[r]; i < in.read_off[r + 1]; ++i) {
+            const uint8_t b = in.bases[i];
+            if (seen[b]) continue;
+            seen[b] = true;
+            if (n_extra < kPdExtraClasses)
+              extra[n_extra++] = b;
+            else
+              is_long = true;
+          }
+        }
+        for (int64_t h = in.region_hap[g]; h < in.region_hap[g + 1]; ++h)
+          is_long |= in.hap_off[h + 1] - in.hap_off[h] > kMaxStagedHap;
+        p.region_pd_long[size_t(g)] = char(is_long);
+        for (int64_t h = in.region_hap[g]; h < in.region_hap[g + 1]; ++h) {
+          const int64_t o = in.hap_off[h], m = in.hap_off[h + 1] - o;
+          st1.resize(size_t(m));
+          st2.resize(size_t(m));
+          const uint32_t e = column_states(in.hap_flags + o, m, kNormal, st1.data());
+          column_states(in.hap_flags + o, m, e, st2.data());
+          end_state[size_t(h)] = e;
+          for (int64_t j = 0; j < m; ++j) {
+            const uint8_t b = in.hap_bases[o + j], f = in.hap_flags[o + j];
+            p.words[size_t(o + j)] =
+                is_long ? pd_word(b, f, st1[size_t(j)], st2[size_t(j)])
+                        : pd_word_classes(pd_class_mask(b, f, extra, n_extra), f, st1[size_t(j)],
+                                          st2[size_t(j)]);
+          }
+        }
       }
     });
   }
@@ -508,6 +540,7 @@ int build_plan(const BatchInput& in, HostPlan& p) {
       for (int64_t r = r0; r < r1; ++r)
         is_long |= in.read_off[r + 1] - in.read_off[r] > kMaxReadLen;
       for (int64_t h = h0; h < h1; ++h) is_long |= rh.len(h) > kMaxStagedHap;
+      if (in.pd) is_long |= p.region_pd_long[size_t(g)] != 0;  // also too many read classes
       for (int64_t r = r0; r < r1; ++r) {
         p.read_region[size_t(r)] = int32_t(g);
         p.read_local[size_t(r)] = int32_t(r - r0);

@@ -74,6 +74,8 @@ struct SweepArgs {
   const SnapEntry* snaps;
   const uint8_t* hap_bytes;
   const uint32_t* hap_words;
+  // PD: the bytes of each region's read-base classes from 4 on (kPdExtraClasses per region).
+  const uint8_t* region_classes;
   const int32_t* items;        // (read, read or -1) per item, or (read, sorted hap) in pair mode
   int32_t n_items;
   const int32_t* n_items_dev;  // when set, the item count is read from device memory
@@ -131,14 +133,35 @@ template <typename T, int R, bool PD> struct Rows {
   T pm[R], px[R], mm[R], im[R], mi[R], md[R], gc[R];
   uint32_t rb[R];
   uint32_t ralt[R];
+  // PD, staged sweep: the row's read-base class, the bit of the column word that says whether
+  // the column accepts the row's base.
+  uint32_t ci[PD ? R : 1];
   // Where the row's deletion-state bits sit in a PD column word: kRowShift (rows >= 2),
   // kRow1Shift (row 1) or kIdentShift (identity rows and lanes past the read).
   uint32_t sh[R];
 };
 
+// The read-base class of a read byte in a region whose other classes are `extra` (see
+// pd_class_mask): A, C, G, T are 0-3, 'N' takes 0, the other bytes 4 on.
+__device__ __forceinline__ uint32_t read_class(uint32_t b, const uint8_t* extra) {
+  switch (b) {
+    case 'A': case 'N': return 0u;
+    case 'C': return 1u;
+    case 'G': return 2u;
+    case 'T': return 3u;
+    default: break;
+  }
+  uint32_t c = 0u;
+  if (extra)
+    for (int i = 0; i < kPdExtraClasses; ++i) c = extra[i] == b && c == 0u ? 4u + i : c;
+  return c;
+}
+
+// `extra` (PD, staged sweep) is the region's read-base classes from 4 on, or null.
 template <typename T, int R, bool PD>
 __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& tab, int64_t o,
-                                          int L, int pad, int lane, Rows<T, R, PD>& rw) {
+                                          int L, int pad, int lane, Rows<T, R, PD>& rw,
+                                          const uint8_t* extra = nullptr) {
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int arow = R * lane + k - pad;
@@ -156,6 +179,7 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       rw.im[k] = tab.im[gq];
       rw.rb[k] = b;
       rw.ralt[k] = alt_bit(b);
+      if constexpr (PD) rw.ci[k] = read_class(b, extra);
       rw.sh[k] = arow == 0 ? kRow1Shift : kRowShift;
     } else {
       // Identity rows: D->D 1 keeps row 0's C/m; I->I 1 is harmless because the row above an
@@ -165,6 +189,7 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       rw.gc[k] = lane < L ? T(1) : T(0);
       rw.rb[k] = 0x100u;
       rw.ralt[k] = 0u;
+      if constexpr (PD) rw.ci[k] = 0u;
       rw.sh[k] = kIdentShift;
     }
   }
@@ -261,7 +286,9 @@ __device__ __forceinline__ double max_nonneg(double a, double b) { return fmax(a
 // read other mask bits but hold zero branch values (or have a zero prior), where the masks do
 // not matter; so are all values of column 0, where a sweep from the first column starts.  The
 // sweep forms the diagonal maxima of its first column afresh.
-template <typename T, int R, bool PD, bool CARRY = false>
+// CLS (PD): the column word holds read-base classes (pd_class_mask) and a row's prior is chosen
+// by its class bit; otherwise it holds the haplotype byte and alternates and bytes are compared.
+template <typename T, int R, bool PD, bool CARRY = false, bool CLS = false>
 __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
                                       const Cell<T>& d, const Cell<T>& u) {
   const uint32_t w = s.w;
@@ -281,9 +308,14 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
     // Bitwise ORs, no short circuit: a branch around the alternate-base test (a load) would turn
     // the prior below into a phi of two loads, which the optimiser folds into one load at a
     // selected offset of the row table, keeping the table in private memory.
-    bool match = hN | (hb == rw.rb[k]);
-    if constexpr (PD) match = match | (((w >> 8) & rw.ralt[k]) != 0u);
-    const T prior = match ? rw.pm[k] : rw.px[k];
+    T prior;
+    if constexpr (PD && CLS) {
+      prior = sel_bits(bit_mask(w, rw.ci[k]), rw.pm[k], rw.px[k]);
+    } else {
+      bool match = hN | (hb == rw.rb[k]);
+      if constexpr (PD) match = match | (((w >> 8) & rw.ralt[k]) != 0u);
+      prior = match ? rw.pm[k] : rw.px[k];
+    }
     if constexpr (!PD) {
       // kernel.rs:1011-1013
       const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[k]);
@@ -478,7 +510,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells<T, R, PD, PD>(s, rw, P, Q);
+      cells<T, R, PD, PD, PD>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -491,26 +523,26 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
     s.w = next_word(cq.x, s.w, head);
     shift_up(s, Q, keep);
-    cells<T, R, PD, PD>(s, rw, P, Q);
+    cells<T, R, PD, PD, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.y, s.w, head);
     shift_up(s, P, keep);
-    cells<T, R, PD, PD>(s, rw, Q, P);
+    cells<T, R, PD, PD, PD>(s, rw, Q, P);
     record(P);
     s.w = next_word(cq.z, s.w, head);
     shift_up(s, Q, keep);
-    cells<T, R, PD, PD>(s, rw, P, Q);
+    cells<T, R, PD, PD, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.w, s.w, head);
     shift_up(s, P, keep);
-    cells<T, R, PD, PD>(s, rw, Q, P);
+    cells<T, R, PD, PD, PD>(s, rw, Q, P);
     record(P);
   }
   for (; t < steps; ++t) {
     s.w = next_word(lw[t], s.w, head);
     shift_up(s, Q, keep);
     if (t >= pos && t - pos < W) {
-      cells<T, R, PD, PD>(s, rw, P, Q);
+      cells<T, R, PD, PD, PD>(s, rw, P, Q);
       P = Q;
       record(P);
     }
@@ -776,7 +808,8 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     const bool in_b = lane >= La;
     Rows<T, R, PD> rw;
     load_rows<T, R, PD>(a, tab, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
-                        in_b ? lane - La : lane, rw);
+                        in_b ? lane - La : lane, rw,
+                        PD ? a.region_classes + int64_t(g) * kPdExtraClasses : nullptr);
     // Each read's second lane starts below DP row 0 when identity rows fill its first lane.
     const bool row0_above = (lane == 1 && pada == R) || (two && lane == La + 1 && padb == R);
     // Zero in the first lane of read B, all ones elsewhere; opaque to the optimiser (see

@@ -405,6 +405,57 @@ def test_pd_flags_on_the_last_columns_carry_their_state_into_the_next_row(gpu):
     assert_close(out, expected, FLOAT_TOL, "pd last columns float")
 
 
+def _pd_class_region(rng, others: bytes):
+    """A region whose reads and haplotypes mix upper- and lowercase bases, 'N' and the bytes in
+    `others`, against SNP alternates (some lowercase-accepting) and deletion runs."""
+    n = 160
+    hap = bytearray(b"ACGT"[rng.below(4)] for _ in range(n))
+    for j in range(0, n, 11):
+        hap[j] = hap[j] | 0x20  # lowercase
+    for j in (7, 50, 93):
+        hap[j] = ord("N")
+    for j, b in zip(range(20, n, 23), others):
+        hap[j] = b
+    flags = [0] * n
+    for j in range(5, n, 13):
+        flags[j] = F.SNP | (F.ALT_A, F.ALT_C, F.ALT_G, F.ALT_T)[j % 4] | (F.ALT_C if j % 3 else 0)
+    for j in (30, 70, 120):
+        flags[j] |= F.DEL_START
+        flags[j + 2] |= F.DEL_END
+    haps = [F.PdHaplotype(bytes(hap), flags),
+            F.PdHaplotype(bytes(hap).upper(), [f & ~F.SNP for f in flags])]
+    reads = []
+    for k in range(10):
+        lo = rng.below(n - 60)
+        rb = bytearray(hap[lo:lo + 40 + rng.below(20)])
+        for i in range(len(rb)):
+            u = rng.below(20)
+            if u == 0:
+                rb[i] = ord("N")
+            elif u == 1:
+                rb[i] = rb[i] ^ 0x20 if chr(rb[i]).isalpha() else rb[i]
+            elif u == 2 and others:
+                rb[i] = others[rng.below(len(others))]
+        reads.append(read(bytes(rb)))
+    return reads, haps
+
+
+@pytest.mark.parametrize("others", [b"", b"acgtR*", b"acgtnRYKM\x07"])
+def test_pd_read_base_classes_match_the_reference_for_any_byte(gpu, others):
+    """A staged PD column carries the classes of read bases it accepts (A, C, G, T and up to
+    eight other bytes of the region's reads; MatchSet::accepts, pd_kernel.rs:196-214): lowercase
+    read bases against SNP alternates and the haplotype's own lowercase bytes, 'N' on either side,
+    other bytes that match only themselves and 'N'.  Nine other bytes send the region to the
+    long kernel, which compares bytes.  Both precisions against the oracle, per call and batched
+    with prefix sharing."""
+    reads, haps = _pd_class_region(Rng(77 + len(others)), others)
+    expected = pd_reference_all(reads, haps)
+    out, _ = pd_compute(F.Precision.DOUBLE, reads, haps)
+    assert_rel(out, expected, what=f"pd classes double {others!r}")
+    out, _ = pd_compute(F.Precision.FLOAT, reads, haps)
+    assert_close(out, expected, FLOAT_TOL, f"pd classes float {others!r}")
+
+
 def test_pd_carried_deletion_state_keeps_later_rows_from_skipping(gpu):
     """The oracle's hand-worked carry case (haplotype GCTT, DEL_END on column 3, DEL_START on
     column 4, read GCT): rows 2-3 start inside a deletion, so the skip of column 3 does not count
