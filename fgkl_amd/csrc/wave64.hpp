@@ -80,10 +80,15 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Makes a wave's global stores visible to its other lanes' later loads (the wave's CU shares one
-// vector L1, so workgroup scope suffices).
+// Makes a wave's stores visible to the loads it issues next, in any lane: waits until its vector
+// memory and LDS operations, stores included, have completed (s_waitcnt vmcnt(0) lgkmcnt(0)), so
+// a record written through an LDS pointer and read back through a generic one is there too.  The
+// wave's CU shares one vector L1, so no cache maintenance is needed; a workgroup-scope fence
+// alone emits no wait on gfx950, and the fences keep the compiler from moving memory operations
+// across.
 __device__ __forceinline__ void wave_sync_global() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0); expcnt unconstrained (gfx9 form)
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
