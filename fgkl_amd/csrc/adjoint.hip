@@ -20,12 +20,14 @@
 //    0).  Gradient terms are summed per block in the working precision and accumulated in FP64
 //    registers per row, written once per strip.
 //  * Two precisions.  FP64: C = 2^1020, K = 8, every pair.  FP32 (the forward, the tape and the
-//    adjoint): C = 2^120, K = 20, the adjoint seeded with w / L so that it is d ln L / d cell
+//    adjoint): C = 2^120, K = 16, the adjoint seeded with w / L so that it is d ln L / d cell
 //    (its values then stay within FP32's normal range from the last row to the first); a pair
 //    whose FP32 likelihood falls below 1e-28 (kernel.rs:897-914's rescue threshold) is flagged,
 //    and the FP64 kernel reruns the flagged pairs of each read and adds their terms.
 //  * Checkpoint memory is n/64 strips x (m+63)/K blocks x 6 values per lane; the tape is K+1
-//    cells per lane in registers.  Work: forward twice plus the reverse sweep.
+//    cells per lane in registers, and the K rows above that the recompute moved in (the reverse
+//    sweep's diagonal and up values, read back instead of moved again).  Work: forward twice
+//    plus the reverse sweep.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,13 +52,16 @@ template <> struct GT<double> {
   static constexpr double kMinAccepted = 0.0;
   static constexpr int kTape = 8;
   static constexpr bool kPrefetch = false;  // its registers would spill
+  static constexpr bool kTapeUp = false;    // likewise: the reverse sweep moves the rows again
 };
 template <> struct GT<float> {
   static constexpr double kInit = 0x1p120;
   static constexpr double kLog10Init = 120.0 * 0.301029995663981195213738894724493027;
   static constexpr double kMinAccepted = 1e-28;
-  static constexpr int kTape = 20;  // about sqrt(m) for haplotypes of a few hundred bases
+  static constexpr int kTape = 16;  // about sqrt(m) for haplotypes of a few hundred bases; the
+                                    // fastest with the row-above tape (measured 10..20)
   static constexpr bool kPrefetch = true;   // the next block's inputs load during this block
+  static constexpr bool kTapeUp = true;     // the recompute keeps the rows above (see fblock)
 };
 
 constexpr double kInvLn10 = 0.434294481903251827651128918916605082;
@@ -221,12 +226,14 @@ __device__ __forceinline__ T prior_of(const Lds<T>& l, int c, const GRow<T>& rw)
 
 // K forward steps from `st` (the state before step t0) with diagonal `P`.  FM freezes lanes
 // that have not reached column 1; ACC adds M + X of columns 1..m; OUT stages lane 63's state.
-// tp[k] receives the state before step k (TAPE).
+// With TAPE, tp[k] receives the state before step k and, when GT<T>::kTapeUp, tq[k] the row
+// above at step k (the next step's diagonal), which the reverse sweep then reads instead of
+// moving them again.
 // MID: the strip holds the first row of a second read (see row0_above).
 template <typename T, bool FM, bool ACC, bool OUT, bool TAPE, bool MID>
-__device__ __forceinline__ void fblock(St<T>& st, St<T>& P, T& acc, St<T>* tp, const GRow<T>& rw,
-                                       const Lds<T>& l, St4<T>* out, int t0, int lane, int m,
-                                       uint32_t keep_f, T init) {
+__device__ __forceinline__ void fblock(St<T>& st, St<T>& P, T& acc, St<T>* tp, St<T>* tq,
+                                       const GRow<T>& rw, const Lds<T>& l, St4<T>* out, int t0,
+                                       int lane, int m, uint32_t keep_f, T init) {
   constexpr int K = GT<T>::kTape;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -234,6 +241,7 @@ __device__ __forceinline__ void fblock(St<T>& st, St<T>& P, T& acc, St<T>* tp, c
     const int c = t0 + k - lane + 1;
     St<T> Q = shr_st(l.stage[k + 1], st);
     if constexpr (MID) row0_above(Q, keep_f, init);
+    if constexpr (TAPE && GT<T>::kTapeUp) tq[k] = Q;
     const T prior = prior_of(l, c, rw);
     const T Y = fma(st.y, rw.g, st.m * rw.md);
     const T M = prior * fma(P.y, rw.ig, fma(P.x, rw.ig, P.m * rw.mm));
@@ -316,11 +324,11 @@ __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw
     const bool fm = t0 < L - 1;
     if (has_next) {
       if (fm)
-        fblock<T, true, ACC, true, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
-                                               keep_f, init);
+        fblock<T, true, ACC, true, false, MID>(st, P, acc, nullptr, nullptr, rw, l, out,
+                                               t0, lane, m, keep_f, init);
       else
-        fblock<T, false, ACC, true, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
-                                                keep_f, init);
+        fblock<T, false, ACC, true, false, MID>(st, P, acc, nullptr, nullptr, rw, l, out,
+                                                t0, lane, m, keep_f, init);
       wave_sync_lds();
       // Lanes 0..K-1 store lane 63's columns t0+lane-62 that lie in 1..m.
       const int c = t0 + lane - 62;
@@ -330,11 +338,11 @@ __device__ T forward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>& rw
       *dst = St4<T>{w.m, w.x, w.y, T(0)};
     } else {
       if (fm)
-        fblock<T, true, ACC, false, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
-                                                keep_f, init);
+        fblock<T, true, ACC, false, false, MID>(st, P, acc, nullptr, nullptr, rw, l, out,
+                                                t0, lane, m, keep_f, init);
       else
-        fblock<T, false, ACC, false, false, MID>(st, P, acc, nullptr, rw, l, out, t0, lane, m,
-                                                 keep_f, init);
+        fblock<T, false, ACC, false, false, MID>(st, P, acc, nullptr, nullptr, rw, l, out,
+                                                 t0, lane, m, keep_f, init);
     }
   }
   return acc;
@@ -349,13 +357,22 @@ template <typename T> struct GSum {
 // The reverse sweep of one block over the tape.  BM zeroes the adjoint outside columns 1..m;
 // SEED adds `seedv` (non-zero only in the read's last row), which only the bottom strip holds.
 template <typename T, bool BM, bool SEED, bool MID>
-__device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const GRow<T>& rw,
-                                       const Lds<T>& l, Adj2<T>* aout, int t0, int lane, int m,
-                                       T seedv, T& AX, T& AY, T& Q, T& Bq, GSum<T>& gs,
-                                       uint32_t keep_f, uint32_t keep_r, T init) {
+__device__ __forceinline__ void rblock(const St<T>* tp, const St<T>* tq, const St<T>& P0,
+                                       const GRow<T>& rw, const Lds<T>& l, Adj2<T>* aout, int t0,
+                                       int lane, int m, T seedv, T& AX, T& AY, T& Q, T& Bq,
+                                       GSum<T>& gs, uint32_t keep_f, uint32_t keep_r, T init) {
   constexpr int K = GT<T>::kTape;
-  St<T> U = shr_st(l.stage[K], tp[K - 1]);  // up values of the last step
-  if constexpr (MID) row0_above(U, keep_f, init);
+  // The row above at step k: from the tape, or moved again from the state before step k.
+  auto up_at = [&](int k) {
+    if constexpr (GT<T>::kTapeUp) {
+      return tq[k];
+    } else {
+      St<T> u = shr_st(l.stage[k + 1], tp[k]);
+      if constexpr (MID) row0_above(u, keep_f, init);
+      return u;
+    }
+  };
+  St<T> U = up_at(K - 1);  // up values of the last step
 #pragma unroll
   for (int k = K - 1; k >= 0; --k) {
     const int col = t0 + k - lane + 1;
@@ -372,10 +389,7 @@ __device__ __forceinline__ void rblock(const St<T>* tp, const St<T>& P0, const G
     }
     const T Qd = Bq;
     Bq = Qn;
-    St<T> D = k ? shr_st(l.stage[k], tp[k - 1]) : P0;
-    if constexpr (MID) {
-      if (k) row0_above(D, keep_f, init);
-    }
+    const St<T> D = k ? up_at(k - 1) : P0;  // the diagonal: the row above one step earlier
     const St<T>& Lf = tp[k];
     const T pr = Qd * rw.ig;
     T ay = fma(AY, rw.g, pr);
@@ -463,24 +477,24 @@ __device__ void backward_strip(const GradArgs& a, const Lds<T>& l, const GRow<T>
     put_above(l, cur.above, lane);
     wave_sync_lds();
     // Recompute the block from its checkpoint into the tape: tp[k] is the state before step k.
-    St<T> tp[K + 1];
+    St<T> tp[K + 1], tq[GT<T>::kTapeUp ? K : 1];
     St<T> st{cur.c[0], cur.c[1], cur.c[2]};
     const St<T> P0{cur.c[3], cur.c[4], cur.c[5]};
     St<T> P = P0;
     T unused = T(0);
     if (t0 < L - 1)
-      fblock<T, true, false, false, true, MID>(st, P, unused, tp, rw, l, nullptr, t0, lane, m,
-                                               keep_f, init);
+      fblock<T, true, false, false, true, MID>(st, P, unused, tp, tq, rw, l, nullptr, t0, lane,
+                                               m, keep_f, init);
     else
-      fblock<T, false, false, false, true, MID>(st, P, unused, tp, rw, l, nullptr, t0, lane, m,
-                                                keep_f, init);
+      fblock<T, false, false, false, true, MID>(st, P, unused, tp, tq, rw, l, nullptr, t0, lane,
+                                                m, keep_f, init);
     GSum<T> gs{T(0), T(0), T(0), T(0), T(0), T(0)};
     if (t0 < L - 1 || t0 + K > m)
-      rblock<T, true, SEED, MID>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs,
+      rblock<T, true, SEED, MID>(tp, tq, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs,
                                  keep_f, keep_r, init);
     else
-      rblock<T, false, SEED, MID>(tp, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq, gs,
-                                  keep_f, keep_r, init);
+      rblock<T, false, SEED, MID>(tp, tq, P0, rw, l, aout, t0, lane, m, seedv, AX, AY, Q, Bq,
+                                  gs, keep_f, keep_r, init);
     ge += double(gs.ge);
     gi += double(gs.gmi - gs.gmm);
     gd += double(gs.gmd - gs.gmm);
