@@ -246,11 +246,13 @@ struct DeviceBatch::Impl {
   void* scratch = nullptr;
   size_t scratch_bytes = 0, scratch_cap = 0;
   // Batches of up to kMaxPairList pairs (the per-call entry points, aggregated calls): every
-  // (read, sorted haplotype) pair grouped by the read's rows per lane (bucket R from
-  // all_off[R - 1]), their FP32 rescue list, and the largest rows per lane.
+  // (read, sorted haplotype) pair as pair-mode items grouped by the read's rows per lane (bucket
+  // R from all_off[R - 1]; an item may carry two reads of a region, see pair_items), their FP32
+  // rescue list, and the largest rows per lane.
   int32_t* all_pairs = nullptr;
   int32_t* all_rescue = nullptr;
-  int64_t n_all_pairs = 0;
+  int64_t n_all_pairs = 0;  // pairs
+  int64_t n_all_items = 0;  // items
   int64_t all_off[kMaxRowsPerLane + 1] = {};
   int all_rows = 0;
   bool pairs_only = false;  // planned for run_latency only
@@ -697,6 +699,48 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
   return kOk;
 }
 
+// Pair-mode items of the reads whose own rows per lane is `R`, for a launch at `run_rows` rows
+// per lane: (read, sorted haplotype k) for every haplotype of the read's region, or, for two
+// reads a < b of one region whose lanes fit one wave together at run_rows, one item per
+// haplotype carrying both as (a, k | (b - a) << 16 | sign bit).  Such an item is the main pass's
+// two-read wave on one haplotype, and a read gets the same bits whatever shares its wave.  Reads
+// pair longest with shortest among those that fit.
+void pair_items(const BatchInput& in, const HostPlan& p, int R, int run_rows,
+                std::vector<int32_t>& all) {
+  const int64_t NR = int64_t(p.read_rows.size());
+  for (int64_t g = 0; g < int64_t(p.region_nhap.size()); ++g) {
+    const int32_t H = p.region_nhap[size_t(g)];
+    std::vector<std::pair<int, int64_t>> reads;  // (lanes at run_rows, read)
+    for (int64_t r = in.region_read[g]; r < in.region_read[g + 1] && r < NR; ++r) {
+      if (p.read_rows[size_t(r)] != R) continue;
+      const int n = int(in.read_off[r + 1] - in.read_off[r]);
+      reads.push_back({(n + run_rows) / run_rows, r});
+    }
+    std::sort(reads.begin(), reads.end(), [](const auto& x, const auto& y) {
+      return x.first != y.first ? x.first > y.first : x.second < y.second;
+    });
+    size_t i = 0, j = reads.size();
+    while (i < j) {
+      int64_t a = reads[i].second, b = -1;
+      if (j - i >= 2 && reads[i].first + reads[j - 1].first <= 64 && H <= 0x10000) {
+        b = reads[j - 1].second;
+        --j;
+        if (b < a) std::swap(a, b);
+        if (b - a >= 0x8000) {  // too far apart to encode: both alone
+          ++j;
+          b = -1;
+          a = reads[i].second;
+        }
+      }
+      ++i;
+      for (int32_t k = 0; k < H; ++k) {
+        all.push_back(int32_t(a));
+        all.push_back(b < 0 ? k : int32_t(uint32_t(k) | uint32_t(b - a) << 16 | 0x80000000u));
+      }
+    }
+  }
+}
+
 int DeviceBatch::assign(const BatchInput& in) {
   Impl& d = *impl_;
   // Work queued with the previous inputs finishes before their memory is reused.
@@ -785,28 +829,30 @@ int DeviceBatch::assign(const BatchInput& in) {
   d.place(&d.long_pairs, p.long_pairs.data(), p.long_pairs.size());
   d.place(&d.long_rescue, static_cast<const int32_t*>(nullptr), p.long_pairs.size());
   std::vector<int32_t> all;
-  d.n_all_pairs = 0;
+  d.n_all_pairs = d.n_all_items = 0;
   d.all_rows = 0;
   d.all_pairs = d.all_rescue = nullptr;
   std::fill(std::begin(d.all_off), std::end(d.all_off), 0);
   if (p.n_pairs <= kMaxPairList) {
-    all.reserve(size_t(2 * p.n_pairs));
+    int64_t n_pairs = 0;
+    for (int64_t r = 0; r < NR; ++r)
+      if (p.read_rows[size_t(r)] > 0) {
+        d.all_rows = std::max(d.all_rows, int(p.read_rows[size_t(r)]));
+        n_pairs += p.region_nhap[size_t(p.read_region[size_t(r)])];
+      }
+    d.n_all_pairs = n_pairs;
+    // One launch runs every item at the largest rows per lane (run_latency); otherwise each
+    // bucket runs at its own.
+    const bool one_launch = n_pairs <= kOneLaunchPairs;
+    all.reserve(size_t(2 * n_pairs));
     for (int R = 1; R <= kMaxRowsPerLane; ++R) {
       d.all_off[R - 1] = int64_t(all.size() / 2);
-      for (int64_t r = 0; r < NR; ++r) {
-        if (p.read_rows[size_t(r)] != R) continue;
-        d.all_rows = R;
-        const int32_t H = p.region_nhap[size_t(p.read_region[size_t(r)])];
-        for (int32_t k = 0; k < H; ++k) {
-          all.push_back(int32_t(r));
-          all.push_back(k);
-        }
-      }
+      pair_items(in, p, R, one_launch ? d.all_rows : R, all);
     }
-    d.n_all_pairs = int64_t(all.size() / 2);
-    d.all_off[kMaxRowsPerLane] = d.n_all_pairs;
+    d.n_all_items = int64_t(all.size() / 2);
+    d.all_off[kMaxRowsPerLane] = d.n_all_items;
     d.place(&d.all_pairs, all.data(), all.size());
-    d.place(&d.all_rescue, static_cast<const int32_t*>(nullptr), all.size());
+    d.place(&d.all_rescue, static_cast<const int32_t*>(nullptr), size_t(2 * n_pairs));
   }
   if (p.max_slots > kMaxSlots || p.max_nodes > kMaxSlots) {
     set_error(kInvalidArgument, "a region holds more than " + std::to_string(kMaxSlots) +
@@ -1077,7 +1123,7 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
   const bool one_launch = d.n_all_pairs <= kOneLaunchPairs;
   for (int R = 1; R <= kMaxRowsPerLane && d.all_rows > 0; ++R) {
     const int64_t first = one_launch ? 0 : d.all_off[R - 1];
-    const int64_t n = one_launch ? (R == d.all_rows ? d.n_all_pairs : 0) : d.all_off[R] - first;
+    const int64_t n = one_launch ? (R == d.all_rows ? d.n_all_items : 0) : d.all_off[R] - first;
     if (!n) continue;
     SweepArgs a = d.base_args();
     a.items = d.all_pairs + 2 * first;

@@ -775,12 +775,15 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
 
     // Everything below is uniform across the wave; readfirstlane keeps it in scalar registers.
     // A main-pass item is one read, or two reads of one region sharing the wave (read A in lanes
-    // [0, La), read B in [La, La+Lb)); a pair-mode item is one (read, sorted haplotype).
+    // [0, La), read B in [La, La+Lb)); a pair-mode item is one (read, sorted haplotype k), or,
+    // with the sign bit set, two reads a < b of one region on haplotype k, encoded
+    // (a, k | (b - a) << 16 | sign).
     int ra, rb, k_begin, k_end;
     ra = __builtin_amdgcn_readfirstlane(a.items[2 * item]);
     if (a.pair_mode) {
-      rb = -1;
-      k_begin = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
+      const int v = __builtin_amdgcn_readfirstlane(a.items[2 * item + 1]);
+      rb = v < 0 ? ra + ((v >> 16) & 0x7FFF) : -1;
+      k_begin = v < 0 ? (v & 0xFFFF) : v;
       k_end = k_begin + 1;
       if (a.dense && __builtin_amdgcn_readfirstlane(a.dense[ra])) continue;
     } else {
