@@ -618,6 +618,45 @@ def test_native_entry_matches_reference_in_both_precisions(gpu):
             assert_close(out, expected, tol, "native float")
 
 
+def test_a_native_call_gives_each_read_its_lone_bits_whatever_it_is_paired_with(gpu):
+    """lib.rs:971-1017 for the per-call path: a call runs each (read, haplotype) pair in a wave
+    of its own or, when two reads of the region fit one wave at the launch's rows per lane, both
+    reads on the haplotype in one wave.  Short reads beside a long one pair up here; every read's
+    results must be the bits of calling it alone, in both precisions."""
+    rng = np.random.default_rng(2024)
+    lens = [300, 60, 70, 55, 250, 80, 64, 127, 30, 41, 9, 1]
+    def seq(alphabet, n, p=None):
+        return rng.choice(np.frombuffer(alphabet, np.uint8), size=n, p=p).astype(np.uint8).tobytes()
+
+    haps = [seq(b"ACGT", int(rng.integers(320, 380))) for _ in range(5)]
+    reads = []
+    for n in lens:
+        reads.append((seq(b"ACGTN", n, [0.24, 0.24, 0.24, 0.24, 0.04]),
+                      rng.integers(2, 42, size=n).astype(np.uint8),
+                      rng.integers(10, 46, size=n).astype(np.uint8),
+                      rng.integers(10, 46, size=n).astype(np.uint8),
+                      np.full(n, 10, np.uint8)))
+
+    def call(rs, dp):
+        ro = np.concatenate([[0], np.cumsum([len(r[0]) for r in rs])]).astype(np.int32)
+        ho = np.concatenate([[0], np.cumsum([len(h) for h in haps])]).astype(np.int32)
+        cat = lambda i: np.concatenate([np.frombuffer(r[0], np.uint8) if i == 0 else r[i]
+                                        for r in rs]).astype(np.uint8)
+        out = np.zeros(len(rs) * len(haps))
+        F.compute_native(cat(0), cat(1), cat(2), cat(3), cat(4), ro,
+                         np.frombuffer(b"".join(haps), np.uint8), ho, dp, out)
+        return out.reshape(len(rs), len(haps))
+
+    for dp in (False, True):
+        together = call(reads, dp)
+        for i, r in enumerate(reads):
+            alone = call([r], dp)[0]
+            assert np.array_equal(together[i], alone), (dp, lens[i])
+        if not dp:
+            refs = [F.ReadRef(r[0], list(r[1]), list(r[2]), list(r[3]), list(r[4])) for r in reads]
+            assert_close(together.ravel(), reference_all(refs, haps), FLOAT_TOL, "paired per-call")
+
+
 @pytest.mark.parametrize("aggregate", [0, 8])
 def test_concurrent_native_calls_give_identical_results(gpu, aggregate):  # FgklPairHmmTest.java:193-211
     """Eight concurrent calls of one region give bitwise the results of a lone call, with each
