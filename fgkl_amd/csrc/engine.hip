@@ -256,6 +256,7 @@ struct DeviceBatch::Impl {
   int64_t all_off[kMaxRowsPerLane + 1] = {};
   int all_rows = 0;
   bool pairs_only = false;  // planned for run_latency only
+  size_t out_rb_off = 8;    // where the results start in the readback buffer
   // per-region pair lists, for the per-call mode
   int32_t* region_pairs = nullptr;
   std::vector<int64_t> region_pair_off;  // (G * 8 + 1) offsets in pairs, bucket-major per region
@@ -815,7 +816,10 @@ int DeviceBatch::assign(const BatchInput& in) {
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.rescue_pairs[R - 1], static_cast<const int32_t*>(nullptr),
             size_t(2 * d.rows_pairs[R - 1]));
+  // The results right after the counters, so a pair run reads back its rescued counts and its
+  // results with one copy.
   d.place(&d.counters, static_cast<const int32_t*>(nullptr), kCounters);
+  d.place(&d.out, static_cast<const double*>(nullptr), size_t(p.n_pairs));
   d.place(&d.dense_flags, static_cast<const uint8_t*>(nullptr), size_t(NR));
   std::fill(std::begin(d.rows_reads), std::end(d.rows_reads), 0);
   if (!in.pd && !in.pairs_only)
@@ -824,7 +828,6 @@ int DeviceBatch::assign(const BatchInput& in) {
   for (int R = 1; R <= kMaxRowsPerLane; ++R)
     d.place(&d.dense_items[R - 1], static_cast<const int32_t*>(nullptr),
             size_t(2 * d.rows_reads[R - 1]));
-  d.place(&d.out, static_cast<const double*>(nullptr), size_t(p.n_pairs));
   d.n_long = int64_t(p.long_pairs.size() / 2);
   d.place(&d.long_pairs, p.long_pairs.data(), p.long_pairs.size());
   d.place(&d.long_rescue, static_cast<const int32_t*>(nullptr), p.long_pairs.size());
@@ -1112,7 +1115,11 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
   }
   const bool fp32 = mode == kModeFp32Rescue || mode == kModeFp32NoRescue;
   const size_t out_bytes = sizeof(double) * size_t(d.plan.n_pairs);
-  if (int rc = d.reserve_readback(8 + out_bytes)) return rc;
+  // One copy brings back counters 16.. (the rescued counts at 16 and 42) and the results, which
+  // the arena places right after the counters.
+  const size_t head = size_t(reinterpret_cast<const char*>(d.out) -
+                             reinterpret_cast<const char*>(d.counters + 16));
+  if (int rc = d.reserve_readback(head + out_bytes)) return rc;
   hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, kCounters, 0);
   HIP_TRY(hipGetLastError());
   if (stats) HIP_TRY(hipEventRecord(d.ev[0], d.stream));
@@ -1159,20 +1166,18 @@ int DeviceBatch::run_latency(int mode, RunStats* stats) {
     if (int rc = d.enqueue_long_rescue()) return rc;
   if (stats) HIP_TRY(hipEventRecord(d.ev[2], d.stream));
   auto* rb = static_cast<char*>(d.readback);
-  if (fp32) {  // the rescued counts of the staged pairs and of the long pairs
-    HIP_TRY(hipMemcpyAsync(rb, d.counters + 16, sizeof(int32_t), hipMemcpyDeviceToHost, d.stream));
-    HIP_TRY(hipMemcpyAsync(rb + 4, d.counters + 42, sizeof(int32_t), hipMemcpyDeviceToHost, d.stream));
-  }
-  HIP_TRY(hipMemcpyAsync(rb + 8, d.out, out_bytes, hipMemcpyDeviceToHost, d.stream));
+  HIP_TRY(hipMemcpyAsync(rb, d.counters + 16, head + out_bytes, hipMemcpyDeviceToHost, d.stream));
   HIP_TRY(hipStreamSynchronize(d.stream));
   int32_t rescued = 0;
-  if (fp32) {
-    int32_t c[2];
-    std::memcpy(c, rb, sizeof(c));
-    rescued = c[0] + c[1];
+  if (fp32) {  // the rescued counts of the staged pairs and of the long pairs
+    int32_t c16 = 0, c42 = 0;
+    std::memcpy(&c16, rb, sizeof(c16));
+    std::memcpy(&c42, rb + (42 - 16) * sizeof(int32_t), sizeof(c42));
+    rescued = c16 + c42;
   }
   d.last_rescued = rescued;
   d.out_on_host = true;
+  d.out_rb_off = head;
   if (stats) {
     *stats = RunStats{};
     float total = 0.f, first = 0.f;
@@ -1288,8 +1293,9 @@ int DeviceBatch::results(double* out, int64_t n) {
     HIP_TRY(hipMemcpyAsync(static_cast<char*>(d.readback) + 8, d.out, bytes,
                            hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
+    d.out_rb_off = 8;
   }
-  std::memcpy(out, static_cast<const char*>(d.readback) + 8, bytes);
+  std::memcpy(out, static_cast<const char*>(d.readback) + d.out_rb_off, bytes);
   return kOk;
 }
 
