@@ -700,8 +700,8 @@ int DeviceBatch::create(const BatchInput& in, std::unique_ptr<DeviceBatch>& out)
   return kOk;
 }
 
-// Pair-mode items of the reads whose own rows per lane is `R`, for a launch at `run_rows` rows
-// per lane: (read, sorted haplotype k) for every haplotype of the read's region, or, for two
+// Pair-mode items of the reads whose own rows per lane is `R` (any, when `R` is 0), for a launch
+// at `run_rows` rows per lane: (read, sorted haplotype k) for every haplotype of the read's region, or, for two
 // reads a < b of one region whose lanes fit one wave together at run_rows, one item per
 // haplotype carrying both as (a, k | (b - a) << 16 | sign bit).  Such an item is the main pass's
 // two-read wave on one haplotype, and a read gets the same bits whatever shares its wave.  Reads
@@ -713,7 +713,7 @@ void pair_items(const BatchInput& in, const HostPlan& p, int R, int run_rows,
     const int32_t H = p.region_nhap[size_t(g)];
     std::vector<std::pair<int, int64_t>> reads;  // (lanes at run_rows, read)
     for (int64_t r = in.region_read[g]; r < in.region_read[g + 1] && r < NR; ++r) {
-      if (p.read_rows[size_t(r)] != R) continue;
+      if (R ? p.read_rows[size_t(r)] != R : p.read_rows[size_t(r)] < 1) continue;
       const int n = int(in.read_off[r + 1] - in.read_off[r]);
       reads.push_back({(n + run_rows) / run_rows, r});
     }
@@ -848,9 +848,14 @@ int DeviceBatch::assign(const BatchInput& in) {
     // bucket runs at its own.
     const bool one_launch = n_pairs <= kOneLaunchPairs;
     all.reserve(size_t(2 * n_pairs));
-    for (int R = 1; R <= kMaxRowsPerLane; ++R) {
-      d.all_off[R - 1] = int64_t(all.size() / 2);
-      pair_items(in, p, R, one_launch ? d.all_rows : R, all);
+    if (one_launch) {  // every item at the largest rows per lane: reads pair across buckets
+      pair_items(in, p, 0, d.all_rows, all);
+      for (int R = 2; R <= kMaxRowsPerLane; ++R) d.all_off[R - 1] = int64_t(all.size() / 2);
+    } else {
+      for (int R = 1; R <= kMaxRowsPerLane; ++R) {
+        d.all_off[R - 1] = int64_t(all.size() / 2);
+        pair_items(in, p, R, R, all);
+      }
     }
     d.n_all_items = int64_t(all.size() / 2);
     d.all_off[kMaxRowsPerLane] = d.n_all_items;
