@@ -393,6 +393,12 @@ class Batch:
         _check(lib().fgkl_batch_work(self._h, w))
         return dict(zip(("nominal", "prefix", "forward", "backward"), (int(x) for x in w)))
 
+    def set_uniform_gaps(self, enable: bool):
+        """Runs the items whose reads have uniform gap qualities with the uniform-gap kernel build
+        (the default) or every item with the general build (fgkl_batch_set_uniform_gaps); the
+        bits are the same."""
+        _check(lib().fgkl_batch_set_uniform_gaps(self._h, int(bool(enable))))
+
     def set_lds_records(self, enable: bool):
         """Lets prefix-sharing launches keep a snapshot record in each wave's LDS (the default)
         or sends every record to HBM (fgkl_batch_set_lds_records); the bits are the same."""

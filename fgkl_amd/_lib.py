@@ -91,6 +91,7 @@ SIGNATURES = {
     "fgkl_batch_pairs": (C.c_int64, [C.c_void_p]),
     "fgkl_batch_work": (C.c_int, [C.c_void_p, i64p]),
     "fgkl_batch_set_lds_records": (C.c_int, [C.c_void_p, C.c_int]),
+    "fgkl_batch_set_uniform_gaps": (C.c_int, [C.c_void_p, C.c_int]),
     "fgkl_batch_snapshots": (C.c_int, [C.c_void_p, i64p]),
     "fgkl_plan_snapshots": (C.c_int, [C.c_int, C.c_int64, i64p, i64p, i64p, i64p, u8p, u8p,
                                       i64p]),

@@ -338,6 +338,13 @@ int fgkl_batch_work(const fgkl_batch* batch, int64_t* cells4) {
   return kOk;
 }
 
+int fgkl_batch_set_uniform_gaps(fgkl_batch* batch, int enable) {
+  if (!batch) return fail(kInvalidArgument, "null batch");
+  if (enable != 0 && enable != 1) return fail(kInvalidArgument, "enable must be 0 or 1");
+  batch->batch->set_uniform_gaps(enable != 0);
+  return kOk;
+}
+
 int fgkl_batch_set_lds_records(fgkl_batch* batch, int enable) {
   if (!batch) return fail(kInvalidArgument, "null batch");
   if (enable != 0 && enable != 1) return fail(kInvalidArgument, "enable must be 0 or 1");

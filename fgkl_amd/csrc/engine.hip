@@ -47,11 +47,14 @@ namespace {
 
 template <typename T> struct Kernels;
 template <> struct Kernels<float> {
-  static KernelF get(bool pd, bool boost, int rows) { return kSweepF32[pd][boost][rows - 1]; }
+  static KernelF get(bool pd, bool boost, int rows, bool ug = false) {
+    return ug && !pd ? kSweepF32Uniform[boost][rows - 1] : kSweepF32[pd][boost][rows - 1];
+  }
 };
-// FP64 has no occupancy-raised build; its flag selects the suffix-sharing build (plain only).
+// FP64 has no occupancy-raised or uniform-gap build; its flag selects the suffix-sharing build
+// (plain only).
 template <> struct Kernels<double> {
-  static KernelD get(bool pd, bool suffix, int rows) {
+  static KernelD get(bool pd, bool suffix, int rows, bool = false) {
     return suffix && !pd ? kSweepF64Suffix[rows - 1] : kSweepF64[pd][rows - 1];
   }
 };
@@ -67,7 +70,7 @@ constexpr int kBlocks[3] = {64, 128, 256};
 int block_index(int block) { return block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1; }
 
 constexpr int64_t kMaxQueueItems = int64_t(1) << 24;  // 64 x (items + waves) < 2^31
-constexpr int kCounters = 48;  // see Impl::counters
+constexpr int kCounters = 56;  // see Impl::counters
 // Uploads above this size go straight from the caller's arrays when no staging buffer fits them.
 constexpr size_t kDirectUploadBytes = size_t(64) << 20;
 // Pairs a one-pair-per-wave run takes in a single launch: 256 CUs hold 8192 waves at 32 per CU.
@@ -81,7 +84,7 @@ struct DeviceState {
   // Resident blocks per CU of each kernel [fp64][pd][boost][rows-1][block], at the LDS size it
   // was queried with.
   // Resident blocks per CU of a sweep kernel at a block size and LDS bytes per block.
-  std::map<std::array<int, 6>, int> occ;
+  std::map<std::array<int, 7>, int> occ;
   // Launch configurations the autotuner chose for FP32 buckets [pd][R-1] (rows 0: none yet).
   LaunchCfg tuned[2][kMaxRowsPerLane];
   // Resident one-wave blocks per CU of long_kernel [fp64][pd] (0: not queried yet).
@@ -144,14 +147,14 @@ int device_state(DeviceState** out) {
 
 template <typename T>
 int resident_blocks(DeviceState* st, bool pd, bool boost, int rows, int block, int lds_bytes,
-                    int* blocks) {
+                    int* blocks, bool ug = false) {
   const int ti = std::is_same<T, double>::value ? 1 : 0;
   std::lock_guard<std::mutex> lock(g_dev_mutex);
-  int& occ = st->occ[{ti, int(pd), int(boost), rows, block, lds_bytes}];
+  int& occ = st->occ[{ti, int(pd), int(boost), rows, block, lds_bytes, int(ug)}];
   if (occ == 0) {
     int nb = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, Kernels<T>::get(pd, boost, rows),
-                                                         block, size_t(lds_bytes)));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, Kernels<T>::get(pd, boost, rows, ug), block, size_t(lds_bytes)));
     occ = std::max(nb, 1);
   }
   *blocks = occ;
@@ -218,6 +221,10 @@ struct DeviceBatch::Impl {
   uint8_t* region_classes = nullptr;  // PD: read-base classes from 4 on, per region
   int32_t* bucket_items[kMaxRowsPerLane] = {};
   int32_t bucket_n[kMaxRowsPerLane] = {};
+  // Plain batches: each bucket's items whose reads all have uniform gap qualities come first,
+  // bucket_ug[R - 1] of them, and run with the uniform-gap build (launch_bucket).
+  int32_t bucket_ug[kMaxRowsPerLane] = {};
+  bool uniform_gaps = true;
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
   int64_t bucket_cells[kMaxRowsPerLane] = {};
   int64_t bucket_work[kMaxRowsPerLane] = {};
@@ -412,7 +419,8 @@ struct DeviceBatch::Impl {
     const int wpb = c.block / 64;
     int per_cu = 0;
     const int words = wave_words ? wave_words : stage_words(rows);
-    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, words * 4 * wpb, &per_cu))
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, words * 4 * wpb, &per_cu,
+                                    c.ug))
       return rc;
     const int64_t want = (max_items + wpb - 1) / wpb;
     *grid = int(std::min<int64_t>(int64_t(per_cu) * st->num_cus, want));
@@ -440,8 +448,10 @@ struct DeviceBatch::Impl {
     const int with = base + lds_record_words<T>(rows);
     if (int64_t(with) * 4 * wpb > 64 * 1024) return kOk;
     int b0 = 0, b1 = 0;
-    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, base * 4 * wpb, &b0)) return rc;
-    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, with * 4 * wpb, &b1)) return rc;
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, base * 4 * wpb, &b0, c.ug))
+      return rc;
+    if (int rc = resident_blocks<T>(st, pd, c.boost, rows, c.block, with * 4 * wpb, &b1, c.ug))
+      return rc;
     if (b1 < b0) return kOk;
     a.lds_words = with;
     a.lds_snap = 1;
@@ -459,6 +469,7 @@ struct DeviceBatch::Impl {
     if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
     if (!std::is_same<T, float>::value) c.boost = a.suffix != 0;  // FP64: the suffix build
+    c.ug = c.ug && std::is_same<T, float>::value && !pd && !a.pair_mode;
     if (int rc = lds_layout<T>(a, c, rows)) return rc;
     int grid = 0;
     if (int rc = grid_waves<T>(c, rows, max_items, &grid, a.lds_words)) return rc;
@@ -475,9 +486,33 @@ struct DeviceBatch::Impl {
         return kInternal;
       }
     }
-    hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows), dim3(grid), dim3(c.block),
+    hipLaunchKernelGGL(Kernels<T>::get(pd, c.boost, rows, c.ug), dim3(grid), dim3(c.block),
                        a.lds_words * 4 * (c.block / 64), on, a, dev_tab<T>(st));
     HIP_TRY(hipGetLastError());
+    return kOk;
+  }
+
+  // Bucket R's main-pass launch over items `a` (primary_args) with configuration `c`: in a plain
+  // FP32 pass, the items of reads with uniform gap qualities with the uniform-gap build and the
+  // others (work-queue counter 48 + R - 1) after them on the same stream; otherwise one launch.
+  template <typename T>
+  int launch_bucket(int R, const SweepArgs& a, int64_t n, LaunchCfg c, hipStream_t on,
+                    size_t scratch_off) {
+    const int64_t u = (std::is_same<T, float>::value && !pd && uniform_gaps) ? bucket_ug[R - 1] : 0;
+    if (u > 0) {
+      SweepArgs au = a;
+      au.n_items = int32_t(u);
+      LaunchCfg cu = c;
+      cu.ug = true;
+      if (int rc = launch<T>(R, au, u, cu, on, scratch_off)) return rc;
+    }
+    if (u < n) {
+      SweepArgs ag = a;
+      ag.items = a.items + 2 * u;
+      ag.n_items = int32_t(n - u);
+      if (u > 0) ag.counter = counters + 48 + (R - 1);
+      if (int rc = launch<T>(R, ag, n - u, c, on, scratch_off)) return rc;
+    }
     return kOk;
   }
 
@@ -633,7 +668,7 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
       const SweepArgs a = primary_args(R, share, fp32);
       if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1)], on));
       if (fp32) {
-        if (int rc = launch<float>(R, a, bucket_n[R - 1], cfg[R - 1], on, off)) return rc;
+        if (int rc = launch_bucket<float>(R, a, bucket_n[R - 1], cfg[R - 1], on, off)) return rc;
       } else {
         if (int rc = launch<double>(R, a, bucket_n[R - 1], LaunchCfg{}, on, off)) return rc;
       }
@@ -793,9 +828,35 @@ int DeviceBatch::assign(const BatchInput& in) {
     d.place(&d.region_classes, p.region_classes.data(), p.region_classes.size());
   } else
     d.place(&d.hap_bytes, in.hap_bases, hb);
+  // Reads whose insertion, deletion and gap-continuation qualities are each the same at every
+  // base (plain batches).
+  std::vector<uint8_t> uniform;
+  if (!in.pd && !in.pairs_only) {
+    uniform.assign(size_t(NR), 0);
+    for (int64_t r = 0; r < NR; ++r) {
+      const int64_t o0 = in.read_off[r], o1 = in.read_off[r + 1];
+      bool u = o1 > o0;
+      for (int64_t i = o0 + 1; i < o1 && u; ++i)
+        u = in.ins[i] == in.ins[o0] && in.del[i] == in.del[o0] && in.gcp[i] == in.gcp[o0];
+      uniform[size_t(r)] = u;
+    }
+  }
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
-    const auto& items = p.bucket_items[R - 1];
+    auto& items = d.plan.bucket_items[R - 1];
     d.bucket_n[R - 1] = int32_t(items.size() / 2);
+    d.bucket_ug[R - 1] = 0;
+    if (!uniform.empty()) {  // uniform items first, each part in its cost order
+      std::vector<int32_t> ug, rest;
+      for (size_t i = 0; i < items.size(); i += 2) {
+        const int32_t ra = items[i], rb = items[i + 1];
+        auto& to = uniform[size_t(ra)] && (rb < 0 || uniform[size_t(rb)]) ? ug : rest;
+        to.push_back(ra);
+        to.push_back(rb);
+      }
+      d.bucket_ug[R - 1] = int32_t(ug.size() / 2);
+      ug.insert(ug.end(), rest.begin(), rest.end());
+      items.swap(ug);
+    }
     const int64_t pairs = p.bucket_pairs[R - 1];
     if (pairs > kMaxQueueItems) {  // a work queue advances by 64 per claim in an int32 counter
       set_error(kInvalidArgument, "a batch holds more than " + std::to_string(kMaxQueueItems) +
@@ -902,7 +963,11 @@ int DeviceBatch::assign(const BatchInput& in) {
         need = std::max(need, d.scratch_need<double>(R, LaunchCfg{}, d.rows_reads[R - 1]));
       if (!d.bucket_n[R - 1]) continue;
       int64_t n = 0;
-      for (const LaunchCfg& c : d.candidates(R, true)) n = std::max(n, d.scratch_need<float>(R, c));
+      for (LaunchCfg c : d.candidates(R, true)) {
+        n = std::max(n, d.scratch_need<float>(R, c));
+        c.ug = !d.pd;  // the uniform-gap build's grid
+        n = std::max(n, d.scratch_need<float>(R, c));
+      }
       n = std::max(n, d.scratch_need<double>(R, LaunchCfg{}));
       if (d.suffix_run(1)) n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, true}));
       need = std::max(need, n);  // tuning runs every bucket on the batch's stream
@@ -1213,7 +1278,8 @@ int DeviceBatch::tune(int share, TuneReport* report) {
         hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, d.stream, d.counters, kCounters, 0);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(d.ev[0], d.stream));
-        if (int rc = d.launch<float>(R, a, d.bucket_n[R - 1], cands[i])) return rc;
+        if (int rc = d.launch_bucket<float>(R, a, d.bucket_n[R - 1], cands[i], d.stream, 0))
+          return rc;
         HIP_TRY(hipEventRecord(d.ev[1], d.stream));
         HIP_TRY(hipStreamSynchronize(d.stream));
         float t = 0.f;
@@ -1252,6 +1318,11 @@ int DeviceBatch::set_launch(int R, const LaunchCfg& cfg) {
   d.cfg[R - 1].rows = cfg.rows ? cfg.rows : R;
   d.drop_graphs();
   return kOk;
+}
+
+void DeviceBatch::set_uniform_gaps(bool on) {
+  impl_->uniform_gaps = on;
+  impl_->drop_graphs();
 }
 
 void DeviceBatch::set_lds_records(bool on) {

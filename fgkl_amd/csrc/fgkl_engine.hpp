@@ -33,6 +33,7 @@ struct LaunchCfg {
   int rows = 0;        // kernel rows per lane, R or R + 1 for bucket R (0: R)
   int block = 256;     // threads per workgroup: 64, 128 or 256
   bool boost = false;  // the build compiled for one more wave per SIMD
+  bool ug = false;     // the uniform-gap build (chosen per launch by the engine, not tuned)
 };
 
 struct TuneReport {
@@ -78,6 +79,9 @@ class DeviceBatch {
   // Whether prefix-sharing launches may keep a snapshot record in the wave's LDS (default on;
   // off: every record goes to an HBM slot).  Results are the same bits either way.
   void set_lds_records(bool on);
+  // Whether plain FP32 main passes run the items whose reads have uniform gap qualities with the
+  // uniform-gap build (default on).  Results are the same bits either way.
+  void set_uniform_gaps(bool on);
   // Snapshot records of the plan per FP32 pass (HBM, LDS-eligible, restarts) and the sweep
   // launches so far that kept a record in LDS.
   void snapshots(int64_t out[4]) const;

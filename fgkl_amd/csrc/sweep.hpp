@@ -129,8 +129,15 @@ __device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
 }
 
 // Per-lane parameters of its R rows (Workspace::prepare, kernel.rs:485-529).
-template <typename T, int R, bool PD> struct Rows {
-  T pm[R], px[R], mm[R], im[R], mi[R], md[R], gc[R];
+// UG (uniform gaps): the read's insertion, deletion and gap-continuation qualities are the same
+// in every row, so mm, im, mi and md are one value per lane, kept at index 0 (g(k) is 0), and
+// identity rows hold the read's values there too: their zero prior zeroes what mm and im scale,
+// the rows above them hold no match value for mi and md to scale, and the one row above that
+// does (read A's last row, above read B's first lane) reaches read B's identity row masked (see
+// shift_up).  gc stays per row: identity rows keep D->D and I->I at 1.
+template <typename T, int R, bool PD, bool UG = false> struct Rows {
+  static constexpr int g(int k) { return UG ? 0 : k; }
+  T pm[R], px[R], mm[UG ? 1 : R], im[UG ? 1 : R], mi[UG ? 1 : R], md[UG ? 1 : R], gc[R];
   uint32_t rb[R];
   uint32_t ralt[R];
   // PD, staged sweep: the row's read-base class, the bit of the column word that says whether
@@ -158,10 +165,19 @@ __device__ __forceinline__ uint32_t read_class(uint32_t b, const uint8_t* extra)
 }
 
 // `extra` (PD, staged sweep) is the region's read-base classes from 4 on, or null.
-template <typename T, int R, bool PD>
+template <typename T, int R, bool PD, bool UG = false>
 __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& tab, int64_t o,
-                                          int L, int pad, int lane, Rows<T, R, PD>& rw,
+                                          int L, int pad, int lane, Rows<T, R, PD, UG>& rw,
                                           const uint8_t* extra = nullptr) {
+  if constexpr (UG) {  // the read's values, from its first base (zero in lanes past the reads)
+    const bool live = lane < L;
+    const int iq = dclamp(a.ins[o]), dq = dclamp(a.del[o]), gq = dclamp(a.gcp[o]);
+    const T mm = tab.mm[((iq >= dq ? iq * (iq + 1) : dq * (dq + 1)) >> 1) + (iq >= dq ? dq : iq)];
+    rw.mm[0] = live ? mm : T(0);
+    rw.mi[0] = live ? tab.ep[iq] : T(0);
+    rw.md[0] = live ? tab.ep[dq] : T(0);
+    rw.im[0] = live ? tab.im[gq] : T(0);
+  }
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int arow = R * lane + k - pad;
@@ -172,11 +188,13 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       const int gq = dclamp(a.gcp[i]);
       rw.pm[k] = tab.pm[q];
       rw.px[k] = b == 'N' ? rw.pm[k] : tab.px[q];  // an N read base matches everything
-      rw.mm[k] = tab.mm[((iq >= dq ? iq * (iq + 1) : dq * (dq + 1)) >> 1) + (iq >= dq ? dq : iq)];
-      rw.mi[k] = tab.ep[iq];
-      rw.md[k] = tab.ep[dq];
+      if constexpr (!UG) {
+        rw.mm[k] = tab.mm[((iq >= dq ? iq * (iq + 1) : dq * (dq + 1)) >> 1) + (iq >= dq ? dq : iq)];
+        rw.mi[k] = tab.ep[iq];
+        rw.md[k] = tab.ep[dq];
+        rw.im[k] = tab.im[gq];
+      }
       rw.gc[k] = tab.ep[gq];  // insertion and deletion extension alike (model.rs:73-75)
-      rw.im[k] = tab.im[gq];
       rw.rb[k] = b;
       rw.ralt[k] = alt_bit(b);
       if constexpr (PD) rw.ci[k] = read_class(b, extra);
@@ -184,8 +202,10 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
     } else {
       // Identity rows: D->D 1 keeps row 0's C/m; I->I 1 is harmless because the row above an
       // identity row is zero.  Lanes past the read: all zero.
-      rw.pm[k] = T(0); rw.px[k] = T(0); rw.mm[k] = T(0); rw.im[k] = T(0);
-      rw.mi[k] = T(0); rw.md[k] = T(0);
+      rw.pm[k] = T(0); rw.px[k] = T(0);
+      if constexpr (!UG) {
+        rw.mm[k] = T(0); rw.im[k] = T(0); rw.mi[k] = T(0); rw.md[k] = T(0);
+      }
       rw.gc[k] = lane < L ? T(1) : T(0);
       rw.rb[k] = 0x100u;
       rw.ralt[k] = 0u;
@@ -237,9 +257,12 @@ __device__ __forceinline__ double and_bits(double v, uint32_t k) {
 // elsewhere).  The mask is applied after the move by an AND on an opaque mask: a select would
 // let the compiler run the move only in the lanes that keep it, and a DPP move reads a lane
 // outside EXEC as zero, which would also cut the insertion value reaching the lane below.
+// `keep_m` masks the match value the same way where rows take uniform gap values (Rows, UG);
+// elsewhere it is all ones and folds away.
 template <typename T, int R, bool PD>
-__device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u, uint32_t keep) {
-  u.m = shr1z(s.m[R - 1]);
+__device__ __forceinline__ void shift_up(const Lane<T, R, PD>& s, Cell<T>& u, uint32_t keep,
+                                         uint32_t keep_m = ~0u) {
+  u.m = and_bits(shr1z(s.m[R - 1]), keep_m);
   u.x = and_bits(shr1z(s.x[R - 1]), keep);
   u.y = shr1z(s.y[R - 1]);
   if constexpr (PD) {
@@ -288,8 +311,8 @@ __device__ __forceinline__ double max_nonneg(double a, double b) { return fmax(a
 // sweep forms the diagonal maxima of its first column afresh.
 // CLS (PD): the column word holds read-base classes (pd_class_mask) and a row's prior is chosen
 // by its class bit; otherwise it holds the haplotype byte and alternates and bytes are compared.
-template <typename T, int R, bool PD, bool CARRY = false, bool CLS = false>
-__device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& rw,
+template <typename T, int R, bool PD, bool CARRY = false, bool CLS = false, bool UG = false>
+__device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG>& rw,
                                       const Cell<T>& d, const Cell<T>& u) {
   const uint32_t w = s.w;
   const uint32_t hb = w & 0xFFu;
@@ -318,8 +341,8 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
     }
     if constexpr (!PD) {
       // kernel.rs:1011-1013
-      const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[k]);
-      s.m[k] = prior * fma(pdy, rw.im[k], fma(pdx, rw.im[k], pdm * rw.mm[k]));
+      const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[rw.g(k)]);
+      s.m[k] = prior * fma(pdy, rw.im[rw.g(k)], fma(pdx, rw.im[rw.g(k)], pdm * rw.mm[rw.g(k)]));
       s.y[k] = Y;
     } else {
       // pd_reference.rs:62-94 for the column's deletion state.
@@ -347,7 +370,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
         lxe = s.dx[k + 1];
         lye = s.dy[k + 1];
       }
-      const T Y = fma(lye, rw.gc[k], lme * rw.md[k]);
+      const T Y = fma(lye, rw.gc[k], lme * rw.md[rw.g(k)]);
       // NORMAL copies the row's values (lme is m there), AFTER_DEL takes the maxima, INSIDE_DEL
       // holds the branch.
       // Opaque to the optimiser: otherwise it distributes the next step's `branch & after` over
@@ -356,7 +379,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
       T BX = sel_bits(I, s.bx[k], lxe);
       T BY = sel_bits(I, s.by[k], lye);
       asm volatile("" : "+v"(BM), "+v"(BX), "+v"(BY));
-      s.m[k] = prior * fma(dye, rw.im[k], fma(dxe, rw.im[k], dme * rw.mm[k]));
+      s.m[k] = prior * fma(dye, rw.im[rw.g(k)], fma(dxe, rw.im[rw.g(k)], dme * rw.mm[rw.g(k)]));
       s.y[k] = Y;
       s.bm[k] = BM;
       s.bx[k] = BX;
@@ -368,14 +391,14 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD>& r
     const T pum = k ? s.m[k - 1] : u.m;
     const T pux = k ? s.x[k - 1] : u.x;
     if constexpr (!PD) {
-      s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[k]);
+      s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[rw.g(k)]);
     } else {
       // pd_reference.rs:95-100
       const T pubm = k ? s.bm[k - 1] : u.bm;
       const T pubx = k ? s.bx[k - 1] : u.bx;
       const T ume = max_nonneg(pum, and_bits(pubm, de));
       const T uxe = max_nonneg(pux, and_bits(pubx, de));
-      s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[k]);
+      s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[rw.g(k)]);
       if constexpr (CARRY) {
         const T puy = k ? s.y[k - 1] : u.y;
         const T puby = k ? s.by[k - 1] : u.by;
@@ -431,12 +454,13 @@ __device__ __forceinline__ uint32_t next_word(uint32_t from_lds, uint32_t w, uin
 // `span` is the lane count of the wave's longest read.
 // `snap_in` (HBM) or `lds_in` (the wave's LDS record) holds the start column's state when the
 // sweep resumes from one; `lrec` is the wave's LDS record that columns marked kLdsField go to.
-template <typename T, int R, bool PD, bool SNAP>
-__device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int span, int L,
+template <typename T, int R, bool PD, bool SNAP, bool UG = false>
+__device__ void sweep(const Rows<T, R, PD, UG>& rw, const uint32_t* lw, int W, int span, int L,
                       int pos, int lane, uint32_t head, T init, const T* snap_in,
                       const T* lds_in, T scale, T* scratch, T* lrec, bool row0_above,
                       uint32_t keep, Lane<T, R, PD>& s) {
   constexpr int NV = snap_values<R, PD>();
+  const uint32_t keep_m = UG ? keep : ~0u;  // see shift_up
   Cell<T> P{T(0), T(0), T(0), T(0), T(0), T(0)};  // diagonal neighbours of the first row
   if (snap_in || lds_in) {
     const T* p = (lds_in ? lds_in : snap_in) + lane * snap_stride<R, PD>();
@@ -454,7 +478,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     s.acc = p[(NV - 1)] * scale;
     // The row above the first row at the start column: the restored last row of the lane above,
     // moved and masked as the sweep moves it, which is the value the lane saw when it recorded.
-    shift_up(s, P, keep);
+    shift_up(s, P, keep, keep_m);
   } else {
     // Column 0: zeros below row 0; identity rows carry row 0's deletion value C/m.
 #pragma unroll
@@ -508,7 +532,7 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
   Cell<T> Q;
   for (; t < ramp; ++t) {
     s.w = next_word(lw[t], s.w, head);
-    shift_up(s, Q, keep);
+    shift_up(s, Q, keep, keep_m);
     if (t >= pos && t - pos < W) {
       cells<T, R, PD, PD, PD>(s, rw, P, Q);
       P = Q;
@@ -522,25 +546,25 @@ __device__ void sweep(const Rows<T, R, PD>& rw, const uint32_t* lw, int W, int s
     const uint4 cq = q;
     q = *reinterpret_cast<const uint4*>(lw + t + 4);
     s.w = next_word(cq.x, s.w, head);
-    shift_up(s, Q, keep);
+    shift_up(s, Q, keep, keep_m);
     cells<T, R, PD, PD, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.y, s.w, head);
-    shift_up(s, P, keep);
+    shift_up(s, P, keep, keep_m);
     cells<T, R, PD, PD, PD>(s, rw, Q, P);
     record(P);
     s.w = next_word(cq.z, s.w, head);
-    shift_up(s, Q, keep);
+    shift_up(s, Q, keep, keep_m);
     cells<T, R, PD, PD, PD>(s, rw, P, Q);
     record(Q);
     s.w = next_word(cq.w, s.w, head);
-    shift_up(s, P, keep);
+    shift_up(s, P, keep, keep_m);
     cells<T, R, PD, PD, PD>(s, rw, Q, P);
     record(P);
   }
   for (; t < steps; ++t) {
     s.w = next_word(lw[t], s.w, head);
-    shift_up(s, Q, keep);
+    shift_up(s, Q, keep, keep_m);
     if (t >= pos && t - pos < W) {
       cells<T, R, PD, PD, PD>(s, rw, P, Q);
       P = Q;
@@ -639,8 +663,8 @@ template <typename T, int R> struct BLane {
 // new values) and one column to the right (match, the old value), and itself one column to the
 // right (deletion); the row below the last row is lane l+1's first row (`qm`, `qx` this column,
 // P the previous one), with parameters `eg`.  Records a node when the word carries one.
-template <typename T, int R>
-__device__ __forceinline__ void bcells(BLane<T, R>& s, const Rows<T, R, false>& rw,
+template <typename T, int R, bool UG>
+__device__ __forceinline__ void bcells(BLane<T, R>& s, const Rows<T, R, false, UG>& rw,
                                        const EdgeRow<T>& eg, T qm, T qx, T sink, T* nodes,
                                        int lane) {
   const uint32_t hb = s.w & 0x1FFu;
@@ -653,16 +677,16 @@ __device__ __forceinline__ void bcells(BLane<T, R>& s, const Rows<T, R, false>& 
     const T pmb = edge ? eg.pm : rw.pm[kb];
     const T pxb = edge ? eg.px : rw.px[kb];
     const uint32_t rbb = edge ? eg.rb : rw.rb[kb];
-    const T mmb = edge ? eg.mm : rw.mm[kb];
-    const T imb = edge ? eg.im : rw.im[kb];
-    const T mib = edge ? eg.mi : rw.mi[kb];
+    const T mmb = edge ? eg.mm : rw.mm[rw.g(kb)];
+    const T imb = edge ? eg.im : rw.im[rw.g(kb)];
+    const T mib = edge ? eg.mi : rw.mi[rw.g(kb)];
     const T iib = edge ? eg.gc : rw.gc[kb];
     const bool match = hN | (hb == rbb);
     const T into = (match ? pmb : pxb) * diag;  // kernel.rs:1071-1082
     const T ind = into * imb;
     T nbx = fma(below_x, iib, ind);
     const T nby = fma(s.by[k], rw.gc[k], ind);
-    T nbm = fma(s.by[k], rw.md[k], fma(below_x, mib, into * mmb));
+    T nbm = fma(s.by[k], rw.md[rw.g(k)], fma(below_x, mib, into * mmb));
     if (edge) {
       nbm += sink;
       nbx += sink;
@@ -697,8 +721,8 @@ __device__ __forceinline__ void bcells(BLane<T, R>& s, const Rows<T, R, false>& 
 // lane L's values into lane L-1, where a stale NaN times the zero parameters below a read's last
 // row would still be NaN.  `full` is the step from which every lane of the item is inside the
 // sweep (the largest delay).
-template <typename T, int R>
-__device__ void bsweep(const Rows<T, R, false>& rw, const EdgeRow<T>& eg, const uint32_t* lw,
+template <typename T, int R, bool UG>
+__device__ void bsweep(const Rows<T, R, false, UG>& rw, const EdgeRow<T>& eg, const uint32_t* lw,
                        int Wb, int steps, int full, int lane, int L, int delay, uint32_t from_lds,
                        T sink, const T* rec_in, T* nodes) {
   constexpr int NS = node_stride<R>();
@@ -753,7 +777,7 @@ constexpr int boost_waves(int R, bool PD) {
 // BOOST selects the occupancy-raised variant (FP32 only).  SFX compiles the suffix-sharing path
 // (backward sweeps and joins); FP64 has a build without it, whose registers are not raised by
 // that path, for the runs that share no suffixes (the rescue of FP32 pairs among them).
-template <typename T, int R, bool PD, bool BOOST, bool SFX>
+template <typename T, int R, bool PD, bool BOOST, bool SFX, bool UG = false>
 __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
     sweep_kernel(SweepArgs a, DevTab<T> tab) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds_all[];
@@ -809,8 +833,8 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
 
     // This lane's read: its offset, lanes, identity rows and lane index within the read.
     const bool in_b = lane >= La;
-    Rows<T, R, PD> rw;
-    load_rows<T, R, PD>(a, tab, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
+    Rows<T, R, PD, UG> rw;
+    load_rows<T, R, PD, UG>(a, tab, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
                         in_b ? lane - La : lane, rw,
                         PD ? a.region_classes + int64_t(g) * kPdExtraClasses : nullptr);
     // Each read's second lane starts below DP row 0 when identity rows fill its first lane.
@@ -943,8 +967,8 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
               const T dy = k ? st.y[k - 1] : uy;
               const bool match = (hc == uint32_t('N')) | (hc == rw.rb[k]);
               const T jm = (match ? rw.pm[k] : rw.px[k]) *
-                           fma(dy, rw.im[k], fma(dx, rw.im[k], dm * rw.mm[k]));
-              const T jy = fma(st.y[k], rw.gc[k], st.m[k] * rw.md[k]);
+                           fma(dy, rw.im[rw.g(k)], fma(dx, rw.im[rw.g(k)], dm * rw.mm[rw.g(k)]));
+              const T jy = fma(st.y[k], rw.gc[k], st.m[k] * rw.md[rw.g(k)]);
               const double term = fma(double(jm), double(rec[k]), double(jy) * double(rec[R + k]));
               const int row = first + k + 1;  // DP row; rows above DP row 0 hold nothing
               if (lane < L && row >= 0) rows[base + row] = term;
@@ -1192,6 +1216,7 @@ using KernelD = void (*)(SweepArgs, DevTab<double>);
 // Every instantiation the host launches: [PD][BOOST][R-1] for FP32 (sweep_f32.hip), [PD][R-1]
 // for FP64 without suffix sharing and [R-1] for plain FP64 with it (sweep_f64.hip).
 extern const KernelF kSweepF32[2][2][kMaxRowsPerLane];
+extern const KernelF kSweepF32Uniform[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64Suffix[kMaxRowsPerLane];
 // long_kernel by [PD].

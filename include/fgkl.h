@@ -224,6 +224,14 @@ int fgkl_batch_work(const fgkl_batch* batch, int64_t* cells4);
  */
 int fgkl_batch_set_lds_records(fgkl_batch* batch, int enable);
 /*
+ * Whether a plain batch's FP32 main passes run the items whose reads have uniform gap qualities
+ * (each read's insertion, deletion and gap-continuation qualities the same at every base, as
+ * GATK passes them by default) with the kernel build that keeps one copy of the gap terms per
+ * lane (1, the default), or every item with the general build (0).  Results are the same bits
+ * either way.  No reference counterpart.
+ */
+int fgkl_batch_set_uniform_gaps(fgkl_batch* batch, int enable);
+/*
  * The batch's snapshot records per FP32 pass as fgkl_plan_snapshots reports them (out4[0..2])
  * and out4[3], the sweep launches so far (eager or captured into a graph) that kept a record in
  * each wave's LDS.

@@ -1281,3 +1281,53 @@ def test_snapshot_records_in_lds_give_the_bits_of_hbm_records(gpu, wl, regions):
     assert_close(runs[(True, F.MODE_FP32_RESCUE)], expected, FLOAT_TOL, f"workload {wl} fp32")
     assert_rel(runs[(True, F.MODE_FP64)], expected, what=f"workload {wl} fp64")
     b.close()
+
+
+def _perturbed_gaps(data, every: int, seed: int):
+    """A copy of `data` in which one base of every `every`-th read (of at least two bases) has
+    its insertion quality raised by one: those reads no longer have uniform gap qualities."""
+    import dataclasses
+
+    ins = data.ins.copy()
+    rng = np.random.default_rng(seed)
+    for r in range(0, len(data.read_offsets) - 1, every):
+        o0, o1 = int(data.read_offsets[r]), int(data.read_offsets[r + 1])
+        if o1 - o0 >= 2:
+            i = int(rng.integers(o0 + 1, o1))
+            ins[i] = min(int(ins[i]) + 1, 60)
+    return dataclasses.replace(data, ins=ins)
+
+
+@pytest.mark.parametrize("case", ["uniform", "mixed", "rescue"])
+def test_uniform_gap_build_gives_the_bits_of_the_general_build(gpu, case):
+    """Items whose reads have the same insertion, deletion and gap-continuation quality at every
+    base run with one copy of the gap terms per lane (the uniform-gap build); the others, and
+    items pairing such a read with one that varies, with the general build after them in the same
+    bucket.  Every run gives the general build's bits: rescued or not, with and without LDS
+    snapshot records, from a graph, and within the bounds of the oracle."""
+    if case == "rescue":  # underflowing reads: lost pairs, dense reads rerun in FP64
+        data = workload(2, seed=913, n_regions=2)
+    else:  # edited haplotype sets: prefix and suffix sharing
+        data = workload(1, seed=911, n_regions=6)
+        if case == "mixed":
+            data = _perturbed_gaps(data, 3, 5)
+    b = F.Batch(data)
+    runs = {}
+    for ug in (True, False):
+        b.set_uniform_gaps(ug)
+        for lds in (True, False):
+            b.set_lds_records(lds)
+            for mode in (F.MODE_FP32_NO_RESCUE, F.MODE_FP32_RESCUE):
+                b.run(mode, share=True)
+                runs[(ug, lds, mode)] = b.results()
+    for lds in (True, False):
+        for mode in (F.MODE_FP32_NO_RESCUE, F.MODE_FP32_RESCUE):
+            assert np.array_equal(runs[(True, lds, mode)], runs[(False, lds, mode)],
+                                  equal_nan=True), (case, lds, mode)
+    b.set_uniform_gaps(True)
+    b.set_lds_records(True)
+    for _ in range(2):
+        b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
+        assert np.array_equal(b.results(), runs[(True, True, F.MODE_FP32_RESCUE)])
+    assert_close(runs[(True, True, F.MODE_FP32_RESCUE)], batch_reference(data), FLOAT_TOL, case)
+    b.close()
