@@ -51,10 +51,10 @@ template <> struct Kernels<float> {
     return ug && !pd ? kSweepF32Uniform[boost][rows - 1] : kSweepF32[pd][boost][rows - 1];
   }
 };
-// FP64 has no occupancy-raised or uniform-gap build; its flag selects the suffix-sharing build
-// (plain only).
+// FP64 has no occupancy-raised build; its flag selects the suffix-sharing build (plain only).
 template <> struct Kernels<double> {
-  static KernelD get(bool pd, bool suffix, int rows, bool = false) {
+  static KernelD get(bool pd, bool suffix, int rows, bool ug = false) {
+    if (ug && !pd) return kSweepF64Uniform[suffix][rows - 1];
     return suffix && !pd ? kSweepF64Suffix[rows - 1] : kSweepF64[pd][rows - 1];
   }
 };
@@ -469,7 +469,7 @@ struct DeviceBatch::Impl {
     if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
     if (!std::is_same<T, float>::value) c.boost = a.suffix != 0;  // FP64: the suffix build
-    c.ug = c.ug && std::is_same<T, float>::value && !pd && !a.pair_mode;
+    c.ug = c.ug && !pd && !a.pair_mode;
     if (int rc = lds_layout<T>(a, c, rows)) return rc;
     int grid = 0;
     if (int rc = grid_waves<T>(c, rows, max_items, &grid, a.lds_words)) return rc;
@@ -498,7 +498,7 @@ struct DeviceBatch::Impl {
   template <typename T>
   int launch_bucket(int R, const SweepArgs& a, int64_t n, LaunchCfg c, hipStream_t on,
                     size_t scratch_off) {
-    const int64_t u = (std::is_same<T, float>::value && !pd && uniform_gaps) ? bucket_ug[R - 1] : 0;
+    const int64_t u = (!pd && uniform_gaps) ? bucket_ug[R - 1] : 0;
     if (u > 0) {
       SweepArgs au = a;
       au.n_items = int32_t(u);
@@ -670,7 +670,7 @@ int DeviceBatch::Impl::enqueue(int mode, int share, bool timed) {
       if (fp32) {
         if (int rc = launch_bucket<float>(R, a, bucket_n[R - 1], cfg[R - 1], on, off)) return rc;
       } else {
-        if (int rc = launch<double>(R, a, bucket_n[R - 1], LaunchCfg{}, on, off)) return rc;
+        if (int rc = launch_bucket<double>(R, a, bucket_n[R - 1], LaunchCfg{}, on, off)) return rc;
       }
       if (timed) HIP_TRY(hipEventRecord(bev[2 * (R - 1) + 1], on));
     }
@@ -968,8 +968,10 @@ int DeviceBatch::assign(const BatchInput& in) {
         c.ug = !d.pd;  // the uniform-gap build's grid
         n = std::max(n, d.scratch_need<float>(R, c));
       }
-      n = std::max(n, d.scratch_need<double>(R, LaunchCfg{}));
-      if (d.suffix_run(1)) n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, true}));
+      for (bool ug : {false, !d.pd}) {  // FP64 launches, general and uniform-gap builds
+        n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, false, ug}));
+        if (d.suffix_run(1)) n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, true, ug}));
+      }
       need = std::max(need, n);  // tuning runs every bucket on the batch's stream
       if (d.side[R - 1]) side_need = std::max(side_need, n);
     }

@@ -1219,6 +1219,7 @@ extern const KernelF kSweepF32[2][2][kMaxRowsPerLane];
 extern const KernelF kSweepF32Uniform[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64Suffix[kMaxRowsPerLane];
+extern const KernelD kSweepF64Uniform[2][kMaxRowsPerLane];
 // long_kernel by [PD].
 extern const KernelF kLongF32[2];
 extern const KernelD kLongF64[2];

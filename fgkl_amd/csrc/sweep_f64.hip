@@ -15,6 +15,17 @@ const KernelD kSweepF64[2][kMaxRowsPerLane] = {FGKL_F64_ROW(false, false),
                                                FGKL_F64_ROW(true, false)};
 const KernelD kSweepF64Suffix[kMaxRowsPerLane] = FGKL_F64_ROW(false, true);
 
+// The uniform-gap builds of the plain sweep (see Rows): without and with suffix sharing.
+#define FGKL_F64_UG_ROW(S)                                                                  \
+  {sweep_kernel<double, 1, false, false, S, true>, sweep_kernel<double, 2, false, false, S, true>, \
+   sweep_kernel<double, 3, false, false, S, true>, sweep_kernel<double, 4, false, false, S, true>, \
+   sweep_kernel<double, 5, false, false, S, true>, sweep_kernel<double, 6, false, false, S, true>, \
+   sweep_kernel<double, 7, false, false, S, true>, sweep_kernel<double, 8, false, false, S, true>}
+
+const KernelD kSweepF64Uniform[2][kMaxRowsPerLane] = {FGKL_F64_UG_ROW(false), FGKL_F64_UG_ROW(true)};
+
+#undef FGKL_F64_UG_ROW
+
 #undef FGKL_F64_ROW
 
 const KernelD kLongF64[2] = {long_kernel<double, false>, long_kernel<double, true>};

@@ -1303,8 +1303,9 @@ def test_uniform_gap_build_gives_the_bits_of_the_general_build(gpu, case):
     """Items whose reads have the same insertion, deletion and gap-continuation quality at every
     base run with one copy of the gap terms per lane (the uniform-gap build); the others, and
     items pairing such a read with one that varies, with the general build after them in the same
-    bucket.  Every run gives the general build's bits: rescued or not, with and without LDS
-    snapshot records, from a graph, and within the bounds of the oracle."""
+    bucket.  Every run gives the general build's bits: FP32 rescued or not and FP64 (prefix and
+    suffix sharing), with and without LDS snapshot records, from a graph, and within the bounds of
+    the oracle."""
     if case == "rescue":  # underflowing reads: lost pairs, dense reads rerun in FP64
         data = workload(2, seed=913, n_regions=2)
     else:  # edited haplotype sets: prefix and suffix sharing
@@ -1317,11 +1318,11 @@ def test_uniform_gap_build_gives_the_bits_of_the_general_build(gpu, case):
         b.set_uniform_gaps(ug)
         for lds in (True, False):
             b.set_lds_records(lds)
-            for mode in (F.MODE_FP32_NO_RESCUE, F.MODE_FP32_RESCUE):
+            for mode in (F.MODE_FP32_NO_RESCUE, F.MODE_FP32_RESCUE, F.MODE_FP64):
                 b.run(mode, share=True)
                 runs[(ug, lds, mode)] = b.results()
     for lds in (True, False):
-        for mode in (F.MODE_FP32_NO_RESCUE, F.MODE_FP32_RESCUE):
+        for mode in (F.MODE_FP32_NO_RESCUE, F.MODE_FP32_RESCUE, F.MODE_FP64):
             assert np.array_equal(runs[(True, lds, mode)], runs[(False, lds, mode)],
                                   equal_nan=True), (case, lds, mode)
     b.set_uniform_gaps(True)
@@ -1329,5 +1330,7 @@ def test_uniform_gap_build_gives_the_bits_of_the_general_build(gpu, case):
     for _ in range(2):
         b.run(F.MODE_FP32_RESCUE, share=True, graph=True)
         assert np.array_equal(b.results(), runs[(True, True, F.MODE_FP32_RESCUE)])
-    assert_close(runs[(True, True, F.MODE_FP32_RESCUE)], batch_reference(data), FLOAT_TOL, case)
+    expected = batch_reference(data)
+    assert_close(runs[(True, True, F.MODE_FP32_RESCUE)], expected, FLOAT_TOL, case)
+    assert_rel(runs[(True, True, F.MODE_FP64)], expected, what=f"{case} fp64")
     b.close()
