@@ -829,13 +829,14 @@ int DeviceBatch::assign(const BatchInput& in) {
   } else
     d.place(&d.hap_bytes, in.hap_bases, hb);
   // Reads whose insertion, deletion and gap-continuation qualities are each the same at every
-  // base (plain batches).
+  // base, with a gap-continuation quality of at least kXsMinGcp (plain batches): the reads the
+  // sweep kernel's own test (rows_match_first) lets scale their insertion values.
   std::vector<uint8_t> uniform;
   if (!in.pd && !in.pairs_only) {
     uniform.assign(size_t(NR), 0);
     for (int64_t r = 0; r < NR; ++r) {
       const int64_t o0 = in.read_off[r], o1 = in.read_off[r + 1];
-      bool u = o1 > o0;
+      bool u = o1 > o0 && in.gcp[o0] >= kXsMinGcp;
       for (int64_t i = o0 + 1; i < o1 && u; ++i)
         u = in.ins[i] == in.ins[o0] && in.del[i] == in.del[o0] && in.gcp[i] == in.gcp[o0];
       uniform[size_t(r)] = u;

@@ -153,6 +153,11 @@ constexpr int64_t kMaxPairList = int64_t(1) << 20;
 // R rows per lane over 64 lanes: reads of up to 64 * 8 - 1 = 511 bases run in the staged sweep.
 constexpr int kMaxRowsPerLane = 8;
 constexpr int kMaxReadLen = 64 * kMaxRowsPerLane - 1;
+// Plain reads whose insertion, deletion and gap-continuation qualities are each the same at every
+// base, with a gap-continuation quality of at least this (a continuation probability of at most
+// 10^-0.4 = 0.4), keep their insertion values divided by the match-to-insertion probability, which
+// saves a product per cell; such reads run with the uniform-gap build of the sweep.
+constexpr int kXsMinGcp = 4;
 // Haplotypes the staged sweep takes: its LDS holds the longest haplotype of a launch per wave,
 // and 2048 columns keep four waves of a 256-thread block within 34 KiB.
 constexpr int kMaxStagedHap = 2048;

@@ -135,9 +135,20 @@ __device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
 // the rows above them hold no match value for mi and md to scale, and the one row above that
 // does (read A's last row, above read B's first lane) reaches read B's identity row masked (see
 // shift_up).  gc stays per row: identity rows keep D->D and I->I at 1.
+//
+// Scaled insertion values (plain reads with uniform gaps and a gap continuation of at most 0.4,
+// `xs_read`): a lane keeps x / mi in place of each insertion value x, so the insertion update is
+// fma(x_up, gc, m_up) with no product (mi is the same in every row of such a read), the match
+// update scales the diagonal insertion value by ix = im * mi, and the last-row sum adds
+// fma(x, ma, m) with ma = mi.  Elsewhere ix = im and ma = 1, which give the unscaled values bit for
+// bit.  Both builds keep the same values for a read: the uniform-gap build always scales, and the
+// general build scales the rows of such reads by setting mi to 1 (`scale_insertions`), since v * 1
+// is exact.  x / mi stays below m_max / (1 - gc) <= 1.7 m_max, so it cannot overflow.
 template <typename T, int R, bool PD, bool UG = false> struct Rows {
   static constexpr int g(int k) { return UG ? 0 : k; }
   T pm[R], px[R], mm[UG ? 1 : R], im[UG ? 1 : R], mi[UG ? 1 : R], md[UG ? 1 : R], gc[R];
+  T ix[UG ? 1 : R];
+  T ma;
   uint32_t rb[R];
   uint32_t ralt[R];
   // PD, staged sweep: the row's read-base class, the bit of the column word that says whether
@@ -177,6 +188,10 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
     rw.mi[0] = live ? tab.ep[iq] : T(0);
     rw.md[0] = live ? tab.ep[dq] : T(0);
     rw.im[0] = live ? tab.im[gq] : T(0);
+    rw.ix[0] = rw.im[0] * rw.mi[0];  // every read here scales its insertion values
+    rw.ma = rw.mi[0];
+  } else {
+    rw.ma = T(1);
   }
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -193,6 +208,7 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
         rw.mi[k] = tab.ep[iq];
         rw.md[k] = tab.ep[dq];
         rw.im[k] = tab.im[gq];
+        rw.ix[k] = rw.im[k];
       }
       rw.gc[k] = tab.ep[gq];  // insertion and deletion extension alike (model.rs:73-75)
       rw.rb[k] = b;
@@ -204,13 +220,48 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
       // identity row is zero.  Lanes past the read: all zero.
       rw.pm[k] = T(0); rw.px[k] = T(0);
       if constexpr (!UG) {
-        rw.mm[k] = T(0); rw.im[k] = T(0); rw.mi[k] = T(0); rw.md[k] = T(0);
+        rw.mm[k] = T(0); rw.im[k] = T(0); rw.mi[k] = T(0); rw.md[k] = T(0); rw.ix[k] = T(0);
       }
       rw.gc[k] = lane < L ? T(1) : T(0);
       rw.rb[k] = 0x100u;
       rw.ralt[k] = 0u;
       if constexpr (PD) rw.ci[k] = 0u;
       rw.sh[k] = kIdentShift;
+    }
+  }
+}
+
+// Whether this lane's rows of a plain read carry the insertion, deletion and gap-continuation
+// quality bytes of the read's first base (true in lanes past the read).  A read scales its
+// insertion values (see Rows) when every lane of it agrees and its gap-continuation quality is
+// at least kXsMinGcp; the host applies the same test to pick the uniform-gap build.
+template <int R>
+__device__ __forceinline__ bool rows_match_first(const SweepArgs& a, int64_t o, int L, int pad,
+                                                 int lane) {
+  const uint32_t i0 = a.ins[o], d0 = a.del[o], g0 = a.gcp[o];
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int arow = R * lane + k - pad;
+    if (lane < L && arow >= 0) {
+      const int64_t i = o + arow;
+      ok = ok && a.ins[i] == i0 && a.del[i] == d0 && a.gcp[i] == g0;
+    }
+  }
+  return ok;
+}
+
+// General build: the rows of a read that scales its insertion values (`xs`) take mi = 1,
+// ix = im * mi and ma = mi, the values the uniform-gap build holds for it; identity rows and
+// lanes past the read keep their zeros.
+template <typename T, int R, bool PD>
+__device__ __forceinline__ void scale_insertions(Rows<T, R, PD, false>& rw, bool xs) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (xs && rw.sh[k] != kIdentShift) {
+      rw.ix[k] = rw.im[k] * rw.mi[k];
+      rw.ma = rw.mi[k];
+      rw.mi[k] = T(1);
     }
   }
 }
@@ -340,9 +391,9 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
       prior = match ? rw.pm[k] : rw.px[k];
     }
     if constexpr (!PD) {
-      // kernel.rs:1011-1013
+      // kernel.rs:1011-1013, the diagonal insertion value scaled by ix (see Rows)
       const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[rw.g(k)]);
-      s.m[k] = prior * fma(pdy, rw.im[rw.g(k)], fma(pdx, rw.im[rw.g(k)], pdm * rw.mm[rw.g(k)]));
+      s.m[k] = prior * fma(pdy, rw.im[rw.g(k)], fma(pdx, rw.ix[rw.g(k)], pdm * rw.mm[rw.g(k)]));
       s.y[k] = Y;
     } else {
       // pd_reference.rs:62-94 for the column's deletion state.
@@ -391,7 +442,10 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
     const T pum = k ? s.m[k - 1] : u.m;
     const T pux = k ? s.x[k - 1] : u.x;
     if constexpr (!PD) {
-      s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[rw.g(k)]);
+      if constexpr (UG)
+        s.x[k] = fma(pux, rw.gc[k], pum);  // scaled insertion values (see Rows)
+      else
+        s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[k]);
     } else {
       // pd_reference.rs:95-100
       const T pubm = k ? s.bm[k - 1] : u.bm;
@@ -408,7 +462,10 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
       }
     }
   }
-  s.acc += s.m[R - 1] + s.x[R - 1];
+  if constexpr (PD)
+    s.acc += s.m[R - 1] + s.x[R - 1];
+  else
+    s.acc += fma(s.x[R - 1], rw.ma, s.m[R - 1]);  // m + x, unscaling x (see Rows)
 }
 
 // Writes a lane's state into its snapshot record (value by value), in HBM or in LDS.
@@ -891,6 +948,18 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       }
     }
 
+    // General build, plain reads: the reads that scale their insertion values (see Rows) take
+    // the uniform-gap build's values, after the backward sweeps, which use the unscaled ones.
+    if constexpr (!PD && !UG) {
+      const bool ok = rows_match_first<R>(a, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
+                                          in_b ? lane - La : lane);
+      const uint64_t bad = __ballot(!ok);
+      const uint64_t lanes_a = La >= 64 ? ~0ull : (1ull << La) - 1;
+      const bool xs_a = !(bad & lanes_a) && a.gcp[oa] >= kXsMinGcp;
+      const bool xs_b = two && !(bad & ~lanes_a) && a.gcp[ob] >= kXsMinGcp;
+      scale_insertions(rw, in_b ? xs_b : xs_a);
+    }
+
     double prev_a = 0.0, prev_b = 0.0;
     int lost_a = 0, lost_b = 0;
     for (int k = k_begin; k < k_end; ++k) {
@@ -967,7 +1036,7 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
               const T dy = k ? st.y[k - 1] : uy;
               const bool match = (hc == uint32_t('N')) | (hc == rw.rb[k]);
               const T jm = (match ? rw.pm[k] : rw.px[k]) *
-                           fma(dy, rw.im[rw.g(k)], fma(dx, rw.im[rw.g(k)], dm * rw.mm[rw.g(k)]));
+                           fma(dy, rw.im[rw.g(k)], fma(dx, rw.ix[rw.g(k)], dm * rw.mm[rw.g(k)]));
               const T jy = fma(st.y[k], rw.gc[k], st.m[k] * rw.md[rw.g(k)]);
               const double term = fma(double(jm), double(rec[k]), double(jy) * double(rec[R + k]));
               const int row = first + k + 1;  // DP row; rows above DP row 0 hold nothing

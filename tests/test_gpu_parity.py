@@ -1334,3 +1334,67 @@ def test_uniform_gap_build_gives_the_bits_of_the_general_build(gpu, case):
     assert_close(runs[(True, True, F.MODE_FP32_RESCUE)], expected, FLOAT_TOL, case)
     assert_rel(runs[(True, True, F.MODE_FP64)], expected, what=f"{case} fp64")
     b.close()
+
+
+def _regapped(data, seed: int):
+    """A copy of `data` whose reads take uniform gap qualities from wide ranges (insertion and
+    deletion 5..90, gap continuation 1..40, so some reads fall below the gap-continuation quality
+    of 4 that scaled insertion values need), every fourth read with the gap-continuation quality of
+    one base changed, so its gaps vary."""
+    import dataclasses
+
+    ins, dels, gcp = data.ins.copy(), data.dels.copy(), data.gcp.copy()
+    rng = np.random.default_rng(seed)
+    for r in range(len(data.read_offsets) - 1):
+        o0, o1 = int(data.read_offsets[r]), int(data.read_offsets[r + 1])
+        ins[o0:o1] = rng.choice([5, 20, 45, 70, 90])
+        dels[o0:o1] = rng.choice([5, 20, 45, 70, 90])
+        gcp[o0:o1] = rng.choice([1, 2, 3, 4, 5, 10, 40])
+        if r % 4 == 3 and o1 - o0 >= 2:
+            i = int(rng.integers(o0 + 1, o1))
+            gcp[i] = gcp[i] + 1
+    return dataclasses.replace(data, ins=ins, dels=dels, gcp=gcp)
+
+
+def test_scaled_insertion_values_give_one_set_of_bits_on_every_path(gpu):
+    """Plain reads with uniform gaps and a gap-continuation quality of at least 4 keep their
+    insertion values divided by the match-to-insertion probability in both sweep builds.  Over wide
+    gap qualities (scaled, unscaled and varying reads, paired in waves), the uniform-gap and the
+    general build give the same bits in FP32 (rescued) and FP64 with prefix and suffix sharing; the
+    one-pair-per-wave runs and one-region batches give each region's bits alike; all lie within the
+    oracle's bounds."""
+    data = _regapped(workload(1, seed=1717, n_regions=6), 3)
+    expected = batch_reference(data)
+    b = F.Batch(data)
+    for mode in (F.MODE_FP32_RESCUE, F.MODE_FP64):
+        runs = {}
+        for ug in (True, False):
+            b.set_uniform_gaps(ug)
+            b.run(mode, share=True)
+            runs[ug] = b.results()
+        assert np.array_equal(runs[True], runs[False], equal_nan=True), mode
+        b.run_pairs(mode)
+        pairs = b.results()
+        if mode == F.MODE_FP64:
+            assert_rel(runs[True], expected, what="fp64 shared")
+            assert_rel(pairs, expected, what="fp64 pairs")
+        else:
+            assert_close(runs[True], expected, FLOAT_TOL, "fp32 shared")
+            assert_close(pairs, expected, FLOAT_TOL, "fp32 pairs")
+    b.set_uniform_gaps(True)
+    b.run_pairs(F.MODE_FP32_RESCUE)
+    whole = b.results()
+    base = data.out_base()
+    for g in range(data.n_regions):  # a region alone in a batch: the pair runs' bits
+        reads = [F.ReadRef(bytes(data.bases[o0:o1]), data.quals[o0:o1], data.ins[o0:o1],
+                           data.dels[o0:o1], data.gcp[o0:o1])
+                 for o0, o1 in ((int(data.read_offsets[r]), int(data.read_offsets[r + 1]))
+                                for r in range(int(data.region_reads[g]),
+                                               int(data.region_reads[g + 1])))]
+        haps = [bytes(data.hap_bases[int(data.hap_offsets[h]):int(data.hap_offsets[h + 1])])
+                for h in range(int(data.region_haps[g]), int(data.region_haps[g + 1]))]
+        one = F.Batch(batch_from_regions([(reads, haps)]))
+        one.run_pairs(F.MODE_FP32_RESCUE)
+        assert np.array_equal(one.results(), whole[base[g]:base[g + 1]]), g
+        one.close()
+    b.close()
