@@ -136,19 +136,25 @@ __device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
 // does (read A's last row, above read B's first lane) reaches read B's identity row masked (see
 // shift_up).  gc stays per row: identity rows keep D->D and I->I at 1.
 //
-// Scaled insertion values (plain reads with uniform gaps and a gap continuation of at most 0.4,
-// `xs_read`): a lane keeps x / mi in place of each insertion value x, so the insertion update is
-// fma(x_up, gc, m_up) with no product (mi is the same in every row of such a read), the match
-// update scales the diagonal insertion value by ix = im * mi, and the last-row sum adds
-// fma(x, ma, m) with ma = mi.  Elsewhere ix = im and ma = 1, which give the unscaled values bit for
-// bit.  Both builds keep the same values for a read: the uniform-gap build always scales, and the
-// general build scales the rows of such reads by setting mi to 1 (`scale_insertions`), since v * 1
-// is exact.  x / mi stays below m_max / (1 - gc) <= 1.7 m_max, so it cannot overflow.
+// Scaled gap values (plain reads with uniform gaps and a gap continuation of at most 0.4,
+// `xs`): a lane keeps x / mi in place of each insertion value x and y / md in place of each
+// deletion value y of DP rows 1..n, so the insertion and deletion updates are single FMAs,
+// fma(x_up, gc, m_up) and fma(y_left, gc, m_left), with no product (mi and md are the same in every
+// row of such a read).  The match update scales the diagonal insertion value by ix = im * mi and
+// the diagonal deletion value by im * md (held in im), and the last-row sum adds fma(x, ma, m) with
+// ma = mi.  DP row 0 keeps its deletion value C/m unscaled (C/m / md could overflow); DP row 1,
+// whose diagonal deletion value it is, divides its priors by md instead.  Elsewhere ix = im,
+// ma = 1 and the priors stay, which give the unscaled values bit for bit.  Both builds keep the
+// same values for a read: the uniform-gap build always scales, and the general build gives the
+// rows of such reads mi = md = 1 (`scale_gaps`), since v * 1 is exact.  x / mi and y / md stay
+// below m_max / (1 - gc) <= 1.7 m_max, so they cannot overflow; `yd` (md of a scaled read, else 1)
+// unscales deletion values where the suffix join needs them.  The backward sweeps of suffix
+// sharing run before the scaling, on the unscaled values.
 template <typename T, int R, bool PD, bool UG = false> struct Rows {
   static constexpr int g(int k) { return UG ? 0 : k; }
   T pm[R], px[R], mm[UG ? 1 : R], im[UG ? 1 : R], mi[UG ? 1 : R], md[UG ? 1 : R], gc[R];
   T ix[UG ? 1 : R];
-  T ma;
+  T ma, yd;
   uint32_t rb[R];
   uint32_t ralt[R];
   // PD, staged sweep: the row's read-base class, the bit of the column word that says whether
@@ -188,11 +194,10 @@ __device__ __forceinline__ void load_rows(const SweepArgs& a, const DevTab<T>& t
     rw.mi[0] = live ? tab.ep[iq] : T(0);
     rw.md[0] = live ? tab.ep[dq] : T(0);
     rw.im[0] = live ? tab.im[gq] : T(0);
-    rw.ix[0] = rw.im[0] * rw.mi[0];  // every read here scales its insertion values
-    rw.ma = rw.mi[0];
-  } else {
-    rw.ma = T(1);
+    rw.ix[0] = rw.im[0];
   }
+  rw.ma = T(1);
+  rw.yd = T(1);
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int arow = R * lane + k - pad;
@@ -251,17 +256,40 @@ __device__ __forceinline__ bool rows_match_first(const SweepArgs& a, int64_t o, 
   return ok;
 }
 
-// General build: the rows of a read that scales its insertion values (`xs`) take mi = 1,
-// ix = im * mi and ma = mi, the values the uniform-gap build holds for it; identity rows and
-// lanes past the read keep their zeros.
-template <typename T, int R, bool PD>
-__device__ __forceinline__ void scale_insertions(Rows<T, R, PD, false>& rw, bool xs) {
+// The rows of a read that scales its gap values (`xs`; always in the uniform-gap build) take
+// their scaled parameters (see Rows); identity rows and lanes past the read keep theirs.  Called
+// after the backward sweeps, which read the unscaled ones.
+template <typename T, int R, bool PD, bool UG>
+__device__ __forceinline__ void scale_gaps(Rows<T, R, PD, UG>& rw, bool xs) {
+  if constexpr (UG) {
+    const T md = rw.md[0];
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    if (xs && rw.sh[k] != kIdentShift) {
-      rw.ix[k] = rw.im[k] * rw.mi[k];
-      rw.ma = rw.mi[k];
-      rw.mi[k] = T(1);
+    for (int k = 0; k < R; ++k) {
+      if (rw.sh[k] == kRow1Shift) {
+        rw.pm[k] = rw.pm[k] / md;
+        rw.px[k] = rw.px[k] / md;
+      }
+    }
+    rw.ix[0] = rw.im[0] * rw.mi[0];
+    rw.im[0] = rw.im[0] * md;
+    rw.ma = rw.mi[0];
+    rw.yd = md;
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      if (xs && rw.sh[k] != kIdentShift) {
+        const T md = rw.md[k];
+        if (rw.sh[k] == kRow1Shift) {
+          rw.pm[k] = rw.pm[k] / md;
+          rw.px[k] = rw.px[k] / md;
+        }
+        rw.ix[k] = rw.im[k] * rw.mi[k];
+        rw.im[k] = rw.im[k] * md;
+        rw.ma = rw.mi[k];
+        rw.yd = md;
+        rw.mi[k] = T(1);
+        rw.md[k] = T(1);
+      }
     }
   }
 }
@@ -391,8 +419,12 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
       prior = match ? rw.pm[k] : rw.px[k];
     }
     if constexpr (!PD) {
-      // kernel.rs:1011-1013, the diagonal insertion value scaled by ix (see Rows)
-      const T Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[rw.g(k)]);
+      // kernel.rs:1011-1013 on scaled gap values (see Rows)
+      T Y;
+      if constexpr (UG)
+        Y = fma(s.y[k], rw.gc[k], s.m[k]);
+      else
+        Y = fma(s.y[k], rw.gc[k], s.m[k] * rw.md[k]);
       s.m[k] = prior * fma(pdy, rw.im[rw.g(k)], fma(pdx, rw.ix[rw.g(k)], pdm * rw.mm[rw.g(k)]));
       s.y[k] = Y;
     } else {
@@ -443,7 +475,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
     const T pux = k ? s.x[k - 1] : u.x;
     if constexpr (!PD) {
       if constexpr (UG)
-        s.x[k] = fma(pux, rw.gc[k], pum);  // scaled insertion values (see Rows)
+        s.x[k] = fma(pux, rw.gc[k], pum);  // scaled gap values (see Rows)
       else
         s.x[k] = fma(pux, rw.gc[k], pum * rw.mi[k]);
     } else {
@@ -948,16 +980,20 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       }
     }
 
-    // General build, plain reads: the reads that scale their insertion values (see Rows) take
-    // the uniform-gap build's values, after the backward sweeps, which use the unscaled ones.
-    if constexpr (!PD && !UG) {
-      const bool ok = rows_match_first<R>(a, in_b ? ob : oa, in_b ? Lb : La, in_b ? padb : pada,
-                                          in_b ? lane - La : lane);
-      const uint64_t bad = __ballot(!ok);
-      const uint64_t lanes_a = La >= 64 ? ~0ull : (1ull << La) - 1;
-      const bool xs_a = !(bad & lanes_a) && a.gcp[oa] >= kXsMinGcp;
-      const bool xs_b = two && !(bad & ~lanes_a) && a.gcp[ob] >= kXsMinGcp;
-      scale_insertions(rw, in_b ? xs_b : xs_a);
+    // Plain reads that scale their gap values (see Rows; every read of the uniform-gap build)
+    // take their scaled parameters, after the backward sweeps, which use the unscaled ones.
+    if constexpr (!PD) {
+      bool xs = true;
+      if constexpr (!UG) {
+        const bool ok = rows_match_first<R>(a, in_b ? ob : oa, in_b ? Lb : La,
+                                            in_b ? padb : pada, in_b ? lane - La : lane);
+        const uint64_t bad = __ballot(!ok);
+        const uint64_t lanes_a = La >= 64 ? ~0ull : (1ull << La) - 1;
+        const bool xs_a = !(bad & lanes_a) && a.gcp[oa] >= kXsMinGcp;
+        const bool xs_b = two && !(bad & ~lanes_a) && a.gcp[ob] >= kXsMinGcp;
+        xs = in_b ? xs_b : xs_a;
+      }
+      scale_gaps(rw, xs);
     }
 
     double prev_a = 0.0, prev_b = 0.0;
@@ -1037,8 +1073,13 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
               const bool match = (hc == uint32_t('N')) | (hc == rw.rb[k]);
               const T jm = (match ? rw.pm[k] : rw.px[k]) *
                            fma(dy, rw.im[rw.g(k)], fma(dx, rw.ix[rw.g(k)], dm * rw.mm[rw.g(k)]));
-              const T jy = fma(st.y[k], rw.gc[k], st.m[k] * rw.md[rw.g(k)]);
-              const double term = fma(double(jm), double(rec[k]), double(jy) * double(rec[R + k]));
+              // The deletion value entering the cut column, unscaled by yd in scaled rows (DP
+              // row 0 keeps C/m unscaled; see Rows).
+              const T jy = UG ? fma(st.y[k], rw.gc[k], st.m[k])
+                              : fma(st.y[k], rw.gc[k], st.m[k] * rw.md[rw.g(k)]);
+              const double yd = rw.sh[k] == kIdentShift ? 1.0 : double(rw.yd);
+              const double term =
+                  fma(double(jm), double(rec[k]), double(jy) * yd * double(rec[R + k]));
               const int row = first + k + 1;  // DP row; rows above DP row 0 hold nothing
               if (lane < L && row >= 0) rows[base + row] = term;
             }
