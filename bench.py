@@ -253,6 +253,7 @@ def side_bench(args, config: int) -> dict:
     data = workload(config, seed=args.seed)
     cells, n_pairs = data.cells(), data.n_pairs()
     b = F.Batch(data)
+    tune = None if args.no_tune else b.tune(share=True)  # per bucket, as for the headline
     st = b.run(F.MODE_FP32_RESCUE, share=True, graph=False)
     eager = b.results()
     primary_ms, rescue_ms, total_ms = st.primary_ms, st.rescue_ms, st.total_ms
@@ -284,6 +285,9 @@ def side_bench(args, config: int) -> dict:
                                  **vs_measured(achieved, "fp32_fma_tflops")},
         "parity": {"max_abs_dlog10L_vs_fp64": float(d.max()), "pairs_over_1e-4": int((d > 1e-4).sum()),
                    "nonfinite": int((~np.isfinite(eager)).sum())},
+        "launches": None if tune is None else ",".join(
+            f"{R}:{tune.rows[R - 1]}:{tune.block[R - 1]}:{int(tune.boost[R - 1])}"
+            for R in range(1, 9) if tune.rows[R - 1]),
     }
 
 

@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libfgkl.so"
-SOURCES = ["sweep_f32.hip", "sweep_f32_uniform.hip", "sweep_f64.hip", "engine.hip", "noalloc.hip", "peak.hip", "adjoint.hip", "sw.hip", "plan.cpp",
+SOURCES = ["sweep_f32.hip", "sweep_f32_uniform.hip", "sweep_f32_pd_uniform.hip", "sweep_f64.hip", "engine.hip", "noalloc.hip", "peak.hip", "adjoint.hip", "sw.hip", "plan.cpp",
            "synthetic.cpp", "replay.cpp", "aggregate.cpp", "capi.cpp", "jni_shim.cpp"]
 ARCH = os.environ.get("FGKL_OFFLOAD_ARCH", "gfx950")
 

@@ -48,7 +48,8 @@ namespace {
 template <typename T> struct Kernels;
 template <> struct Kernels<float> {
   static KernelF get(bool pd, bool boost, int rows, bool ug = false) {
-    return ug && !pd ? kSweepF32Uniform[boost][rows - 1] : kSweepF32[pd][boost][rows - 1];
+    if (ug) return pd ? kSweepF32PdUniform[boost][rows - 1] : kSweepF32Uniform[boost][rows - 1];
+    return kSweepF32[pd][boost][rows - 1];
   }
 };
 // FP64 has no occupancy-raised build; its flag selects the suffix-sharing build (plain only).
@@ -221,8 +222,9 @@ struct DeviceBatch::Impl {
   uint8_t* region_classes = nullptr;  // PD: read-base classes from 4 on, per region
   int32_t* bucket_items[kMaxRowsPerLane] = {};
   int32_t bucket_n[kMaxRowsPerLane] = {};
-  // Plain batches: each bucket's items whose reads all have uniform gap qualities come first,
-  // bucket_ug[R - 1] of them, and run with the uniform-gap build (launch_bucket).
+  // Each bucket's items whose reads all have uniform gap qualities (and a gap-continuation
+  // quality of at least kXsMinGcp) come first, bucket_ug[R - 1] of them, and run with the
+  // uniform-gap build (launch_bucket).
   int32_t bucket_ug[kMaxRowsPerLane] = {};
   bool uniform_gaps = true;
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
@@ -469,7 +471,7 @@ struct DeviceBatch::Impl {
     if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
     if (!std::is_same<T, float>::value) c.boost = a.suffix != 0;  // FP64: the suffix build
-    c.ug = c.ug && !pd && !a.pair_mode;
+    c.ug = c.ug && !a.pair_mode;  // FP64 PD has no uniform-gap build (Kernels<double>)
     if (int rc = lds_layout<T>(a, c, rows)) return rc;
     int grid = 0;
     if (int rc = grid_waves<T>(c, rows, max_items, &grid, a.lds_words)) return rc;
@@ -492,13 +494,13 @@ struct DeviceBatch::Impl {
     return kOk;
   }
 
-  // Bucket R's main-pass launch over items `a` (primary_args) with configuration `c`: in a plain
-  // FP32 pass, the items of reads with uniform gap qualities with the uniform-gap build and the
-  // others (work-queue counter 48 + R - 1) after them on the same stream; otherwise one launch.
+  // Bucket R's main-pass launch over items `a` (primary_args) with configuration `c`: the items
+  // of reads with uniform gap qualities with the uniform-gap build (FP64 PD: the general one)
+  // and the others (work-queue counter 48 + R - 1) after them on the same stream.
   template <typename T>
   int launch_bucket(int R, const SweepArgs& a, int64_t n, LaunchCfg c, hipStream_t on,
                     size_t scratch_off) {
-    const int64_t u = (!pd && uniform_gaps) ? bucket_ug[R - 1] : 0;
+    const int64_t u = uniform_gaps ? bucket_ug[R - 1] : 0;
     if (u > 0) {
       SweepArgs au = a;
       au.n_items = int32_t(u);
@@ -829,10 +831,10 @@ int DeviceBatch::assign(const BatchInput& in) {
   } else
     d.place(&d.hap_bytes, in.hap_bases, hb);
   // Reads whose insertion, deletion and gap-continuation qualities are each the same at every
-  // base, with a gap-continuation quality of at least kXsMinGcp (plain batches): the reads the
-  // sweep kernel's own test (rows_match_first) lets scale their insertion values.
+  // base, with a gap-continuation quality of at least kXsMinGcp: the reads the sweep kernel's own
+  // test (rows_match_first) lets scale their gap values.
   std::vector<uint8_t> uniform;
-  if (!in.pd && !in.pairs_only) {
+  if (!in.pairs_only) {
     uniform.assign(size_t(NR), 0);
     for (int64_t r = 0; r < NR; ++r) {
       const int64_t o0 = in.read_off[r], o1 = in.read_off[r + 1];
@@ -966,7 +968,7 @@ int DeviceBatch::assign(const BatchInput& in) {
       int64_t n = 0;
       for (LaunchCfg c : d.candidates(R, true)) {
         n = std::max(n, d.scratch_need<float>(R, c));
-        c.ug = !d.pd;  // the uniform-gap build's grid
+        c.ug = true;  // the uniform-gap build's grid
         n = std::max(n, d.scratch_need<float>(R, c));
       }
       for (bool ug : {false, !d.pd}) {  // FP64 launches, general and uniform-gap builds

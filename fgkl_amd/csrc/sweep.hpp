@@ -136,9 +136,10 @@ __device__ __forceinline__ uint32_t alt_bit(uint32_t b) {  // pd.rs:103-108
 // does (read A's last row, above read B's first lane) reaches read B's identity row masked (see
 // shift_up).  gc stays per row: identity rows keep D->D and I->I at 1.
 //
-// Scaled gap values (plain reads with uniform gaps and a gap continuation of at most 0.4,
-// `xs`): a lane keeps x / mi in place of each insertion value x and y / md in place of each
-// deletion value y of DP rows 1..n, so the insertion and deletion updates are single FMAs,
+// Scaled gap values (reads with uniform gaps and a gap continuation of at most 0.4, `xs`): a
+// lane keeps x / mi in place of each insertion value x and y / md in place of each deletion value
+// y of DP rows 1..n (PD: the insertion and deletion branch values alike, so the maxima of a value
+// and its branch value keep the scale), so the insertion and deletion updates are single FMAs,
 // fma(x_up, gc, m_up) and fma(y_left, gc, m_left), with no product (mi and md are the same in every
 // row of such a read).  The match update scales the diagonal insertion value by ix = im * mi and
 // the diagonal deletion value by im * md (held in im), and the last-row sum adds fma(x, ma, m) with
@@ -453,7 +454,11 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
         lxe = s.dx[k + 1];
         lye = s.dy[k + 1];
       }
-      const T Y = fma(lye, rw.gc[k], lme * rw.md[rw.g(k)]);
+      T Y;  // on scaled gap values (see Rows): the maxima keep the scale of their operands
+      if constexpr (UG)
+        Y = fma(lye, rw.gc[k], lme);
+      else
+        Y = fma(lye, rw.gc[k], lme * rw.md[k]);
       // NORMAL copies the row's values (lme is m there), AFTER_DEL takes the maxima, INSIDE_DEL
       // holds the branch.
       // Opaque to the optimiser: otherwise it distributes the next step's `branch & after` over
@@ -462,7 +467,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
       T BX = sel_bits(I, s.bx[k], lxe);
       T BY = sel_bits(I, s.by[k], lye);
       asm volatile("" : "+v"(BM), "+v"(BX), "+v"(BY));
-      s.m[k] = prior * fma(dye, rw.im[rw.g(k)], fma(dxe, rw.im[rw.g(k)], dme * rw.mm[rw.g(k)]));
+      s.m[k] = prior * fma(dye, rw.im[rw.g(k)], fma(dxe, rw.ix[rw.g(k)], dme * rw.mm[rw.g(k)]));
       s.y[k] = Y;
       s.bm[k] = BM;
       s.bx[k] = BX;
@@ -484,7 +489,10 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
       const T pubx = k ? s.bx[k - 1] : u.bx;
       const T ume = max_nonneg(pum, and_bits(pubm, de));
       const T uxe = max_nonneg(pux, and_bits(pubx, de));
-      s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[rw.g(k)]);
+      if constexpr (UG)
+        s.x[k] = fma(uxe, rw.gc[k], ume);
+      else
+        s.x[k] = fma(uxe, rw.gc[k], ume * rw.mi[k]);
       if constexpr (CARRY) {
         const T puy = k ? s.y[k - 1] : u.y;
         const T puby = k ? s.by[k - 1] : u.by;
@@ -494,10 +502,7 @@ __device__ __forceinline__ void cells(Lane<T, R, PD>& s, const Rows<T, R, PD, UG
       }
     }
   }
-  if constexpr (PD)
-    s.acc += s.m[R - 1] + s.x[R - 1];
-  else
-    s.acc += fma(s.x[R - 1], rw.ma, s.m[R - 1]);  // m + x, unscaling x (see Rows)
+  s.acc += fma(s.x[R - 1], rw.ma, s.m[R - 1]);  // m + x, unscaling x (see Rows)
 }
 
 // Writes a lane's state into its snapshot record (value by value), in HBM or in LDS.
@@ -980,9 +985,9 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
       }
     }
 
-    // Plain reads that scale their gap values (see Rows; every read of the uniform-gap build)
-    // take their scaled parameters, after the backward sweeps, which use the unscaled ones.
-    if constexpr (!PD) {
+    // Reads that scale their gap values (see Rows; every read of the uniform-gap build) take
+    // their scaled parameters, after the backward sweeps, which use the unscaled ones.
+    {
       bool xs = true;
       if constexpr (!UG) {
         const bool ok = rows_match_first<R>(a, in_b ? ob : oa, in_b ? Lb : La,
@@ -1324,9 +1329,11 @@ using KernelF = void (*)(SweepArgs, DevTab<float>);
 using KernelD = void (*)(SweepArgs, DevTab<double>);
 
 // Every instantiation the host launches: [PD][BOOST][R-1] for FP32 (sweep_f32.hip), [PD][R-1]
-// for FP64 without suffix sharing and [R-1] for plain FP64 with it (sweep_f64.hip).
+// for FP64 without suffix sharing and [R-1] for plain FP64 with it (sweep_f64.hip); the
+// uniform-gap builds by [BOOST][R-1] (FP32 plain and PD) and [SFX][R-1] (FP64 plain).
 extern const KernelF kSweepF32[2][2][kMaxRowsPerLane];
 extern const KernelF kSweepF32Uniform[2][kMaxRowsPerLane];
+extern const KernelF kSweepF32PdUniform[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64Suffix[kMaxRowsPerLane];
 extern const KernelD kSweepF64Uniform[2][kMaxRowsPerLane];

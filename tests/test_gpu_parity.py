@@ -1356,14 +1356,16 @@ def _regapped(data, seed: int):
     return dataclasses.replace(data, ins=ins, dels=dels, gcp=gcp)
 
 
-def test_scaled_insertion_values_give_one_set_of_bits_on_every_path(gpu):
-    """Plain reads with uniform gaps and a gap-continuation quality of at least 4 keep their
-    insertion values divided by the match-to-insertion probability in both sweep builds.  Over wide
-    gap qualities (scaled, unscaled and varying reads, paired in waves), the uniform-gap and the
-    general build give the same bits in FP32 (rescued) and FP64 with prefix and suffix sharing; the
-    one-pair-per-wave runs and one-region batches give each region's bits alike; all lie within the
-    oracle's bounds."""
-    data = _regapped(workload(1, seed=1717, n_regions=6), 3)
+@pytest.mark.parametrize("pd", [False, True])
+def test_scaled_gap_values_give_one_set_of_bits_on_every_path(gpu, pd):
+    """Reads with uniform gaps and a gap-continuation quality of at least 4 keep their insertion
+    and deletion values divided by the match-to-insertion and match-to-deletion probabilities in
+    both sweep builds (PD: the branch values alike).  Over wide gap qualities (scaled, unscaled and
+    varying reads, paired in waves), the uniform-gap and the general build give the same bits in
+    FP32 (rescued) and FP64 with prefix sharing (plain: and suffix sharing); the one-pair-per-wave
+    runs and one-region batches give each region's bits alike; all lie within the oracle's
+    bounds."""
+    data = _regapped(workload(3 if pd else 1, seed=1717, n_regions=6), 3)
     expected = batch_reference(data)
     b = F.Batch(data)
     for mode in (F.MODE_FP32_RESCUE, F.MODE_FP64):
@@ -1386,14 +1388,8 @@ def test_scaled_insertion_values_give_one_set_of_bits_on_every_path(gpu):
     whole = b.results()
     base = data.out_base()
     for g in range(data.n_regions):  # a region alone in a batch: the pair runs' bits
-        reads = [F.ReadRef(bytes(data.bases[o0:o1]), data.quals[o0:o1], data.ins[o0:o1],
-                           data.dels[o0:o1], data.gcp[o0:o1])
-                 for o0, o1 in ((int(data.read_offsets[r]), int(data.read_offsets[r + 1]))
-                                for r in range(int(data.region_reads[g]),
-                                               int(data.region_reads[g + 1])))]
-        haps = [bytes(data.hap_bases[int(data.hap_offsets[h]):int(data.hap_offsets[h + 1])])
-                for h in range(int(data.region_haps[g]), int(data.region_haps[g + 1]))]
-        one = F.Batch(batch_from_regions([(reads, haps)]))
+        haps = data.region_pd_haps_list(g) if pd else data.region_haps_list(g)
+        one = F.Batch(batch_from_regions([(data.region_reads_list(g), haps)], pd=pd))
         one.run_pairs(F.MODE_FP32_RESCUE)
         assert np.array_equal(one.results(), whole[base[g]:base[g + 1]]), g
         one.close()
