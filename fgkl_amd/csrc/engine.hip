@@ -55,7 +55,7 @@ template <> struct Kernels<float> {
 // FP64 has no occupancy-raised build; its flag selects the suffix-sharing build (plain only).
 template <> struct Kernels<double> {
   static KernelD get(bool pd, bool suffix, int rows, bool ug = false) {
-    if (ug && !pd) return kSweepF64Uniform[suffix][rows - 1];
+    if (ug) return pd ? kSweepF64PdUniform[rows - 1] : kSweepF64Uniform[suffix][rows - 1];
     return suffix && !pd ? kSweepF64Suffix[rows - 1] : kSweepF64[pd][rows - 1];
   }
 };
@@ -227,6 +227,7 @@ struct DeviceBatch::Impl {
   // uniform-gap build (launch_bucket).
   int32_t bucket_ug[kMaxRowsPerLane] = {};
   bool uniform_gaps = true;
+  bool all_uniform = false;  // every read of the batch takes the uniform-gap build
   int64_t bucket_pairs[kMaxRowsPerLane] = {};
   int64_t bucket_cells[kMaxRowsPerLane] = {};
   int64_t bucket_work[kMaxRowsPerLane] = {};
@@ -471,7 +472,9 @@ struct DeviceBatch::Impl {
     if (!on) on = stream;
     const int rows = c.rows ? c.rows : R;
     if (!std::is_same<T, float>::value) c.boost = a.suffix != 0;  // FP64: the suffix build
-    c.ug = c.ug && !a.pair_mode;  // FP64 PD has no uniform-gap build (Kernels<double>)
+    // Pair runs, rescues and the FP64 dense pass mix the batch's reads in one item list: they
+    // take the uniform-gap build when every read may.
+    c.ug = uniform_gaps && (all_uniform || (c.ug && !a.pair_mode));
     if (int rc = lds_layout<T>(a, c, rows)) return rc;
     int grid = 0;
     if (int rc = grid_waves<T>(c, rows, max_items, &grid, a.lds_words)) return rc;
@@ -495,8 +498,8 @@ struct DeviceBatch::Impl {
   }
 
   // Bucket R's main-pass launch over items `a` (primary_args) with configuration `c`: the items
-  // of reads with uniform gap qualities with the uniform-gap build (FP64 PD: the general one)
-  // and the others (work-queue counter 48 + R - 1) after them on the same stream.
+  // of reads with uniform gap qualities with the uniform-gap build and the others (work-queue
+  // counter 48 + R - 1) after them on the same stream.
   template <typename T>
   int launch_bucket(int R, const SweepArgs& a, int64_t n, LaunchCfg c, hipStream_t on,
                     size_t scratch_off) {
@@ -833,17 +836,19 @@ int DeviceBatch::assign(const BatchInput& in) {
   // Reads whose insertion, deletion and gap-continuation qualities are each the same at every
   // base, with a gap-continuation quality of at least kXsMinGcp: the reads the sweep kernel's own
   // test (rows_match_first) lets scale their gap values.
-  std::vector<uint8_t> uniform;
-  if (!in.pairs_only) {
-    uniform.assign(size_t(NR), 0);
-    for (int64_t r = 0; r < NR; ++r) {
-      const int64_t o0 = in.read_off[r], o1 = in.read_off[r + 1];
-      bool u = o1 > o0 && in.gcp[o0] >= kXsMinGcp;
-      for (int64_t i = o0 + 1; i < o1 && u; ++i)
-        u = in.ins[i] == in.ins[o0] && in.del[i] == in.del[o0] && in.gcp[i] == in.gcp[o0];
-      uniform[size_t(r)] = u;
-    }
+  // When every read is such a read (`all_uniform`), every launch of the batch, pair runs and
+  // rescues included, may take the uniform-gap build.
+  std::vector<uint8_t> uniform(size_t(NR), 0);
+  d.all_uniform = true;
+  for (int64_t r = 0; r < NR; ++r) {
+    const int64_t o0 = in.read_off[r], o1 = in.read_off[r + 1];
+    bool u = o1 > o0 && in.gcp[o0] >= kXsMinGcp;
+    for (int64_t i = o0 + 1; i < o1 && u; ++i)
+      u = in.ins[i] == in.ins[o0] && in.del[i] == in.del[o0] && in.gcp[i] == in.gcp[o0];
+    uniform[size_t(r)] = u;
+    d.all_uniform = d.all_uniform && u;
   }
+  if (in.pairs_only) uniform.clear();
   for (int R = 1; R <= kMaxRowsPerLane; ++R) {
     auto& items = d.plan.bucket_items[R - 1];
     d.bucket_n[R - 1] = int32_t(items.size() / 2);
@@ -963,7 +968,9 @@ int DeviceBatch::assign(const BatchInput& in) {
     for (int R = 1; R <= kMaxRowsPerLane; ++R) {
       // The dense FP64 rescue of the reads with R rows per lane.
       if (d.rows_reads[R - 1])
-        need = std::max(need, d.scratch_need<double>(R, LaunchCfg{}, d.rows_reads[R - 1]));
+        for (bool ug : {false, true})  // the general and the uniform-gap build's grids
+          need = std::max(need, d.scratch_need<double>(R, LaunchCfg{0, 256, false, ug},
+                                                       d.rows_reads[R - 1]));
       if (!d.bucket_n[R - 1]) continue;
       int64_t n = 0;
       for (LaunchCfg c : d.candidates(R, true)) {
@@ -971,7 +978,7 @@ int DeviceBatch::assign(const BatchInput& in) {
         c.ug = true;  // the uniform-gap build's grid
         n = std::max(n, d.scratch_need<float>(R, c));
       }
-      for (bool ug : {false, !d.pd}) {  // FP64 launches, general and uniform-gap builds
+      for (bool ug : {false, true}) {  // FP64 launches, general and uniform-gap builds
         n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, false, ug}));
         if (d.suffix_run(1)) n = std::max(n, d.scratch_need<double>(R, LaunchCfg{0, 256, true, ug}));
       }

@@ -1330,10 +1330,12 @@ using KernelD = void (*)(SweepArgs, DevTab<double>);
 
 // Every instantiation the host launches: [PD][BOOST][R-1] for FP32 (sweep_f32.hip), [PD][R-1]
 // for FP64 without suffix sharing and [R-1] for plain FP64 with it (sweep_f64.hip); the
-// uniform-gap builds by [BOOST][R-1] (FP32 plain and PD) and [SFX][R-1] (FP64 plain).
+// uniform-gap builds by [BOOST][R-1] (FP32 plain and PD), [SFX][R-1] (FP64 plain) and [R-1]
+// (FP64 PD).
 extern const KernelF kSweepF32[2][2][kMaxRowsPerLane];
 extern const KernelF kSweepF32Uniform[2][kMaxRowsPerLane];
 extern const KernelF kSweepF32PdUniform[2][kMaxRowsPerLane];
+extern const KernelD kSweepF64PdUniform[kMaxRowsPerLane];
 extern const KernelD kSweepF64[2][kMaxRowsPerLane];
 extern const KernelD kSweepF64Suffix[kMaxRowsPerLane];
 extern const KernelD kSweepF64Uniform[2][kMaxRowsPerLane];

@@ -24,6 +24,13 @@ const KernelD kSweepF64Suffix[kMaxRowsPerLane] = FGKL_F64_ROW(false, true);
 
 const KernelD kSweepF64Uniform[2][kMaxRowsPerLane] = {FGKL_F64_UG_ROW(false), FGKL_F64_UG_ROW(true)};
 
+// The uniform-gap build of the PD sweep.
+const KernelD kSweepF64PdUniform[kMaxRowsPerLane] = {
+    sweep_kernel<double, 1, true, false, false, true>, sweep_kernel<double, 2, true, false, false, true>,
+    sweep_kernel<double, 3, true, false, false, true>, sweep_kernel<double, 4, true, false, false, true>,
+    sweep_kernel<double, 5, true, false, false, true>, sweep_kernel<double, 6, true, false, false, true>,
+    sweep_kernel<double, 7, true, false, false, true>, sweep_kernel<double, 8, true, false, false, true>};
+
 #undef FGKL_F64_UG_ROW
 
 #undef FGKL_F64_ROW

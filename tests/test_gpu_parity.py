@@ -1362,21 +1362,24 @@ def test_scaled_gap_values_give_one_set_of_bits_on_every_path(gpu, pd):
     and deletion values divided by the match-to-insertion and match-to-deletion probabilities in
     both sweep builds (PD: the branch values alike).  Over wide gap qualities (scaled, unscaled and
     varying reads, paired in waves), the uniform-gap and the general build give the same bits in
-    FP32 (rescued) and FP64 with prefix sharing (plain: and suffix sharing); the one-pair-per-wave
-    runs and one-region batches give each region's bits alike; all lie within the oracle's
-    bounds."""
+    FP32 (rescued) and FP64 with prefix sharing (plain: and suffix sharing) and in the
+    one-pair-per-wave runs; a region alone in a batch, all of whose reads may take the uniform-gap
+    build in every launch, gets its bits in the whole batch's pair runs; all lie within the
+    oracle's bounds."""
     data = _regapped(workload(3 if pd else 1, seed=1717, n_regions=6), 3)
     expected = batch_reference(data)
     b = F.Batch(data)
     for mode in (F.MODE_FP32_RESCUE, F.MODE_FP64):
-        runs = {}
+        runs, pair_runs = {}, {}
         for ug in (True, False):
             b.set_uniform_gaps(ug)
             b.run(mode, share=True)
             runs[ug] = b.results()
+            b.run_pairs(mode)
+            pair_runs[ug] = b.results()
         assert np.array_equal(runs[True], runs[False], equal_nan=True), mode
-        b.run_pairs(mode)
-        pairs = b.results()
+        assert np.array_equal(pair_runs[True], pair_runs[False], equal_nan=True), mode
+        pairs = pair_runs[True]
         if mode == F.MODE_FP64:
             assert_rel(runs[True], expected, what="fp64 shared")
             assert_rel(pairs, expected, what="fp64 pairs")
