@@ -1069,6 +1069,9 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
             double* chunks = rows + 64 * R;  // after the 64 x R rows a wave holds
             const int base = in_b ? na + 1 : 0;
             const int first = R * (in_b ? lane - La : lane) - (in_b ? padb : pada);  // row - 1
+            // Opaque per join, so the optimiser keeps no per-row factors live across the sweeps.
+            T yd = rw.yd;
+            asm volatile("" : "+v"(yd));
             wave_sync_lds();
 #pragma unroll
             for (int k = 0; k < R; ++k) {
@@ -1082,9 +1085,9 @@ __global__ void __launch_bounds__(256, BOOST ? boost_waves(R, PD) : 1)
               // row 0 keeps C/m unscaled; see Rows).
               const T jy = UG ? fma(st.y[k], rw.gc[k], st.m[k])
                               : fma(st.y[k], rw.gc[k], st.m[k] * rw.md[rw.g(k)]);
-              const double yd = rw.sh[k] == kIdentShift ? 1.0 : double(rw.yd);
+              const double f = rw.sh[k] == kIdentShift ? 1.0 : double(yd);
               const double term =
-                  fma(double(jm), double(rec[k]), double(jy) * yd * double(rec[R + k]));
+                  fma(double(jm), double(rec[k]), double(jy) * f * double(rec[R + k]));
               const int row = first + k + 1;  // DP row; rows above DP row 0 hold nothing
               if (lane < L && row >= 0) rows[base + row] = term;
             }
